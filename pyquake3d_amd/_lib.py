@@ -1,0 +1,95 @@
+"""ctypes binding of the HIP C-ABI library ``libpq3d.so``.
+
+There is no fallback: if the library is missing, or a call returns a nonzero
+status, a ``RuntimeError`` carrying ``pq3d_last_error()`` is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_bool, c_double, c_float, c_int, c_int64, c_size_t, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libpq3d.so"
+
+_D = POINTER(c_double)
+_F = POINTER(c_float)
+_L = POINTER(c_int64)
+_I = POINTER(c_int)
+
+# name -> (restype, argtypes). Kept in the same order as include/pyquake3d_amd.h.
+SIGNATURES = {
+    "pq3d_last_error": (ctypes.c_char_p, []),
+    "pq3d_version": (c_int, []),
+    # reference drop-in symbols (src/TDstressFS_C.cpp:2048, :2098)
+    "TDstressFS_C": (None, [_D, _D, _D, c_size_t, _D, _D, _D, c_double, c_double, c_double,
+                            c_double, c_double, c_bool, _D, _D]),
+    "TDstressEachSourceAtReceiver_C": (None, [c_double, c_double, c_double, _D, _D, _D, c_size_t,
+                                              c_double, c_double, c_double, c_double, c_double,
+                                              c_bool, _D, _D]),
+    # device-pointer TDE kernels
+    "pq3d_tde_source_records": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "pq3d_tde_stress_points": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_double,
+                                       c_double, c_double, c_double, c_double, c_void_p, c_void_p,
+                                       c_void_p]),
+    "pq3d_tde_stress_sources": (c_int, [c_double, c_double, c_double, c_void_p, c_int64, c_double,
+                                        c_double, c_double, c_double, c_double, c_void_p,
+                                        c_void_p, c_void_p]),
+    "pq3d_tde_traction_block": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p,
+                                        c_int64, c_int64, c_double, c_double, c_void_p, c_int64,
+                                        c_int64, c_int64, c_int64, c_void_p]),
+    "pq3d_tde_traction_block_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                                            c_void_p, c_int64, c_int64, c_double, c_double,
+                                            c_void_p, c_int64, c_int64, c_int64, c_int64,
+                                            c_void_p]),
+}
+
+_lib = None
+
+
+def load(path: Path | str | None = None) -> ctypes.CDLL:
+    """Load libpq3d.so once and bind every declared symbol."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"HIP library {p} is not built; run `python -m pyquake3d_amd.build` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def exported(lib: ctypes.CDLL) -> dict:
+    return {name: hasattr(lib, name) for name in SIGNATURES}
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = load().pq3d_last_error()
+        raise RuntimeError(f"{what or 'pyquake3d_amd'}: status {rc}: "
+                           f"{msg.decode() if msg else ''}")
+
+
+def call(name: str, *args) -> None:
+    fn = getattr(load(), name)
+    check(fn(*args), name)
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a torch tensor, or 0 for None."""
+    return 0 if t is None else int(t.data_ptr())
+
+
+def stream_handle(stream=None) -> int:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)

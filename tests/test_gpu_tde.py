@@ -1,0 +1,120 @@
+"""GPU parity of the FP64 TDE kernels against the reference golden vectors and
+the CPU oracle. Tolerance: per point, max_k |d sigma_k| <= 1e-10 max_k |sigma_k|
+(FP64), identical non-finite pattern."""
+import numpy as np
+import pytest
+import torch
+
+from pyquake3d_amd import tde
+from pyquake3d_amd.mesh import get_eleVec, undulating_fault, planar_fault
+
+pytestmark = pytest.mark.gpu
+MU, LAM = 32e9, 32e9
+TOL = 1e-10
+
+
+def rel_err(a, b):
+    den = np.maximum(np.abs(b).max(axis=-1), 1e-300)
+    return np.abs(a - b).max(axis=-1) / den
+
+
+def test_tdstressfs_c_matches_reference_golden(gpu, pq3d_lib, golden):
+    g = golden["g1_points"]
+    tri, pts, bv, ml = g["tri"], g["pts"], g["bv"], g["ml"]
+    key = np.concatenate([tri, bv, ml], axis=1)
+    starts = np.r_[0, np.where((np.diff(key, axis=0) != 0).any(1))[0] + 1, len(key)]
+    S = np.zeros_like(g["stress"])
+    E = np.zeros_like(g["strain"])
+    for a, b in zip(starts[:-1], starts[1:]):
+        s, e = tde.TDstressFS_C(pts[a:b, 0], pts[a:b, 1], pts[a:b, 2], tri[a, 0:3], tri[a, 3:6],
+                                tri[a, 6:9], *bv[a], *ml[a])
+        S[a:b], E[a:b] = s, e
+    R = g["stress"]
+    fin = np.isfinite(R).all(1)
+    assert (np.isfinite(S).all(1) == fin).all()
+    assert rel_err(S[fin], R[fin]).max() < TOL
+    assert rel_err(E[fin], g["strain"][fin]).max() < TOL
+    zero = fin & (np.abs(R).max(1) == 0)
+    assert zero.sum() > 0 and (S[zero] == 0).all()
+
+
+def test_horizontal_triangle_nonfinite(gpu, pq3d_lib, golden):
+    g = golden["g1_points"]
+    P = g["h_tri"][0]
+    hp = g["h_pts"]
+    s, _ = tde.TDstressFS_C(hp[:, 0], hp[:, 1], hp[:, 2], P[0:3], P[3:6], P[6:9], 1.0, 0.0, 0.0,
+                            MU, LAM)
+    assert not np.isfinite(s).any()
+
+
+def test_each_source_matches_reference_golden(gpu, pq3d_lib, golden):
+    g = golden["g2_sources"]
+    t = g["tri"]
+    for r, (x, y, z) in enumerate(g["recv"]):
+        for b, bvec in enumerate(g["bvs"]):
+            s, e = tde.TDstressEachSourceAtReceiver_C(x, y, z, t[:, 0:3], t[:, 3:6], t[:, 6:9],
+                                                      *bvec, MU, LAM)
+            assert rel_err(s, g["stress"][r, b]).max() < TOL
+            assert rel_err(e, g["strain"][r, b]).max() < TOL
+
+
+def test_empty_inputs_and_halfspace_flag(gpu, pq3d_lib):
+    s, e = tde.TDstressFS_C([], [], [], [0, 0, 0], [1, 0, 0], [0, 0, 1], 1, 0, 0, MU, LAM)
+    assert s.shape == (0, 6)
+    with pytest.raises(RuntimeError, match="half-space"):
+        tde.TDstressFS_C([1.0], [2.0], [-3.0], [0, 0, -1], [1, 0, -1], [0, 0, -2], 1, 0, 0, MU,
+                         LAM, jud_halfspace=True)
+
+
+@pytest.mark.parametrize("tag", ["planar", "undul"])
+def test_traction_block_matches_reference_golden(gpu, pq3d_lib, golden, tag):
+    g = golden["g3_blocks"]
+    dm = tde.DeviceMesh(g[f"{tag}_nodes"], g[f"{tag}_elelst"], MU, LAM, device=gpu)
+    K = g[f"{tag}_K"]
+    out = dm.traction_block().cpu().numpy()
+    scale = np.abs(K).max(axis=(0, 2))[None, :, None]
+    assert (np.abs(out - K) / scale).max() < TOL
+
+
+def test_traction_block_gather_and_layouts(gpu, pq3d_lib, oracle):
+    mesh = undulating_fault(length=20e3, depth=8e3, nx=20, nz=8, seed=5)
+    dm = tde.DeviceMesh.from_mesh(mesh, MU, LAM, device=gpu)
+    rng = np.random.default_rng(0)
+    rows = rng.permutation(mesh.n)[:77]
+    cols = rng.permutation(mesh.n)[:131]
+    K = oracle.traction_block(dm.xg, dm.eleVec, mesh.triangles(), rows, cols, MU, LAM)
+    got = dm.traction_block(rows, cols).cpu().numpy()
+    scale = np.abs(K).max(axis=(0, 2))[None, :, None]
+    assert (np.abs(got - K) / scale).max() < TOL
+    # transposed storage and contiguous ranges
+    gt = dm.traction_block(None, None, row0=10, m=50, col0=100, n=60, order="ji").cpu().numpy()
+    K2 = oracle.traction_block(dm.xg, dm.eleVec, mesh.triangles(), np.arange(10, 60),
+                               np.arange(100, 160), MU, LAM)
+    assert np.abs(gt.transpose(0, 2, 1) - K2).max() <= TOL * np.abs(K2).max()
+    # FP32 output is the FP64 result rounded once
+    g32 = dm.traction_block(rows, cols, dtype=torch.float32).cpu().numpy()
+    assert np.array_equal(g32, got.astype(np.float32))
+
+
+def test_traction_block_full_assembly_mid_size(gpu, pq3d_lib, oracle):
+    """Every pair of a 1,600-element undulating mesh (2.56 M pairs)."""
+    mesh = undulating_fault(length=40e3, depth=10e3, nx=40, nz=20, seed=2)
+    dm = tde.DeviceMesh.from_mesh(mesh, MU, LAM, device=gpu)
+    got = dm.traction_block().cpu().numpy()
+    K = oracle.traction_block(dm.xg, dm.eleVec, mesh.triangles(), np.arange(mesh.n),
+                              np.arange(mesh.n), MU, LAM)
+    scale = np.abs(K).max(axis=(0, 2))[None, :, None]
+    assert np.isfinite(got).all()
+    assert (np.abs(got - K) / scale).max() < TOL
+
+
+def test_invalid_ranges_raise(gpu, pq3d_lib):
+    mesh = planar_fault(length=4e3, depth=2e3, nx=4, nz=2)
+    dm = tde.DeviceMesh.from_mesh(mesh, MU, LAM, device=gpu)
+    with pytest.raises(ValueError):
+        dm.traction_block(None, None, row0=5, m=100)
+    with pytest.raises(ValueError):
+        dm.traction_block([0, 99], [0])
+    out = torch.empty((6, 2, 2), dtype=torch.float64, device=gpu)
+    with pytest.raises(ValueError):
+        dm.traction_block([0, 1, 2], [0, 1], out=out)
