@@ -60,9 +60,11 @@ void TDstressEachSourceAtReceiver_C(double x, double y, double z, const double* 
 
 /* ---------------------------------------------- device TDE kernels */
 
-/* Per-triangle setup shared by all receivers: d_tri (n x 9) -> d_rec (n x 32
- * doubles). Hoists src/TDstressFS_C.cpp:292-395 out of the pair loop. */
-int pq3d_tde_source_records(const double* d_tri, int64_t n, double* d_rec, void* stream);
+/* Per-triangle setup shared by all receivers, computed on the HOST with the C
+ * library's acos/cos/sin (so that it is bitwise what the reference computes
+ * per call, src/TDstressFS_C.cpp:292-395): tri (n x 9, host) -> rec (n x 32
+ * doubles, host). Upload rec to the device for the calls below. */
+int pq3d_tde_source_records(const double* tri, int64_t n, double* rec);
 
 /* Device form of TDstressFS_C: n receivers x one source record d_rec1. */
 int pq3d_tde_stress_points(const double* d_X, const double* d_Y, const double* d_Z, int64_t n,

@@ -27,7 +27,7 @@ SIGNATURES = {
                                               c_double, c_double, c_double, c_double, c_double,
                                               c_bool, _D, _D]),
     # device-pointer TDE kernels
-    "pq3d_tde_source_records": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "pq3d_tde_source_records": (c_int, [_D, c_int64, _D]),
     "pq3d_tde_stress_points": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_double,
                                        c_double, c_double, c_double, c_double, c_void_p, c_void_p,
                                        c_void_p]),

@@ -1,6 +1,6 @@
 // FP64 triangular-dislocation kernels and their C-ABI.
 //
-//  * pq3d_tde_source_records   per-triangle setup, one thread per triangle
+//  * pq3d_tde_source_records   per-triangle setup on the host (C library trig)
 //  * pq3d_tde_stress_points    n receivers x 1 source (device pointers)
 //  * pq3d_tde_stress_sources   1 receiver x n sources (device pointers)
 //  * pq3d_tde_traction_block   m receivers x n sources, unit strike and unit dip
@@ -34,18 +34,6 @@ void set_error(const char* fmt, ...) {
 void clear_error() { g_err[0] = 0; }
 
 // ---------------------------------------------------------------- kernels
-
-__global__ void __launch_bounds__(256) k_source_records(const double* __restrict__ tri, int64_t n,
-                                                        double* __restrict__ rec) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double* t = tri + 9 * i;
-  double r[kSrcRec];
-  tde_source_record(t, t + 3, t + 6, r);
-  double* o = rec + (int64_t)kSrcRec * i;
-#pragma unroll
-  for (int k = 0; k < kSrcRec; ++k) o[k] = r[k];
-}
 
 __device__ inline void load_rec(const double* __restrict__ src, double* r) {
 #pragma unroll
@@ -168,13 +156,13 @@ const char* pq3d_last_error(void) { return pq3d::g_err; }
 
 int pq3d_version(void) { return 1; }
 
-int pq3d_tde_source_records(const double* d_tri, int64_t n, double* d_rec, void* stream) {
+int pq3d_tde_source_records(const double* tri, int64_t n, double* rec) {
   clear_error();
   PQ3D_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return PQ3D_OK;
-  PQ3D_REQUIRE(d_tri && d_rec, "null pointer");
-  k_source_records<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(d_tri, n, d_rec);
-  PQ3D_LAUNCH_CHECK();
+  PQ3D_REQUIRE(tri && rec, "null pointer");
+  for (int64_t i = 0; i < n; ++i)
+    tde_source_record_host(tri + 9 * i, tri + 9 * i + 3, tri + 9 * i + 6, rec + kSrcRec * i);
   return PQ3D_OK;
 }
 
@@ -293,9 +281,9 @@ static int host_stress_points(const double* X, const double* Y, const double* Z,
   PQ3D_HIP_CHECK(hipMemcpyAsync(dX, X, nb, hipMemcpyHostToDevice, s));
   PQ3D_HIP_CHECK(hipMemcpyAsync(dY, Y, nb, hipMemcpyHostToDevice, s));
   PQ3D_HIP_CHECK(hipMemcpyAsync(dZ, Z, nb, hipMemcpyHostToDevice, s));
-  PQ3D_HIP_CHECK(hipMemcpyAsync(dT, tri, sizeof(tri), hipMemcpyHostToDevice, s));
-  rc = pq3d_tde_source_records(dT, 1, dR, s);
-  if (rc) return rc;
+  double rec[kSrcRec];
+  tde_source_record_host(tri, tri + 3, tri + 6, rec);
+  PQ3D_HIP_CHECK(hipMemcpyAsync(dR, rec, sizeof(rec), hipMemcpyHostToDevice, s));
   rc = pq3d_tde_stress_points(dX, dY, dZ, (int64_t)n, dR, Ss, Ds, Ts, mu, lam, dS, dE, s);
   if (rc) return rc;
   PQ3D_HIP_CHECK(hipMemcpyAsync(stress, dS, 6 * nb, hipMemcpyDeviceToHost, s));
@@ -309,25 +297,19 @@ static int host_stress_sources(double x, double y, double z, const double* P1f,
                                double Ds, double Ts, double mu, double lam, double* stress,
                                double* strain) {
   std::lock_guard<std::mutex> lk(g_pool.mu);
-  const size_t total = (9 * ns + kSrcRec * ns + 12 * ns) * sizeof(double);
+  const size_t total = (kSrcRec * ns + 12 * ns) * sizeof(double);
   void* base = nullptr;
   int rc = pool_reserve(total, &base);
   if (rc) return rc;
-  double* dT = (double*)base;
-  double* dR = dT + 9 * ns;
+  double* dR = (double*)base;
   double* dS = dR + (size_t)kSrcRec * ns;
   double* dE = dS + 6 * ns;
   hipStream_t s = g_pool.stream;
-  std::vector<double> tri(9 * ns);
+  std::vector<double> recs((size_t)kSrcRec * ns);
   for (size_t i = 0; i < ns; ++i)
-    for (int k = 0; k < 3; ++k) {
-      tri[9 * i + k] = P1f[3 * i + k];
-      tri[9 * i + 3 + k] = P2f[3 * i + k];
-      tri[9 * i + 6 + k] = P3f[3 * i + k];
-    }
-  PQ3D_HIP_CHECK(hipMemcpyAsync(dT, tri.data(), 9 * ns * sizeof(double), hipMemcpyHostToDevice, s));
-  rc = pq3d_tde_source_records(dT, (int64_t)ns, dR, s);
-  if (rc) return rc;
+    tde_source_record_host(P1f + 3 * i, P2f + 3 * i, P3f + 3 * i, recs.data() + kSrcRec * i);
+  PQ3D_HIP_CHECK(hipMemcpyAsync(dR, recs.data(), recs.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
   rc = pq3d_tde_stress_sources(x, y, z, dR, (int64_t)ns, Ss, Ds, Ts, mu, lam, dS, dE, s);
   if (rc) return rc;
   PQ3D_HIP_CHECK(hipMemcpyAsync(stress, dS, 6 * ns * sizeof(double), hipMemcpyDeviceToHost, s));

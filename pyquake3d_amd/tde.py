@@ -65,6 +65,15 @@ def TDstressEachSourceAtReceiver_C(x, y, z, P1_flat, P2_flat, P3_flat, Ss, Ds, T
     return stress.reshape(ns, 6), strain.reshape(ns, 6)
 
 
+def source_records(tri: np.ndarray) -> np.ndarray:
+    """(N, 32) per-triangle records, computed on the host by the library."""
+    tri = np.ascontiguousarray(tri, dtype=np.float64).reshape(-1, 9)
+    rec = np.zeros((tri.shape[0], 32))
+    _lib.check(_lib.load().pq3d_tde_source_records(_dp(tri), tri.shape[0], _dp(rec)),
+               "pq3d_tde_source_records")
+    return rec
+
+
 class DeviceMesh:
     """Device-resident geometry of one fault mesh.
 
@@ -86,9 +95,7 @@ class DeviceMesh:
         self.n = tri.shape[0]
         self.tri = torch.from_numpy(np.ascontiguousarray(tri)).to(self.device)
         self.recv = torch.from_numpy(receivers(self.eleVec, self.xg)).to(self.device)
-        self.rec = torch.empty((self.n, 32), dtype=torch.float64, device=self.device)
-        _lib.call("pq3d_tde_source_records", _lib.ptr(self.tri), self.n, _lib.ptr(self.rec),
-                  _lib.stream_handle())
+        self.rec = torch.from_numpy(source_records(tri)).to(self.device)
 
     @classmethod
     def from_mesh(cls, mesh: FaultMesh, mu, lambda_, device="cuda"):
