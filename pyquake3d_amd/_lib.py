@@ -41,6 +41,33 @@ SIGNATURES = {
                                             c_void_p, c_int64, c_int64, c_double, c_double,
                                             c_void_p, c_int64, c_int64, c_int64, c_int64,
                                             c_void_p]),
+    # H-matrix
+    "pq3d_hmat_dense_assemble": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_double,
+                                         c_double, c_void_p, c_int, c_void_p]),
+    "pq3d_hmat_aca": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_double,
+                              c_double, c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p]),
+    "pq3d_hmat_aca_pack": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                   c_int, c_void_p]),
+    "pq3d_hmat_matvec": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p,
+                                 c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p]),
+    # time step
+    "pq3d_rk45_attempt": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int64,
+                                  c_void_p]),
+    "pq3d_rk45_commit": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    "pq3d_deficit": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p]),
+    "pq3d_qd_rhs": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_double, c_double, c_double, c_void_p, c_void_p]),
+    "pq3d_add_loading": (c_int, [c_int64, c_void_p, c_void_p, c_int, c_void_p]),
 }
 
 _lib = None

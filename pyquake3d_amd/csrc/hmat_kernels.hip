@@ -1,0 +1,450 @@
+// H-matrix kernels: batched near-field assembly, in-kernel ACA of the far
+// field, and the streamed H-matvec (replaces src/Hmatrix.py:1014-1066
+// ACA_worker / calc_stressfunc_* and :1445-1538 blocks_process_MVM).
+//
+// Index space: every vector and block range is in cluster order (the cluster
+// tree's permutation applied once at upload), so every block is a pair of
+// contiguous ranges [r0, r0+m) x [c0, c0+n).
+//
+// Operator stack. The six traction operators of a block are treated as one
+// stacked matrix with 3m rows (AdotV1, AdotV2, dsigma rows) and 2n columns
+// (strike-slip then dip-slip sources):
+//     K[c*m + i, s*n + j] = op(c, s)[i, j],
+//     op(0,0)=A1s op(1,0)=A2s op(2,0)=Bs op(0,1)=A1d op(1,1)=A2d op(2,1)=Bd,
+// so that one product K [v1; v2] yields the three traction rates the time
+// step needs (src/QDsim.py:906-917). Far-field blocks store K ~= U V with
+// U (3m x k, column-major) and V (k x 2n, row-major); near-field blocks store
+// the six m x n operators, each column-major.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pq3d_common.h"
+#include "tde_device.h"
+
+namespace pq3d {
+
+// dense block table: r0, m, c0, n, base (pool offset), unused
+enum { DB_R0 = 0, DB_M, DB_C0, DB_N, DB_BASE, DB_W };
+// low-rank block table: r0, m, c0, n, ubase, vbase, k, toff
+enum { LB_R0 = 0, LB_M, LB_C0, LB_N, LB_UB, LB_VB, LB_K, LB_TOFF, LB_W };
+// matvec work item: kind (0 dense, 1 low-rank), base, ld, plane, count, xoff
+enum { IT_KIND = 0, IT_BASE, IT_LD, IT_PLANE, IT_COUNT, IT_XOFF, IT_W };
+// V-row item: vbase (row start), n, c0, t index
+enum { VR_BASE = 0, VR_N, VR_C0, VR_T, VR_W };
+
+__device__ inline void load_rec_g(const double* __restrict__ src, double* r) {
+#pragma unroll
+  for (int k = 0; k < kSrcRec; ++k) r[k] = src[k];
+}
+
+// ------------------------------------------------------ near-field assembly
+// One workgroup per dense block; lane <-> receiver row (column-major writes).
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_assemble(const int64_t* __restrict__ tab,
+                                                        const double* __restrict__ recv,
+                                                        const double* __restrict__ rec,
+                                                        double mu, double lam, double nu,
+                                                        T* __restrict__ pool) {
+  const int64_t* b = tab + (int64_t)blockIdx.x * DB_W;
+  const int64_t r0 = b[DB_R0], m = b[DB_M], c0 = b[DB_C0], n = b[DB_N], base = b[DB_BASE];
+  const int64_t mn = m * n;
+  for (int64_t p = threadIdx.x; p < mn; p += blockDim.x) {
+    const int64_t i = p % m, j = p / m;
+    double r[kSrcRec];
+    load_rec_g(rec + kSrcRec * (c0 + j), r);
+    const double* q = recv + 12 * (r0 + i);
+    double ts[3], td[3];
+    tde_pair_traction_sd(r, q, q + 3, mu, lam, nu, ts, td);
+    T* o = pool + base + j * m + i;
+    o[0 * mn] = (T)ts[0];
+    o[1 * mn] = (T)ts[1];
+    o[2 * mn] = (T)ts[2];
+    o[3 * mn] = (T)td[0];
+    o[4 * mn] = (T)td[1];
+    o[5 * mn] = (T)td[2];
+  }
+}
+
+// ------------------------------------------------------------------- ACA
+constexpr int kAcaThreads = 256;
+
+struct ArgMax {
+  double v;
+  int64_t i;
+};
+
+__device__ inline ArgMax block_argmax(ArgMax a, double* sv, int64_t* si) {
+  const int t = threadIdx.x;
+  sv[t] = a.v;
+  si[t] = a.i;
+  __syncthreads();
+  for (int s = kAcaThreads / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      const double o = sv[t + s];
+      const int64_t oi = si[t + s];
+      // larger value wins; ties go to the smaller index (deterministic)
+      if (o > sv[t] || (o == sv[t] && oi >= 0 && (si[t] < 0 || oi < si[t]))) {
+        sv[t] = o;
+        si[t] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  ArgMax r{sv[0], si[0]};
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double block_sum(double v, double* sv) {
+  const int t = threadIdx.x;
+  sv[t] = v;
+  __syncthreads();
+  for (int s = kAcaThreads / 2; s > 0; s >>= 1) {
+    if (t < s) sv[t] += sv[t + s];
+    __syncthreads();
+  }
+  const double r = sv[0];
+  __syncthreads();
+  return r;
+}
+
+// Partially pivoted ACA of the stacked 3m x 2n block, one workgroup per block.
+// Entries are evaluated in-kernel (TDE pair kernel), never materialised.
+// Workspace: U column l at wU[ub + l*3m], V row l at wV[vb + l*2n] (FP64).
+// Stops when |u_k| |v_k| <= eps |U_k V_k|_F (Frobenius estimate updated
+// incrementally). The table's k column holds the block's rank cap (its
+// workspace size). out_rank[b] = rank, or -1 when the cap was reached without
+// convergence (the caller then stores that block densely).
+// Bit masks of used rows/cols live in global scratch (flags, 1 byte each).
+__global__ void __launch_bounds__(kAcaThreads) k_aca(
+    const int64_t* __restrict__ tab, const int64_t* __restrict__ wsoff,
+    const double* __restrict__ recv, const double* __restrict__ rec, double mu, double lam,
+    double nu, double eps, int kmax, double* __restrict__ wU, double* __restrict__ wV,
+    uint8_t* __restrict__ flags, int64_t* __restrict__ out_rank) {
+  __shared__ double sv[kAcaThreads];
+  __shared__ int64_t si[kAcaThreads];
+  const int tid = threadIdx.x;
+  const int64_t bid = blockIdx.x;
+  const int64_t* b = tab + bid * LB_W;
+  const int64_t r0 = b[LB_R0], m = b[LB_M], c0 = b[LB_C0], n = b[LB_N];
+  const int64_t R = 3 * m, C = 2 * n;
+  // per-block rank cap (the workspace of this block holds kcap columns/rows)
+  const int kcap = (int)((b[LB_K] < kmax) ? b[LB_K] : kmax);
+  double* U = wU + wsoff[4 * bid + 0];
+  double* V = wV + wsoff[4 * bid + 1];
+  uint8_t* urow = flags + wsoff[4 * bid + 2];
+  uint8_t* ucol = urow + R;
+  for (int64_t r = tid; r < R; r += kAcaThreads) urow[r] = 0;
+  for (int64_t c = tid; c < C; c += kAcaThreads) ucol[c] = 0;
+  __syncthreads();
+
+  double normS2 = 0.0;
+  int64_t rstar = 0;  // first pivot row: component 0 of the first receiver
+  int rank = 0;
+  int zero_rows = 0;
+  bool converged = false;
+  for (int k = 0; k < kcap;) {
+    // ---- residual of row rstar = (cs, is)
+    const int cs = (int)(rstar / m);
+    const int64_t is = rstar % m;
+    double* vk = V + (int64_t)k * C;
+    const double* q = recv + 12 * (r0 + is);
+    ArgMax loc{-1.0, -1};
+    for (int64_t j = tid; j < n; j += kAcaThreads) {
+      double rr[kSrcRec];
+      load_rec_g(rec + kSrcRec * (c0 + j), rr);
+      double ts[3], td[3];
+      tde_pair_traction_sd(rr, q, q + 3, mu, lam, nu, ts, td);
+      double a = ts[cs], d = td[cs];
+      for (int l = 0; l < k; ++l) {
+        const double ul = U[(int64_t)l * R + rstar];
+        a -= ul * V[(int64_t)l * C + j];
+        d -= ul * V[(int64_t)l * C + n + j];
+      }
+      vk[j] = a;
+      vk[n + j] = d;
+      if (!ucol[j] && fabs(a) > loc.v) { loc.v = fabs(a); loc.i = j; }
+      if (!ucol[n + j] && fabs(d) > loc.v) { loc.v = fabs(d); loc.i = n + j; }
+    }
+    __syncthreads();
+    const ArgMax piv = block_argmax(loc, sv, si);
+    if (tid == 0) urow[rstar] = 1;
+    __syncthreads();
+    if (piv.i < 0 || !(piv.v > 0.0)) {
+      // a (numerically) zero row: move to the next unused row
+      ++zero_rows;
+      ArgMax nx{-1.0, -1};
+      for (int64_t r = tid; r < R; r += kAcaThreads)
+        if (!urow[r]) { nx.v = 1.0; nx.i = r; break; }
+      // smallest unused index: use -index as the value so the max picks it
+      if (nx.i >= 0) nx.v = -(double)nx.i;
+      else nx.v = -1e300;
+      const ArgMax nr = block_argmax(nx, sv, si);
+      if (nr.i < 0 || zero_rows > 3 * 64) { converged = true; break; }
+      rstar = nr.i;
+      continue;
+    }
+    const int64_t jstar = piv.i;
+    const double delta = vk[jstar];
+    __syncthreads();
+    for (int64_t j = tid; j < C; j += kAcaThreads) vk[j] /= delta;
+    if (tid == 0) ucol[jstar] = 1;
+    __syncthreads();
+    // ---- residual of column jstar = (ss, js): 3 stack rows per receiver
+    const int ss = (int)(jstar / n);
+    const int64_t js = jstar % n;
+    double* uk = U + (int64_t)k * R;
+    {
+      double rr[kSrcRec];
+      load_rec_g(rec + kSrcRec * (c0 + js), rr);
+      for (int64_t i = tid; i < m; i += kAcaThreads) {
+        const double* qi = recv + 12 * (r0 + i);
+        double ts[3], td[3];
+        tde_pair_traction_sd(rr, qi, qi + 3, mu, lam, nu, ts, td);
+        for (int c = 0; c < 3; ++c) {
+          double v = ss ? td[c] : ts[c];
+          const int64_t row = c * m + i;
+          for (int l = 0; l < k; ++l) v -= U[(int64_t)l * R + row] * V[(int64_t)l * C + jstar];
+          uk[row] = v;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- norms and Frobenius estimate
+    double pu = 0.0, pv = 0.0;
+    ArgMax nxt{-1.0, -1};
+    for (int64_t r = tid; r < R; r += kAcaThreads) {
+      const double u = uk[r];
+      pu += u * u;
+      if (!urow[r] && fabs(u) > nxt.v) { nxt.v = fabs(u); nxt.i = r; }
+    }
+    for (int64_t c = tid; c < C; c += kAcaThreads) pv += vk[c] * vk[c];
+    const double nu2 = block_sum(pu, sv);
+    const double nv2 = block_sum(pv, sv);
+    double cross = 0.0;
+    for (int l = 0; l < k; ++l) {
+      double a = 0.0, c2 = 0.0;
+      for (int64_t r = tid; r < R; r += kAcaThreads) a += U[(int64_t)l * R + r] * uk[r];
+      for (int64_t c = tid; c < C; c += kAcaThreads) c2 += V[(int64_t)l * C + c] * vk[c];
+      const double ua = block_sum(a, sv);
+      const double vb = block_sum(c2, sv);
+      cross += ua * vb;
+    }
+    normS2 += 2.0 * cross + nu2 * nv2;
+    ++k;
+    rank = k;
+    const ArgMax nr = block_argmax(nxt, sv, si);
+    if (sqrt(nu2 * nv2) <= eps * sqrt(fabs(normS2))) { converged = true; break; }
+    if (nr.i < 0) { converged = true; break; }  // every row used: exact
+    rstar = nr.i;
+  }
+  if (tid == 0) out_rank[bid] = converged ? rank : -1;
+}
+
+// Copy the converged factors into the storage pool (type T).
+template <typename T>
+__global__ void __launch_bounds__(256) k_aca_pack(const int64_t* __restrict__ tab,
+                                                  const int64_t* __restrict__ wsoff,
+                                                  const double* __restrict__ wU,
+                                                  const double* __restrict__ wV,
+                                                  T* __restrict__ pool) {
+  const int64_t bid = blockIdx.x;
+  const int64_t* b = tab + bid * LB_W;
+  const int64_t m = b[LB_M], n = b[LB_N], k = b[LB_K];
+  if (k <= 0) return;
+  const double* U = wU + wsoff[4 * bid + 0];
+  const double* V = wV + wsoff[4 * bid + 1];
+  T* pu = pool + b[LB_UB];
+  T* pv = pool + b[LB_VB];
+  const int64_t nu = 3 * m * k, nvv = 2 * n * k;
+  for (int64_t p = threadIdx.x; p < nu; p += blockDim.x) pu[p] = (T)U[p];
+  for (int64_t p = threadIdx.x; p < nvv; p += blockDim.x) pv[p] = (T)V[p];
+}
+
+// ------------------------------------------------------------- H-matvec
+// Phase 1: t[row] = V[row, :] . [v1[c0:c0+n], v2[c0:c0+n]], one wavefront per
+// V row, FP64 accumulation.
+template <typename T>
+__global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ vrows,
+                                                    int64_t nrows, const T* __restrict__ pool,
+                                                    const double* __restrict__ v1,
+                                                    const double* __restrict__ v2,
+                                                    double* __restrict__ t) {
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  const int64_t* it = vrows + row * VR_W;
+  const T* v = pool + it[VR_BASE];
+  const int64_t n = it[VR_N], c0 = it[VR_C0];
+  double acc = 0.0;
+  for (int64_t j = lane; j < n; j += 64) {
+    acc += (double)v[j] * v1[c0 + j];
+    acc += (double)v[n + j] * v2[c0 + j];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) t[it[VR_T]] = acc;
+}
+
+constexpr int kTileWaves = 4;
+
+// Phase 2: one workgroup per row tile (a leaf cluster of <= 64 rows); lane <->
+// row. Waves split the tile's work items; partial sums meet in LDS.
+// y1 = AdotV1, y2 = AdotV2, y3 = dsigma (K part only).
+template <typename T>
+__global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
+    const int64_t* __restrict__ tiles, const int64_t* __restrict__ tile_ptr,
+    const int64_t* __restrict__ items, const T* __restrict__ pool,
+    const double* __restrict__ v1, const double* __restrict__ v2, const double* __restrict__ t,
+    double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3) {
+  __shared__ double red[kTileWaves][3][64];
+  const int64_t tile = blockIdx.x;
+  const int64_t R0 = tiles[2 * tile], mt = tiles[2 * tile + 1];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const bool live = lane < mt;
+  double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int64_t q = tile_ptr[tile] + w; q < tile_ptr[tile + 1]; q += kTileWaves) {
+    const int64_t* it = items + q * IT_W;
+    const int64_t kind = it[IT_KIND], base = it[IT_BASE], ld = it[IT_LD], plane = it[IT_PLANE];
+    const int64_t cnt = it[IT_COUNT], xo = it[IT_XOFF];
+    if (!live) continue;
+    const T* p = pool + base + lane;
+    if (kind == 0) {
+      for (int64_t c = 0; c < cnt; ++c) {
+        const double x1 = v1[xo + c], x2 = v2[xo + c];
+        const T* pc = p + c * ld;
+        a1 += (double)pc[0] * x1 + (double)pc[3 * plane] * x2;
+        a2 += (double)pc[plane] * x1 + (double)pc[4 * plane] * x2;
+        a3 += (double)pc[2 * plane] * x1 + (double)pc[5 * plane] * x2;
+      }
+    } else {
+      for (int64_t l = 0; l < cnt; ++l) {
+        const double tl = t[xo + l];
+        const T* pc = p + l * ld;
+        a1 += (double)pc[0] * tl;
+        a2 += (double)pc[plane] * tl;
+        a3 += (double)pc[2 * plane] * tl;
+      }
+    }
+  }
+  red[w][0][lane] = a1;
+  red[w][1][lane] = a2;
+  red[w][2][lane] = a3;
+  __syncthreads();
+  if (w == 0 && live) {
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kTileWaves; ++k) {
+      s1 += red[k][0][lane];
+      s2 += red[k][1][lane];
+      s3 += red[k][2][lane];
+    }
+    y1[R0 + lane] = s1;
+    y2[R0 + lane] = s2;
+    y3[R0 + lane] = s3;
+  }
+}
+
+}  // namespace pq3d
+
+using namespace pq3d;
+
+extern "C" {
+
+int pq3d_hmat_dense_assemble(const int64_t* d_tab, int64_t nblocks, const double* d_recv,
+                             const double* d_rec, double mu, double lam, void* d_pool,
+                             int dtype_bytes, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(nblocks >= 0, "nblocks must be >= 0");
+  if (nblocks == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_tab && d_recv && d_rec && d_pool, "null pointer");
+  PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
+  PQ3D_REQUIRE(nblocks < (1LL << 31), "too many blocks");
+  const double nu = 1.0 / (1.0 + lam / mu) / 2.0;
+  if (dtype_bytes == 4)
+    k_dense_assemble<float><<<(unsigned)nblocks, 256, 0, as_stream(stream)>>>(
+        d_tab, d_recv, d_rec, mu, lam, nu, (float*)d_pool);
+  else
+    k_dense_assemble<double><<<(unsigned)nblocks, 256, 0, as_stream(stream)>>>(
+        d_tab, d_recv, d_rec, mu, lam, nu, (double*)d_pool);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
+                  const double* d_recv, const double* d_rec, double mu, double lam, double eps,
+                  int kmax, double* d_wU, double* d_wV, uint8_t* d_flags, int64_t* d_rank,
+                  void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(nblocks >= 0, "nblocks must be >= 0");
+  if (nblocks == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_tab && d_wsoff && d_recv && d_rec && d_wU && d_wV && d_flags && d_rank,
+               "null pointer");
+  PQ3D_REQUIRE(kmax >= 1 && kmax <= 4096, "kmax out of range");
+  PQ3D_REQUIRE(eps > 0.0, "eps must be > 0");
+  PQ3D_REQUIRE(nblocks < (1LL << 31), "too many blocks");
+  const double nu = 1.0 / (1.0 + lam / mu) / 2.0;
+  k_aca<<<(unsigned)nblocks, kAcaThreads, 0, as_stream(stream)>>>(
+      d_tab, d_wsoff, d_recv, d_rec, mu, lam, nu, eps, kmax, d_wU, d_wV, d_flags, d_rank);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
+                       const double* d_wU, const double* d_wV, void* d_pool, int dtype_bytes,
+                       void* stream) {
+  clear_error();
+  if (nblocks == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_tab && d_wsoff && d_wU && d_wV && d_pool, "null pointer");
+  PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
+  if (dtype_bytes == 4)
+    k_aca_pack<float><<<(unsigned)nblocks, 256, 0, as_stream(stream)>>>(d_tab, d_wsoff, d_wU,
+                                                                         d_wV, (float*)d_pool);
+  else
+    k_aca_pack<double><<<(unsigned)nblocks, 256, 0, as_stream(stream)>>>(d_tab, d_wsoff, d_wU,
+                                                                          d_wV, (double*)d_pool);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+// y1 = AdotV1, y2 = AdotV2, y3 = dsigma (K part) for deficits v1, v2 (all FP64,
+// cluster order). d_t: scratch of nvrows doubles.
+int pq3d_hmat_matvec(const int64_t* d_vrows, int64_t nvrows, const int64_t* d_tiles,
+                     const int64_t* d_tile_ptr, int64_t ntiles, const int64_t* d_items,
+                     const void* d_pool, int dtype_bytes, const double* d_v1,
+                     const double* d_v2, double* d_t, double* d_y1, double* d_y2, double* d_y3,
+                     void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(ntiles >= 0 && nvrows >= 0, "sizes must be >= 0");
+  PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
+  PQ3D_REQUIRE(d_tiles && d_tile_ptr && d_items && d_pool && d_v1 && d_v2 && d_y1 && d_y2 &&
+                   d_y3,
+               "null pointer");
+  hipStream_t s = as_stream(stream);
+  if (nvrows > 0) {
+    PQ3D_REQUIRE(d_vrows && d_t, "null pointer");
+    const unsigned g = (unsigned)((nvrows + 3) / 4);
+    if (dtype_bytes == 4)
+      k_hmv_phase1<float><<<g, 256, 0, s>>>(d_vrows, nvrows, (const float*)d_pool, d_v1, d_v2,
+                                            d_t);
+    else
+      k_hmv_phase1<double><<<g, 256, 0, s>>>(d_vrows, nvrows, (const double*)d_pool, d_v1,
+                                             d_v2, d_t);
+    PQ3D_LAUNCH_CHECK();
+  }
+  if (ntiles > 0) {
+    if (dtype_bytes == 4)
+      k_hmv_phase2<float><<<(unsigned)ntiles, 64 * kTileWaves, 0, s>>>(
+          d_tiles, d_tile_ptr, d_items, (const float*)d_pool, d_v1, d_v2, d_t, d_y1, d_y2, d_y3);
+    else
+      k_hmv_phase2<double><<<(unsigned)ntiles, 64 * kTileWaves, 0, s>>>(
+          d_tiles, d_tile_ptr, d_items, (const double*)d_pool, d_v1, d_v2, d_t, d_y1, d_y2,
+          d_y3);
+    PQ3D_LAUNCH_CHECK();
+  }
+  return PQ3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,346 @@
+// Quasi-dynamic traction-rate RHS and the adaptive Dormand-Prince RK45 step,
+// FP64, one thread per fault cell (replaces src/QDsim.py:760-813 `derivative`,
+// :1001-1134 `RungeKutte_solve_Dormand_Prince` and the state update of
+// :960-993 `simu_forward`).
+//
+// Within one time step the reference freezes the elastic traction rates
+// (K (V - Vpl), computed once per step, :906-922) and the slip rate V used by
+// the aging law (:803). The seven RHS evaluations of an attempt are therefore
+// purely cell-local, so a whole RK45 attempt -- all stages, the 5th- and
+// 4th-order solutions and the embedded error ratios -- is ONE kernel that keeps
+// every stage in registers; the two error maxima go to device memory with an
+// order-independent atomic max.
+//
+// Cell state (MPa for tractions, as the reference stores them): sigma (Tno),
+// tau1 (Tt1o), tau2 (Tt2o), psi (state). The arithmetic follows the
+// reference's expressions term by term.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "pq3d_common.h"
+
+namespace pq3d {
+
+struct CellParams {
+  double V0, mu, Cs;
+};
+
+// RHS of one cell (src/QDsim.py:760-813). Inputs in MPa; returns
+// (dpsi/dt, dsigma/dt [MPa/s], dtau1/dt [MPa/s], dtau2/dt [MPa/s]).
+__device__ inline void qd_rhs(double Tno, double Tt1o, double Tt2o, double state, double a,
+                              double b, double dc, double f0, double V0, double mu, double Cs,
+                              double AdotV1, double AdotV2, double dsigmadt,
+                              double shear_loading, double slipv, double out[4]) {
+  Tno = Tno * 1e6;
+  Tt1o = Tt1o * 1e6;
+  Tt2o = Tt2o * 1e6;
+  const double e1p = exp(-state / a + Tt1o / (a * Tno));
+  const double e1m = exp(-state / a - Tt1o / (a * Tno));
+  const double e2p = exp(-state / a + Tt2o / (a * Tno));
+  const double e2m = exp(-state / a - Tt2o / (a * Tno));
+  const double dV1dtau = 2 * V0 / (a * Tno) * 0.5 * (e1p + e1m);
+  const double dV2dtau = 2 * V0 / (a * Tno) * 0.5 * (e2p + e2m);
+  const double dV1dsigma = -2 * V0 * Tt1o / (a * (Tno * Tno)) * 0.5 * (e1p + e1m);
+  const double dV2dsigma = -2 * V0 * Tt2o / (a * (Tno * Tno)) * 0.5 * (e2p + e2m);
+  const double dV1dstate = -2 * V0 / a * 0.5 * (e1p - e1m);
+  const double dV2dstate = -2 * V0 / a * 0.5 * (e2p - e2m);
+  double xs = (f0 - state) / b;
+  xs = fmin(fmax(xs, -700.0), 700.0);
+  const double dstatedt = b / dc * (V0 * exp(xs) - slipv);
+  const double eta = mu / (2.0 * Cs);
+  const double dtau1dt =
+      (-AdotV1 + shear_loading - eta * (dV1dsigma * dsigmadt + dV1dstate * dstatedt)) /
+      (1.0 + eta * dV1dtau);
+  const double dtau2dt =
+      (-AdotV2 + shear_loading - eta * (dV2dsigma * dsigmadt + dV2dstate * dstatedt)) /
+      (1.0 + eta * dV2dtau);
+  out[0] = dstatedt;
+  out[1] = dsigmadt / 1e6;
+  out[2] = dtau1dt / 1e6;
+  out[3] = dtau2dt / 1e6;
+}
+
+// Monotone encoding of |x| for atomic max (NaN sorts above +inf).
+__device__ inline unsigned long long abs_bits(double x) {
+  return (unsigned long long)__double_as_longlong(fabs(x));
+}
+
+// Dormand-Prince 5(4) coefficients (src/QDsim.py:1002-1032). a63 is a
+// runtime argument: the reference uses -46732/5247 (src/QDsim.py:1019), the
+// published tableau +46732/5247.
+#define B21 (.2)
+#define B31 (3. / 40)
+#define B32 (9. / 40.)
+#define B41 (44. / 45.)
+#define B42 (-56. / 15)
+#define B43 (32. / 9)
+#define B51 (19372. / 6561.)
+#define B52 (-25360 / 2187.)
+#define B53 (64448. / 6561.)
+#define B54 (-212. / 729.)
+#define B61 (9017. / 3168.)
+#define B62 (-355. / 33.)
+#define B64 (49. / 176.)
+#define B65 (-5103. / 18656.)
+#define B71 (35. / 384.)
+#define B73 (500. / 1113.)
+#define B74 (125. / 192.)
+#define B75 (-2187. / 6784.)
+#define B76 (11. / 84.)
+#define B81 (5179. / 57600.)
+#define B83 (7571. / 16695.)
+#define B84 (393. / 640.)
+#define B85 (-92097. / 339200.)
+#define B86 (187. / 2100.)
+#define B87 (1. / 40.)
+
+// One RK45 attempt of step h for n cells. y = (Tno, Tt1o, Tt2o, state) in,
+// yn = 5th-order candidate out; err[4g..4g+3] = max over cells of
+// |d psi/psi8|, |d tau1/tau1_8|, |d tau2/tau2_8|, |d sigma/sigma8| (bit patterns
+// of non-negative doubles, NaN above +inf; zeroed by the entry point). Batched form: instance of cell i is
+// inst[i] (or 0 when inst is null), with its own h and error slots.
+__global__ void __launch_bounds__(256) k_rk45_attempt(
+    int64_t n, const int32_t* __restrict__ inst, const double* __restrict__ hvec,
+    const double* __restrict__ Tno, const double* __restrict__ Tt1o,
+    const double* __restrict__ Tt2o, const double* __restrict__ st,
+    const double* __restrict__ pa, const double* __restrict__ pb,
+    const double* __restrict__ pdc, const double* __restrict__ pf0,
+    const double* __restrict__ AdotV1, const double* __restrict__ AdotV2,
+    const double* __restrict__ dsig, const double* __restrict__ shear_loading,
+    const double* __restrict__ slipv, const double* __restrict__ cellpar, double* __restrict__ yTno,
+    double* __restrict__ yTt1o, double* __restrict__ yTt2o, double* __restrict__ yst,
+    unsigned long long* __restrict__ err, double B63) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int g = inst ? inst[i] : 0;
+  const double h = hvec[g];
+  const double V0 = cellpar[3 * g + 0], mu = cellpar[3 * g + 1], Cs = cellpar[3 * g + 2];
+  const double a = pa[i], b = pb[i], dc = pdc[i], f0 = pf0[i];
+  const double A1 = AdotV1[i], A2 = AdotV2[i], ds = dsig[i], sl = shear_loading[i], V = slipv[i];
+  const double s0 = Tno[i], t10 = Tt1o[i], t20 = Tt2o[i], p0 = st[i];
+  double k1[4], k2[4], k3[4], k4[4], k5[4], k6[4], k7[4];
+  // order in k*: (dstate, dsigma, dtau1, dtau2)
+  qd_rhs(s0, t10, t20, p0, a, b, dc, f0, V0, mu, Cs, A1, A2, ds, sl, V, k1);
+  double s = s0 + h * B21 * k1[1];
+  double t1 = t10 + h * B21 * k1[2];
+  double t2 = t20 + h * B21 * k1[3];
+  double p = p0 + h * B21 * k1[0];
+  qd_rhs(s, t1, t2, p, a, b, dc, f0, V0, mu, Cs, A1, A2, ds, sl, V, k2);
+  s = s0 + h * (B31 * k1[1] + B32 * k2[1]);
+  t1 = t10 + h * (B31 * k1[2] + B32 * k2[2]);
+  t2 = t20 + h * (B31 * k1[3] + B32 * k2[3]);
+  p = p0 + h * (B31 * k1[0] + B32 * k2[0]);
+  qd_rhs(s, t1, t2, p, a, b, dc, f0, V0, mu, Cs, A1, A2, ds, sl, V, k3);
+  s = s0 + h * (B41 * k1[1] + B42 * k2[1] + B43 * k3[1]);
+  t1 = t10 + h * (B41 * k1[2] + B42 * k2[2] + B43 * k3[2]);
+  t2 = t20 + h * (B41 * k1[3] + B42 * k2[3] + B43 * k3[3]);
+  p = p0 + h * (B41 * k1[0] + B42 * k2[0] + B43 * k3[0]);
+  qd_rhs(s, t1, t2, p, a, b, dc, f0, V0, mu, Cs, A1, A2, ds, sl, V, k4);
+  s = s0 + h * (B51 * k1[1] + B52 * k2[1] + B53 * k3[1] + B54 * k4[1]);
+  t1 = t10 + h * (B51 * k1[2] + B52 * k2[2] + B53 * k3[2] + B54 * k4[2]);
+  t2 = t20 + h * (B51 * k1[3] + B52 * k2[3] + B53 * k3[3] + B54 * k4[3]);
+  p = p0 + h * (B51 * k1[0] + B52 * k2[0] + B53 * k3[0] + B54 * k4[0]);
+  qd_rhs(s, t1, t2, p, a, b, dc, f0, V0, mu, Cs, A1, A2, ds, sl, V, k5);
+  s = s0 + h * (B61 * k1[1] + B62 * k2[1] + B63 * k3[1] + B64 * k4[1] + B65 * k5[1]);
+  t1 = t10 + h * (B61 * k1[2] + B62 * k2[2] + B63 * k3[2] + B64 * k4[2] + B65 * k5[2]);
+  t2 = t20 + h * (B61 * k1[3] + B62 * k2[3] + B63 * k3[3] + B64 * k4[3] + B65 * k5[3]);
+  p = p0 + h * (B61 * k1[0] + B62 * k2[0] + B63 * k3[0] + B64 * k4[0] + B65 * k5[0]);
+  qd_rhs(s, t1, t2, p, a, b, dc, f0, V0, mu, Cs, A1, A2, ds, sl, V, k6);
+  const double s7 = s0 + h * (B71 * k1[1] + B73 * k3[1] + B74 * k4[1] + B75 * k5[1] + B76 * k6[1]);
+  const double t17 = t10 + h * (B71 * k1[2] + B73 * k3[2] + B74 * k4[2] + B75 * k5[2] + B76 * k6[2]);
+  const double t27 = t20 + h * (B71 * k1[3] + B73 * k3[3] + B74 * k4[3] + B75 * k5[3] + B76 * k6[3]);
+  const double p7 = p0 + h * (B71 * k1[0] + B73 * k3[0] + B74 * k4[0] + B75 * k5[0] + B76 * k6[0]);
+  qd_rhs(s7, t17, t27, p7, a, b, dc, f0, V0, mu, Cs, A1, A2, ds, sl, V, k7);
+  const double s8 = s0 + h * (B81 * k1[1] + B83 * k3[1] + B84 * k4[1] + B85 * k5[1] + B86 * k6[1] + B87 * k7[1]);
+  const double t18 = t10 + h * (B81 * k1[2] + B83 * k3[2] + B84 * k4[2] + B85 * k5[2] + B86 * k6[2] + B87 * k7[2]);
+  const double t28 = t20 + h * (B81 * k1[3] + B83 * k3[3] + B84 * k4[3] + B85 * k5[3] + B86 * k6[3] + B87 * k7[3]);
+  const double p8 = p0 + h * (B81 * k1[0] + B83 * k3[0] + B84 * k4[0] + B85 * k5[0] + B86 * k6[0] + B87 * k7[0]);
+  yTno[i] = s7;
+  yTt1o[i] = t17;
+  yTt2o[i] = t27;
+  yst[i] = p7;
+  // embedded error ratios (src/QDsim.py:1090-1110)
+  const double e_state = fabs(fabs(p8 - p7) / p8);
+  const double e_t1 = fabs(fabs(t18 - t17) / t18);
+  const double e_t2 = fabs(fabs(t28 - t27) / t28);
+  const double e_s = fabs(fabs(s8 - s7) / s8);
+  // four separate maxima: the host combines them with the reference's own
+  // (NaN-order-sensitive) max() of the three traction terms
+  atomicMax(&err[4 * g + 0], abs_bits(e_state));
+  atomicMax(&err[4 * g + 1], abs_bits(e_t1));
+  atomicMax(&err[4 * g + 2], abs_bits(e_t2));
+  atomicMax(&err[4 * g + 3], abs_bits(e_s));
+}
+
+// Accept a step: state <- candidate, slip rates from the regularized law per
+// component, rake, V floor, slip (src/QDsim.py:960-993). vmax |= max V.
+__global__ void __launch_bounds__(256) k_rk45_commit(
+    int64_t n, const int32_t* __restrict__ inst, const double* __restrict__ hvec,
+    const uint8_t* __restrict__ accept, const double* __restrict__ pa,
+    const double* __restrict__ cellpar, const double* __restrict__ yTno,
+    const double* __restrict__ yTt1o, const double* __restrict__ yTt2o,
+    const double* __restrict__ yst, double* __restrict__ Tno, double* __restrict__ Tt1o,
+    double* __restrict__ Tt2o, double* __restrict__ st, double* __restrict__ slipv1,
+    double* __restrict__ slipv2, double* __restrict__ slipv, double* __restrict__ rake,
+    double* __restrict__ slip1, double* __restrict__ slip2,
+    unsigned long long* __restrict__ vmax) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int g = inst ? inst[i] : 0;
+  if (accept && !accept[g]) return;
+  const double h = hvec[g];
+  const double V0 = cellpar[3 * g + 0];
+  const double a = pa[i];
+  const double s = yTno[i], t1 = yTt1o[i], t2 = yTt2o[i], p = yst[i];
+  Tno[i] = s;
+  Tt1o[i] = t1;
+  Tt2o[i] = t2;
+  st[i] = p;
+  const double v1 = (2.0 * V0) * exp(-p / a) * sinh(t1 / s / a);
+  const double v2 = (2.0 * V0) * exp(-p / a) * sinh(t2 / s / a);
+  double v = sqrt(v1 * v1 + v2 * v2);
+  slipv1[i] = v1;
+  slipv2[i] = v2;
+  rake[i] = atan2(t2, t1);
+  if (v < 1e-30) v = 1e-30;
+  slipv[i] = v;
+  slip1[i] = slip1[i] + v1 * h;
+  slip2[i] = slip2[i] + v2 * h;
+  atomicMax(&vmax[g], abs_bits(v));
+}
+
+// Slip-rate deficit (src/QDsim.py:902-903): x1 = V1 - Vpl cos(rake0),
+// x2 = V2 - Vpl sin(rake0), with the products pl1 = Vpl cos(rake0) and
+// pl2 = Vpl sin(rake0) precomputed per cell (same rounding).
+__global__ void __launch_bounds__(256) k_deficit(int64_t n, const double* __restrict__ slipv1,
+                                                 const double* __restrict__ slipv2,
+                                                 const double* __restrict__ pl1,
+                                                 const double* __restrict__ pl2,
+                                                 double* __restrict__ x1,
+                                                 double* __restrict__ x2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x1[i] = slipv1[i] - pl1[i];
+  x2[i] = slipv2[i] - pl2[i];
+}
+
+// dsigma/dt = K part + normal loading (src/QDsim.py:911-913), or the loading
+// alone when the normal traction is fixed.
+__global__ void __launch_bounds__(256) k_add_loading(int64_t n, double* __restrict__ dsig,
+                                                     const double* __restrict__ load, int fix_tn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dsig[i] = fix_tn ? load[i] : dsig[i] + load[i];
+}
+
+// Stand-alone RHS over n cells (the reference's `derivative`): out is 4 x n
+// (dpsi/dt, dsigma/dt, dtau1/dt, dtau2/dt), the last three in MPa/s.
+__global__ void __launch_bounds__(256) k_qd_rhs(
+    int64_t n, const double* __restrict__ Tno, const double* __restrict__ Tt1o,
+    const double* __restrict__ Tt2o, const double* __restrict__ st, const double* __restrict__ pa,
+    const double* __restrict__ pb, const double* __restrict__ pdc, const double* __restrict__ pf0,
+    const double* __restrict__ AdotV1, const double* __restrict__ AdotV2,
+    const double* __restrict__ dsig, const double* __restrict__ shear_loading,
+    const double* __restrict__ slipv, double V0, double mu, double Cs, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double k[4];
+  qd_rhs(Tno[i], Tt1o[i], Tt2o[i], st[i], pa[i], pb[i], pdc[i], pf0[i], V0, mu, Cs, AdotV1[i],
+         AdotV2[i], dsig[i], shear_loading[i], slipv[i], k);
+  out[i] = k[0];
+  out[n + i] = k[1];
+  out[2 * n + i] = k[2];
+  out[3 * n + i] = k[3];
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+}  // namespace pq3d
+
+using namespace pq3d;
+
+extern "C" {
+
+int pq3d_rk45_attempt(int64_t n, const int32_t* d_inst, const double* d_h, const double* d_Tno,
+                      const double* d_Tt1o, const double* d_Tt2o, const double* d_state,
+                      const double* d_a, const double* d_b, const double* d_dc,
+                      const double* d_f0, const double* d_AdotV1, const double* d_AdotV2,
+                      const double* d_dsigmadt, const double* d_shear_loading,
+                      const double* d_slipv, const double* d_cellpar, double* d_yTno,
+                      double* d_yTt1o, double* d_yTt2o, double* d_ystate,
+                      unsigned long long* d_err, double a63, int64_t n_inst, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0 && n_inst >= 1, "bad sizes");
+  if (n == 0) return PQ3D_OK;
+  hipStream_t s = as_stream(stream);
+  k_zero_u64<<<blocks_for(4 * n_inst, 256), 256, 0, s>>>(d_err, 4 * n_inst);
+  k_rk45_attempt<<<blocks_for(n, 256), 256, 0, s>>>(
+      n, d_inst, d_h, d_Tno, d_Tt1o, d_Tt2o, d_state, d_a, d_b, d_dc, d_f0, d_AdotV1, d_AdotV2,
+      d_dsigmadt, d_shear_loading, d_slipv, d_cellpar, d_yTno, d_yTt1o, d_yTt2o, d_ystate, d_err,
+      a63);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_rk45_commit(int64_t n, const int32_t* d_inst, const double* d_h,
+                     const uint8_t* d_accept, const double* d_a, const double* d_cellpar,
+                     const double* d_yTno, const double* d_yTt1o, const double* d_yTt2o,
+                     const double* d_ystate, double* d_Tno, double* d_Tt1o, double* d_Tt2o,
+                     double* d_state, double* d_slipv1, double* d_slipv2, double* d_slipv,
+                     double* d_rake, double* d_slip1, double* d_slip2,
+                     unsigned long long* d_vmax, int64_t n_inst, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0 && n_inst >= 1, "bad sizes");
+  if (n == 0) return PQ3D_OK;
+  hipStream_t s = as_stream(stream);
+  k_zero_u64<<<blocks_for(n_inst, 256), 256, 0, s>>>(d_vmax, n_inst);
+  k_rk45_commit<<<blocks_for(n, 256), 256, 0, s>>>(
+      n, d_inst, d_h, d_accept, d_a, d_cellpar, d_yTno, d_yTt1o, d_yTt2o, d_ystate, d_Tno,
+      d_Tt1o, d_Tt2o, d_state, d_slipv1, d_slipv2, d_slipv, d_rake, d_slip1, d_slip2, d_vmax);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_deficit(int64_t n, const double* d_slipv1, const double* d_slipv2,
+                 const double* d_pl1, const double* d_pl2, double* d_x1, double* d_x2,
+                 void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0, "bad size");
+  if (n == 0) return PQ3D_OK;
+  k_deficit<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(n, d_slipv1, d_slipv2, d_pl1,
+                                                                  d_pl2, d_x1, d_x2);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_qd_rhs(int64_t n, const double* d_Tno, const double* d_Tt1o, const double* d_Tt2o,
+                const double* d_state, const double* d_a, const double* d_b, const double* d_dc,
+                const double* d_f0, const double* d_AdotV1, const double* d_AdotV2,
+                const double* d_dsigmadt, const double* d_shear_loading, const double* d_slipv,
+                double V0, double mu, double Cs, double* d_out, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0, "bad size");
+  if (n == 0) return PQ3D_OK;
+  k_qd_rhs<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
+      n, d_Tno, d_Tt1o, d_Tt2o, d_state, d_a, d_b, d_dc, d_f0, d_AdotV1, d_AdotV2, d_dsigmadt,
+      d_shear_loading, d_slipv, V0, mu, Cs, d_out);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_add_loading(int64_t n, double* d_dsig, const double* d_load, int fix_tn,
+                     void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0, "bad size");
+  if (n == 0) return PQ3D_OK;
+  k_add_loading<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(n, d_dsig, d_load, fix_tn);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,240 @@
+"""Device-resident quasi-dynamic earthquake-cycle stepper (one GPU, no MPI).
+
+Mirrors the reference's ``QDsim`` time step (src/QDsim.py:892-996,
+``simu_forward``) with the same semantics:
+  * one operator application per step on the slip-rate deficit
+    (V1 - Vpl cos rake0, V2 - Vpl sin rake0), frozen for every RK stage and
+    every rejected attempt (:902-922);
+  * Dormand-Prince 5(4) attempt with the max-relative error of psi and of
+    (tau1, tau2, sigma) divided by the 4th-order values, RelTol (1e-4 in the
+    reference, :1122-1123), hnew = 0.9 h (tol/err)^0.2 (:1001-1134);
+  * accept if max(cond1, cond2) < 1 and neither is NaN; dtnext = min(hnew1,
+    hnew2, 1.5 h); otherwise h = max(0.5 h, min(hnew1, hnew2)), abort when
+    h < 1e-15 or after 20 rejections (:931-958);
+  * commit: regularized law per component, rake, V floor 1e-30, slip (:960-993).
+The RK attempt is one fused HIP kernel (all 7 stages per cell in registers);
+the operator is an ``HMatrix`` (FP32 or FP64 storage) or a ``DenseOperator``.
+All device arrays are in the operator's cluster order.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .problem import Problem
+
+
+class StepAborted(RuntimeError):
+    """dt fell below 1e-15 s or more than 20 attempts were rejected."""
+
+
+class DenseOperator:
+    """FP64 dense six-operator matrix (the reference's plain method):
+    [[A1s, A1d], [A2s, A2d], [Bs, Bd]] (3N x 2N), applied with one GEMV."""
+
+    def __init__(self, mesh, mu, lambda_, device="cuda", dtype=torch.float64):
+        from .tde import DeviceMesh
+        dm = DeviceMesh.from_mesh(mesh, mu, lambda_, device=device)
+        K = dm.traction_block()
+        n = dm.n
+        self.n = n
+        self.device = dm.device
+        self.M = torch.empty((3 * n, 2 * n), dtype=dtype, device=self.device)
+        for c in range(3):
+            self.M[c * n:(c + 1) * n, :n] = K[c]
+            self.M[c * n:(c + 1) * n, n:] = K[3 + c]
+        del K
+        self.perm = np.arange(n)
+        self.xg = dm.xg
+        self.eleVec = dm.eleVec
+        self._x = torch.empty(2 * n, dtype=dtype, device=self.device)
+        self.matvec_bytes = self.M.numel() * self.M.element_size() + 5 * 8 * n
+
+    def matvec(self, v1, v2, out=None, stream=None):
+        n = self.n
+        if out is None:
+            out = torch.empty((3, n), dtype=torch.float64, device=self.device)
+        self._x[:n] = v1
+        self._x[n:] = v2
+        y = torch.mv(self.M, self._x.to(self.M.dtype))
+        out.view(-1).copy_(y)
+        return out
+
+
+class QDsim:
+    """Time stepper on device. ``problem`` arrays are in original element
+    order; ``op`` provides ``perm`` (cluster order) and ``matvec``."""
+
+    def __init__(self, problem: Problem, op, reltol: float = 1e-4, device="cuda",
+                 tableau: str = "ref"):
+        """tableau "ref": the reference's coefficients (a63 = -46732/5247,
+        src/QDsim.py:1019); "dp5": the published Dormand-Prince tableau."""
+        if tableau not in ("ref", "dp5"):
+            raise ValueError("tableau must be 'ref' or 'dp5'")
+        self.a63 = -46732. / 5247. if tableau == "ref" else 46732. / 5247.
+        self.p = problem
+        self.op = op
+        self.reltol = float(reltol)
+        dev = torch.device(device)
+        self.device = dev
+        self.n = n = problem.n
+        perm = np.asarray(op.perm)
+        self.perm = perm
+
+        def up(x):
+            return torch.from_numpy(np.ascontiguousarray(np.asarray(x, dtype=np.float64)[perm])).to(dev)
+
+        Tno, Tt1o, Tt2o, state, slipv1, slipv2, slipv = problem.initial_state()
+        self.Tno, self.Tt1o, self.Tt2o, self.state = up(Tno), up(Tt1o), up(Tt2o), up(state)
+        self.slipv1, self.slipv2, self.slipv = up(slipv1), up(slipv2), up(slipv)
+        self.rake = up(problem.rake)
+        self.slip1 = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.slip2 = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.a, self.b, self.dc, self.f0 = up(problem.a), up(problem.b), up(problem.dc), up(problem.f0)
+        rake0 = np.asarray(problem.rake, dtype=np.float64)
+        self.pl1 = up(problem.Vpl * np.cos(rake0))
+        self.pl2 = up(problem.Vpl * np.sin(rake0))
+        self.shear_loading = up(problem.shear_loading)
+        self.normal_loading = up(problem.normal_loading)
+        self.fix_Tn = bool(problem.fix_Tn)
+        self.V0 = float(problem.V0)
+        self.mu = float(problem.mu)
+        self.Cs = float(problem.Cs)
+        self.cellpar = torch.tensor([self.V0, self.mu, self.Cs], dtype=torch.float64, device=dev)
+        # candidate state, deficits, operator output (AdotV1, AdotV2, dsigmadt)
+        self.y = torch.empty((4, n), dtype=torch.float64, device=dev)
+        self.x = torch.empty((2, n), dtype=torch.float64, device=dev)
+        self.K = torch.empty((3, n), dtype=torch.float64, device=dev)
+        self.hdev = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.err = torch.zeros(4, dtype=torch.int64, device=dev)
+        self.vmax = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.err_host = torch.zeros(4, dtype=torch.int64).pin_memory() if torch.cuda.is_available() else torch.zeros(4, dtype=torch.int64)
+        self.time = 0.0
+        self.htry = 1e-3
+        self.maxslipv0 = float(np.max(slipv))
+        self.Relerrormax1 = self.Relerrormax2 = 0.0
+        self.Relerrormax1_last = self.Relerrormax2_last = 0.0
+        self.nsteps = 0
+        self.nattempts = 0
+        self.nrejects = 0
+
+    # -------------------------------------------------------------- kernels
+    def _stream(self):
+        return _lib.stream_handle()
+
+    def compute_K(self):
+        """Deficit -> operator -> (AdotV1, AdotV2, dsigmadt) on device."""
+        s = self._stream()
+        _lib.call("pq3d_deficit", self.n, _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
+                  _lib.ptr(self.pl1), _lib.ptr(self.pl2), _lib.ptr(self.x[0]),
+                  _lib.ptr(self.x[1]), s)
+        self.op.matvec(self.x[0], self.x[1], out=self.K)
+        _lib.call("pq3d_add_loading", self.n, _lib.ptr(self.K[2]), _lib.ptr(self.normal_loading),
+                  int(self.fix_Tn), s)
+
+    def _attempt(self, h):
+        """One RK45 attempt; returns (Relerrormax1, Relerrormax2 parts)."""
+        self.hdev.fill_(h)
+        s = self._stream()
+        _lib.call("pq3d_rk45_attempt", self.n, 0, _lib.ptr(self.hdev), _lib.ptr(self.Tno),
+                  _lib.ptr(self.Tt1o), _lib.ptr(self.Tt2o), _lib.ptr(self.state),
+                  _lib.ptr(self.a), _lib.ptr(self.b), _lib.ptr(self.dc), _lib.ptr(self.f0),
+                  _lib.ptr(self.K[0]), _lib.ptr(self.K[1]), _lib.ptr(self.K[2]),
+                  _lib.ptr(self.shear_loading), _lib.ptr(self.slipv), _lib.ptr(self.cellpar),
+                  _lib.ptr(self.y[0]), _lib.ptr(self.y[1]), _lib.ptr(self.y[2]),
+                  _lib.ptr(self.y[3]), _lib.ptr(self.err), self.a63, 1, s)
+        self.err_host.copy_(self.err, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        e = self.err_host.numpy().view(np.float64)
+        self.nattempts += 1
+        return float(e[0]), float(e[1]), float(e[2]), float(e[3])
+
+    def _commit(self, h):
+        self.hdev.fill_(h)
+        _lib.call("pq3d_rk45_commit", self.n, 0, _lib.ptr(self.hdev), 0, _lib.ptr(self.a),
+                  _lib.ptr(self.cellpar), _lib.ptr(self.y[0]), _lib.ptr(self.y[1]),
+                  _lib.ptr(self.y[2]), _lib.ptr(self.y[3]), _lib.ptr(self.Tno),
+                  _lib.ptr(self.Tt1o), _lib.ptr(self.Tt2o), _lib.ptr(self.state),
+                  _lib.ptr(self.slipv1), _lib.ptr(self.slipv2), _lib.ptr(self.slipv),
+                  _lib.ptr(self.rake), _lib.ptr(self.slip1), _lib.ptr(self.slip2),
+                  _lib.ptr(self.vmax), 1, self._stream())
+
+    # ---------------------------------------------------- reference surface
+    def RungeKutte_solve_Dormand_Prince(self, h):
+        """One attempt (src/QDsim.py:1001-1134): returns
+        (condition1, condition2, hnew1, hnew2); the candidate stays on device."""
+        e_state, e_t1, e_t2, e_s = self._attempt(h)
+        self.Relerrormax1 = np.float64(e_state) + 1e-10
+        self.Relerrormax2 = max(np.float64(e_t1), np.float64(e_t2), np.float64(e_s)) + 1e-10
+        RelTol1 = RelTol2 = self.reltol
+        condition1 = self.Relerrormax1 / RelTol1
+        condition2 = self.Relerrormax2 / RelTol2
+        hnew1 = h * 0.9 * (RelTol1 / self.Relerrormax1) ** 0.2
+        hnew2 = h * 0.9 * (RelTol2 / self.Relerrormax2) ** 0.2
+        return condition1, condition2, hnew1, hnew2
+
+    def simu_forward(self, dttry):
+        """One accepted time step (src/QDsim.py:892-996): returns (h, dtnext)."""
+        self.compute_K()
+        nrjct = 0
+        h = dttry
+        while True:
+            condition1, condition2, hnew1, hnew2 = self.RungeKutte_solve_Dormand_Prince(h)
+            if max(condition1, condition2) < 1.0 and not (np.isnan(condition1) or np.isnan(condition2)):
+                dtnext = min(hnew1, hnew2)
+                dtnext = min(1.5 * h, dtnext)
+                self.Relerrormax1_last = self.Relerrormax1
+                self.Relerrormax2_last = self.Relerrormax2
+                break
+            nrjct = nrjct + 1
+            self.nrejects += 1
+            dtnext = min(hnew1, hnew2)
+            h = max(0.5 * h, dtnext)
+            if h < 1.e-15 or nrjct > 20:
+                raise StepAborted(f"dt is too small (h={h:g}, rejections={nrjct})")
+        self.time = self.time + h
+        self._commit(h)
+        self.nsteps += 1
+        return h, dtnext
+
+    def derivative(self, Tno, Tt1o, Tt2o, state):
+        """RHS at the given cell arrays (device tensors, cluster order) with the
+        current frozen K (src/QDsim.py:760-813): (dstatedt, dsigmadt/1e6,
+        dtau1dt/1e6, dtau2dt/1e6)."""
+        out = torch.empty((4, self.n), dtype=torch.float64, device=self.device)
+        _lib.call("pq3d_qd_rhs", self.n, _lib.ptr(Tno), _lib.ptr(Tt1o), _lib.ptr(Tt2o),
+                  _lib.ptr(state), _lib.ptr(self.a), _lib.ptr(self.b), _lib.ptr(self.dc),
+                  _lib.ptr(self.f0), _lib.ptr(self.K[0]), _lib.ptr(self.K[1]),
+                  _lib.ptr(self.K[2]), _lib.ptr(self.shear_loading), _lib.ptr(self.slipv),
+                  self.V0, self.mu, self.Cs, _lib.ptr(out), self._stream())
+        return out[0], out[1], out[2], out[3]
+
+    # ------------------------------------------------------------- helpers
+    def max_slipv(self) -> float:
+        return float(self.slipv.max())
+
+    def to_original(self, x: torch.Tensor) -> np.ndarray:
+        out = np.empty(self.n)
+        out[self.perm] = x.detach().cpu().numpy()
+        return out
+
+    def snapshot(self) -> dict:
+        g = self.to_original
+        return dict(time=self.time, Tno=g(self.Tno), Tt1o=g(self.Tt1o), Tt2o=g(self.Tt2o),
+                    state=g(self.state), slipv1=g(self.slipv1), slipv2=g(self.slipv2),
+                    slipv=g(self.slipv), rake=g(self.rake), slip1=g(self.slip1),
+                    slip2=g(self.slip2))
+
+    def run(self, nsteps: int, dttry=None, log=None, vmax_every=1):
+        """Run ``nsteps`` accepted steps; returns rows (step, h, t, maxV)."""
+        dt = self.htry if dttry is None else dttry
+        rows = []
+        for i in range(nsteps):
+            h, dt = self.simu_forward(dt)
+            if vmax_every and (i % vmax_every == 0):
+                rows.append((self.nsteps, h, self.time, self.max_slipv()))
+        self.htry = dt
+        return rows

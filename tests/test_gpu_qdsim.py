@@ -1,0 +1,77 @@
+"""GPU time step (fused RK45 attempt, controller, commit) against the CPU
+oracle of the reference step on the same dense FP64 operators."""
+import numpy as np
+import pytest
+import torch
+
+from oracle.qdsim_oracle import OracleSim, dense_operators
+from pyquake3d_amd.mesh import get_eleVec
+from pyquake3d_amd.problem import planar_c1
+from pyquake3d_amd.qdsim import DenseOperator, QDsim
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(oracle):
+    p = planar_c1(nx=12, nz=6, seed=0)
+    ev, xg = get_eleVec(p.mesh.nodelst, p.mesh.elelst)
+    K = dense_operators(oracle, xg, ev, p.mesh.triangles(), p.mu, p.lambda_)
+    return p, K
+
+
+def _pair(p, K, gpu, tableau="ref"):
+    op = DenseOperator(p.mesh, p.mu, p.lambda_, device=gpu)
+    sim = QDsim(p, op, device=gpu, tableau=tableau)
+    ora = OracleSim(K, p.a, p.b, p.dc, p.f0, p.V0, p.mu, p.Cs, p.Vpl, p.rake, p.initial_state(),
+                    tableau=tableau)
+    return sim, ora
+
+
+def test_dense_operator_matches_oracle(gpu, pq3d_lib, small):
+    p, K = small
+    op = DenseOperator(p.mesh, p.mu, p.lambda_, device=gpu)
+    rng = np.random.default_rng(0)
+    v1, v2 = rng.standard_normal(p.n), rng.standard_normal(p.n)
+    y = op.matvec(torch.from_numpy(v1).to(gpu), torch.from_numpy(v2).to(gpu)).cpu().numpy()
+    ref = np.stack([K["A1s"] @ v1 + K["A1d"] @ v2, K["A2s"] @ v1 + K["A2d"] @ v2,
+                    K["Bs"] @ v1 + K["Bd"] @ v2])
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_derivative_matches_oracle(gpu, pq3d_lib, small):
+    p, K = small
+    sim, ora = _pair(p, K, gpu)
+    sim.compute_K()
+    ora.simu_forward(1e-3)  # sets ora.AdotV1 ... (state before the step is what we compare)
+    ora2 = OracleSim(K, p.a, p.b, p.dc, p.f0, p.V0, p.mu, p.Cs, p.Vpl, p.rake,
+                     p.initial_state())
+    ora2.AdotV1, ora2.AdotV2, ora2.dsigmadt = ora.AdotV1, ora.AdotV2, ora.dsigmadt
+    rng = np.random.default_rng(1)
+    Tno = ora2.Tno * (1 + 0.01 * rng.standard_normal(p.n))
+    Tt1 = ora2.Tt1o * (1 + 0.01 * rng.standard_normal(p.n))
+    Tt2 = 0.01 * ora2.Tt1o * rng.standard_normal(p.n)
+    st = ora2.state + 0.001 * rng.standard_normal(p.n)
+    ref = ora2.derivative(Tno, Tt1, Tt2, st)
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(gpu)
+    got = [x.cpu().numpy() for x in sim.derivative(d(Tno), d(Tt1), d(Tt2), d(st))]
+    for g, r in zip(got, ref):
+        assert np.abs(g - r).max() <= 1e-12 * np.abs(r).max() + 1e-300
+
+
+@pytest.mark.parametrize("tableau", ["ref", "dp5"])
+def test_trajectory_matches_oracle(gpu, pq3d_lib, small, tableau):
+    """300 accepted steps from the seeded nucleation state: step sizes, times
+    and slip-rate fields agree to near rounding."""
+    p, K = small
+    sim, ora = _pair(p, K, gpu, tableau)
+    dt_g = dt_o = 1e-3
+    for i in range(300):
+        hg, dt_g = sim.simu_forward(dt_g)
+        ho, dt_o = ora.simu_forward(dt_o)
+        assert hg == pytest.approx(ho, rel=1e-8), i
+    assert sim.time == pytest.approx(ora.time, rel=1e-8)
+    snap = sim.snapshot()
+    for k in ("Tno", "Tt1o", "state", "slipv"):
+        ref = getattr(ora, k)
+        assert np.abs(snap[k] - ref).max() <= 1e-7 * np.abs(ref).max(), k

@@ -1,0 +1,117 @@
+"""Host logic of the H-matrix on the CPU: cluster tree, block partition,
+pool layout and matvec work lists. The two matvec kernels are emulated in
+numpy item by item (same indexing as pyquake3d_amd/csrc/hmat_kernels.hip) over
+a pool filled from the dense FP64 oracle (near field) and truncated SVDs of the
+stacked 3m x 2n blocks (far field), and compared with the dense product."""
+import numpy as np
+import pytest
+
+from pyquake3d_amd.cluster import build_block_partition, build_cluster_tree
+from pyquake3d_amd.hmatrix import build_worklists, plan_layout
+from pyquake3d_amd.mesh import get_eleVec, undulating_fault
+
+MU = LAM = 32e9
+
+
+def test_cluster_tree_partitions_and_is_contiguous():
+    mesh = undulating_fault(length=30e3, depth=10e3, nx=40, nz=15, seed=3)
+    _, xg = get_eleVec(mesh.nodelst, mesh.elelst)
+    t = build_cluster_tree(xg, leaf=32)
+    assert sorted(t.perm.tolist()) == list(range(mesh.n))
+    leaves = t.leaves()
+    assert (t.size[leaves] <= 32).all()
+    # leaves tile [0, n) without gaps
+    s = t.start[leaves]
+    assert s[0] == 0 and (np.diff(s) == t.size[leaves][:-1]).all()
+    assert s[-1] + t.size[leaves][-1] == mesh.n
+    # every internal node is split at the bbox midpoint of its longest extent
+    for k in range(t.n_nodes):
+        if t.left[k] < 0:
+            continue
+        pts = xg[t.perm[t.start[k]:t.start[k] + t.size[k]]]
+        d = int(np.argmax(pts.max(0) - pts.min(0)))
+        g = (pts[:, d].min() + pts[:, d].max()) / 2
+        L = xg[t.perm[t.start[t.left[k]]:t.start[t.left[k]] + t.size[t.left[k]]]]
+        R = xg[t.perm[t.start[t.right[k]]:t.start[t.right[k]] + t.size[t.right[k]]]]
+        assert (L[:, d] <= g).all() and (R[:, d] > g).all()
+
+
+def test_block_partition_covers_matrix_once():
+    mesh = undulating_fault(length=30e3, depth=10e3, nx=40, nz=15, seed=3)
+    _, xg = get_eleVec(mesh.nodelst, mesh.elelst)
+    t = build_cluster_tree(xg, leaf=32)
+    P = build_block_partition(t, eta=2.0)
+    cover = np.zeros((mesh.n, mesh.n), dtype=np.int32)
+    for a, b in zip(P.rows, P.cols):
+        cover[t.start[a]:t.start[a] + t.size[a], t.start[b]:t.start[b] + t.size[b]] += 1
+    assert (cover == 1).all()
+    assert P.adm.any() and (~P.adm).any()
+
+
+def emulate_matvec(W, pool, v1, v2, n):
+    t = np.zeros(max(W["nt"], 1))
+    for vb, nb, c0, ti in W["vrows"]:
+        v = pool[vb:vb + 2 * nb]
+        t[ti] = v[:nb] @ v1[c0:c0 + nb] + v[nb:] @ v2[c0:c0 + nb]
+    y = np.zeros((3, n))
+    items, ptr = W["items"], W["tile_ptr"]
+    for q, (R0, mt) in enumerate(W["tiles"]):
+        lanes = np.arange(mt)
+        for it in items[ptr[q]:ptr[q + 1]]:
+            kind, base, ld, plane, cnt, xo = it
+            for c in range(cnt):
+                p = base + lanes + c * ld
+                if kind == 0:
+                    x1, x2 = v1[xo + c], v2[xo + c]
+                    y[0, R0:R0 + mt] += pool[p] * x1 + pool[p + 3 * plane] * x2
+                    y[1, R0:R0 + mt] += pool[p + plane] * x1 + pool[p + 4 * plane] * x2
+                    y[2, R0:R0 + mt] += pool[p + 2 * plane] * x1 + pool[p + 5 * plane] * x2
+                else:
+                    tl = t[xo + c]
+                    y[0, R0:R0 + mt] += pool[p] * tl
+                    y[1, R0:R0 + mt] += pool[p + plane] * tl
+                    y[2, R0:R0 + mt] += pool[p + 2 * plane] * tl
+    return y
+
+
+@pytest.mark.parametrize("leaf", [32, 64])
+def test_layout_and_worklists_reproduce_dense_product(oracle, leaf):
+    mesh = undulating_fault(length=40e3, depth=12e3, nx=44, nz=18, seed=4)
+    ev, xg = get_eleVec(mesh.nodelst, mesh.elelst)
+    n = mesh.n
+    t = build_cluster_tree(xg, leaf=leaf)
+    P = build_block_partition(t, eta=2.0)
+    perm = t.perm
+    K = oracle.traction_block(xg, ev, mesh.triangles(), perm, perm, MU, LAM)  # cluster order
+    r0, m = t.start[P.rows], t.size[P.rows]
+    c0, nn = t.start[P.cols], t.size[P.cols]
+    lr_idx = np.where(P.adm)[0]
+    ranks, fac = [], {}
+    for bi in lr_idx:
+        a, b = slice(r0[bi], r0[bi] + m[bi]), slice(c0[bi], c0[bi] + nn[bi])
+        S = np.block([[K[0][a, b], K[3][a, b]], [K[1][a, b], K[4][a, b]], [K[2][a, b], K[5][a, b]]])
+        U, s, Vt = np.linalg.svd(S, full_matrices=False)
+        k = int((s > 1e-9 * s[0]).sum())
+        ranks.append(k)
+        fac[bi] = (U[:, :k] * s[:k], Vt[:k])
+    ranks = np.array(ranks)
+    L = plan_layout(r0, m, c0, nn, P.adm, lr_idx, ranks)
+    pool = np.zeros(L["total"])
+    for bi, base in zip(L["dense_idx"], L["d_base"]):
+        bm, bn = m[bi], nn[bi]
+        for op in range(6):
+            blk = K[op][r0[bi]:r0[bi] + bm, c0[bi]:c0[bi] + bn]
+            pool[base + op * bm * bn: base + (op + 1) * bm * bn] = blk.T.ravel()  # column-major
+    for bi, ub, vb, k in zip(L["lr_idx"], L["l_ub"], L["l_vb"], L["k"]):
+        U, V = fac[bi]
+        pool[ub:ub + U.size] = U.T.ravel()   # column l of U contiguous
+        pool[vb:vb + V.size] = V.ravel()     # row l of V contiguous
+    W = build_worklists(t, n, r0, m, c0, nn, L["dense_idx"], L["d_base"], L["lr_idx"],
+                        L["l_ub"], L["l_vb"], L["k"])
+    rng = np.random.default_rng(0)
+    v1, v2 = rng.standard_normal(n), rng.standard_normal(n)
+    y = emulate_matvec(W, pool, v1, v2, n)
+    ref = np.stack([K[0] @ v1 + K[3] @ v2, K[1] @ v1 + K[4] @ v2, K[2] @ v1 + K[5] @ v2])
+    scale = np.abs(K).max() * np.abs(np.concatenate([v1, v2])).sum()
+    assert np.abs(y - ref).max() <= 1e-8 * scale
+    assert len(L["lr_idx"]) > 0 and len(L["dense_idx"]) > 0
