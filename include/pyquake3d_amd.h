@@ -94,6 +94,96 @@ int pq3d_tde_traction_block_f32(const double* d_recv, const double* d_rec,
                                 double lambda_, float* d_out, int64_t si, int64_t sj,
                                 int64_t plane, int64_t out_len, void* stream);
 
+/* ------------------------------------------------------------ H-matrix
+ * Index space: cluster order (the cluster tree's permutation); blocks are
+ * contiguous ranges. Pool element type: float (dtype_bytes 4) or double (8).
+ * Stacked block operator: K[c*m+i, s*n+j] = op(c,s)[i,j] with
+ * op(0,0)=A1s op(1,0)=A2s op(2,0)=Bs op(0,1)=A1d op(1,1)=A2d op(2,1)=Bd. */
+
+/* Near-field blocks (replaces ACA_worker's dense branch with
+ * calc_stressfunc_fromCS/CD, src/Hmatrix.py:1049-1064, :425-511).
+ * d_tab: nblocks x 6 int64 (r0, m, c0, n, pool base, unused); block data is
+ * six column-major m x n planes at d_pool[base + op*m*n + j*m + i]. */
+int pq3d_hmat_dense_assemble(const int64_t* d_tab, int64_t nblocks, const double* d_recv,
+                             const double* d_rec, double mu, double lambda_, void* d_pool,
+                             int dtype_bytes, void* stream);
+
+/* Far-field ACA (replaces ACA_plus_full_traceS/D, src/Hmatrix.py:581-1011,
+ * with a relative Frobenius stop): one workgroup per block of the stacked
+ * 3m x 2n operator; entries are evaluated in the kernel. d_tab: nblocks x 8
+ * int64 (r0, m, c0, n, -, -, rank cap, -); d_wsoff: nblocks x 4 int64
+ * offsets into d_wU (cap*3m doubles per block), d_wV (cap*2n) and d_flags
+ * (3m+2n bytes). d_rank[b] = rank, or -1 if the cap was reached. */
+int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
+                  const double* d_recv, const double* d_rec, double mu, double lambda_,
+                  double eps, int kmax, double* d_wU, double* d_wV, uint8_t* d_flags,
+                  int64_t* d_rank, void* stream);
+
+/* Copy converged factors into the pool: U (3m x k, column-major) at
+ * d_tab[b][4], V (k x 2n, row-major) at d_tab[b][5], k = d_tab[b][6]. */
+int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
+                       const double* d_wU, const double* d_wV, void* d_pool, int dtype_bytes,
+                       void* stream);
+
+/* Six-operator H-matvec (replaces the six blocks_process_MVM calls of one
+ * step, src/Hmatrix.py:1445-1538 via src/QDsim.py:906-917):
+ * y1 = A1s v1 + A1d v2, y2 = A2s v1 + A2d v2, y3 = Bs v1 + Bd v2 (FP64
+ * vectors, FP64 accumulation). Work lists as built by
+ * pyquake3d_amd.hmatrix.build_worklists; d_t: nvrows doubles of scratch. */
+int pq3d_hmat_matvec(const int64_t* d_vrows, int64_t nvrows, const int64_t* d_tiles,
+                     const int64_t* d_tile_ptr, int64_t ntiles, const int64_t* d_items,
+                     const void* d_pool, int dtype_bytes, const double* d_v1,
+                     const double* d_v2, double* d_t, double* d_y1, double* d_y2, double* d_y3,
+                     void* stream);
+
+/* ---------------------------------------------------------- time step
+ * Cells may belong to n_inst independent instances (d_inst[i], or NULL for
+ * one instance); d_h, d_cellpar (V0, mu, Cs per instance), d_err (4 per
+ * instance) and d_vmax are indexed by instance. Error and max slots are
+ * uint64 bit patterns of non-negative doubles (zeroed by the calls). */
+
+/* Slip-rate deficit (src/QDsim.py:902-903): x = V - Vpl (cos, sin)(rake0). */
+int pq3d_deficit(int64_t n, const double* d_slipv1, const double* d_slipv2,
+                 const double* d_pl1, const double* d_pl2, double* d_x1, double* d_x2,
+                 void* stream);
+
+/* dsigma/dt = K part + normal loading (src/QDsim.py:911-913); fix_tn: loading only. */
+int pq3d_add_loading(int64_t n, double* d_dsig, const double* d_load, int fix_tn,
+                     void* stream);
+
+/* Traction-rate RHS (replaces QDsim.derivative, src/QDsim.py:760-813):
+ * d_out = 4 x n (dpsi/dt, dsigma/dt, dtau1/dt, dtau2/dt in MPa/s). */
+int pq3d_qd_rhs(int64_t n, const double* d_Tno, const double* d_Tt1o, const double* d_Tt2o,
+                const double* d_state, const double* d_a, const double* d_b, const double* d_dc,
+                const double* d_f0, const double* d_AdotV1, const double* d_AdotV2,
+                const double* d_dsigmadt, const double* d_shear_loading, const double* d_slipv,
+                double V0, double mu, double Cs, double* d_out, void* stream);
+
+/* One fused Dormand-Prince 5(4) attempt of step d_h[inst] (replaces
+ * RungeKutte_solve_Dormand_Prince, src/QDsim.py:1001-1134): candidate state
+ * to d_y*, error maxima (psi, tau1, tau2, sigma) to d_err. a63 is the
+ * tableau's a63 (the reference uses -46732/5247, the published value is
+ * +46732/5247). */
+int pq3d_rk45_attempt(int64_t n, const int32_t* d_inst, const double* d_h, const double* d_Tno,
+                      const double* d_Tt1o, const double* d_Tt2o, const double* d_state,
+                      const double* d_a, const double* d_b, const double* d_dc,
+                      const double* d_f0, const double* d_AdotV1, const double* d_AdotV2,
+                      const double* d_dsigmadt, const double* d_shear_loading,
+                      const double* d_slipv, const double* d_cellpar, double* d_yTno,
+                      double* d_yTt1o, double* d_yTt2o, double* d_ystate,
+                      unsigned long long* d_err, double a63, int64_t n_inst, void* stream);
+
+/* Accept the candidate (src/QDsim.py:960-993) for instances with
+ * d_accept[inst] != 0 (all when NULL): state, slip rates from the
+ * regularized law per component, rake, V floor 1e-30, slip; d_vmax = max V. */
+int pq3d_rk45_commit(int64_t n, const int32_t* d_inst, const double* d_h,
+                     const uint8_t* d_accept, const double* d_a, const double* d_cellpar,
+                     const double* d_yTno, const double* d_yTt1o, const double* d_yTt2o,
+                     const double* d_ystate, double* d_Tno, double* d_Tt1o, double* d_Tt2o,
+                     double* d_state, double* d_slipv1, double* d_slipv2, double* d_slipv,
+                     double* d_rake, double* d_slip1, double* d_slip2,
+                     unsigned long long* d_vmax, int64_t n_inst, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
