@@ -106,17 +106,38 @@ def check(rc: int, what: str = "") -> None:
                            f"{msg.decode() if msg else ''}")
 
 
+class DevPtr(c_void_p):
+    """A device/host address. Pointer parameters only accept DevPtr (or
+    None / 0 for a null pointer) and integer/float parameters reject it, so an
+    argument list that is out of order fails before anything is launched."""
+
+
+def _checked_args(name: str, fn, args):
+    types = fn.argtypes
+    if len(args) != len(types):
+        raise TypeError(f"{name}: expected {len(types)} arguments, got {len(args)}")
+    for k, (a, t) in enumerate(zip(args, types)):
+        if t is c_void_p:
+            if not (isinstance(a, DevPtr) or a is None or (isinstance(a, int) and a == 0)):
+                raise TypeError(f"{name}: argument {k} must be a pointer (DevPtr/None), "
+                                f"got {type(a).__name__}")
+        elif t in (c_int64, c_int, c_size_t, c_double, c_float, c_bool):
+            if isinstance(a, c_void_p):
+                raise TypeError(f"{name}: argument {k} must be a {t.__name__}, got a pointer")
+    return args
+
+
 def call(name: str, *args) -> None:
     fn = getattr(load(), name)
-    check(fn(*args), name)
+    check(fn(*_checked_args(name, fn, args)), name)
 
 
-def ptr(t) -> int:
-    """Device (or host) address of a torch tensor, or 0 for None."""
-    return 0 if t is None else int(t.data_ptr())
+def ptr(t):
+    """Device (or host) address of a torch tensor as a DevPtr, or None."""
+    return None if t is None else DevPtr(int(t.data_ptr()))
 
 
-def stream_handle(stream=None) -> int:
+def stream_handle(stream=None):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
-    return int(s.cuda_stream)
+    return DevPtr(int(s.cuda_stream)) if s.cuda_stream else None

@@ -35,6 +35,11 @@ DENSE_CHUNK = 32   # columns per dense work item
 LR_CHUNK = 16      # rank columns per low-rank work item
 
 
+def _sync(dev):
+    if dev.type == "cuda":
+        torch.cuda.synchronize(dev)
+
+
 @dataclass
 class HStats:
     n: int
@@ -207,7 +212,7 @@ class HMatrix:
         _lib.call("pq3d_hmat_dense_assemble", _lib.ptr(self.dtab), len(dense_idx),
                   _lib.ptr(self.recv), _lib.ptr(self.rec), self.mu, self.lambda_,
                   _lib.ptr(self.pool), self.esize, stream)
-        torch.cuda.synchronize(dev)
+        _sync(dev)
         tm["dense"] = time.perf_counter() - t0
 
         # pack low-rank factors (per ACA batch)
@@ -224,11 +229,12 @@ class HMatrix:
                 ltab[q] = (r0[gb], m[gb], c0[gb], nn[gb], ub_of.get(gb, 0), vb_of.get(gb, 0),
                            kq if ok else 0, 0)
             lt = torch.from_numpy(ltab).to(dev)
-            _lib.call("pq3d_hmat_aca_pack", _lib.ptr(lt), len(sel), _lib.ptr(wsoff),
+            _lib.call("pq3d_hmat_aca_pack", _lib.ptr(lt), _lib.ptr(wsoff), len(sel),
                       _lib.ptr(wsU), _lib.ptr(wsV), _lib.ptr(self.pool), self.esize, stream)
-        torch.cuda.synchronize(dev)
+        _sync(dev)
         del lr_results
-        torch.cuda.empty_cache()
+        if dev.type == "cuda":
+            torch.cuda.empty_cache()
         tm["pack"] = time.perf_counter() - t0
 
         # ---------------- matvec work lists

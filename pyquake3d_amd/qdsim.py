@@ -111,7 +111,9 @@ class QDsim:
         self.hdev = torch.zeros(1, dtype=torch.float64, device=dev)
         self.err = torch.zeros(4, dtype=torch.int64, device=dev)
         self.vmax = torch.zeros(1, dtype=torch.int64, device=dev)
-        self.err_host = torch.zeros(4, dtype=torch.int64).pin_memory() if torch.cuda.is_available() else torch.zeros(4, dtype=torch.int64)
+        self.err_host = torch.zeros(4, dtype=torch.int64)
+        if dev.type == "cuda":
+            self.err_host = self.err_host.pin_memory()
         self.time = 0.0
         self.htry = 1e-3
         self.maxslipv0 = float(np.max(slipv))
@@ -147,7 +149,8 @@ class QDsim:
                   _lib.ptr(self.y[0]), _lib.ptr(self.y[1]), _lib.ptr(self.y[2]),
                   _lib.ptr(self.y[3]), _lib.ptr(self.err), self.a63, 1, s)
         self.err_host.copy_(self.err, non_blocking=True)
-        torch.cuda.current_stream().synchronize()
+        if self.device.type == "cuda":
+            torch.cuda.current_stream().synchronize()
         e = self.err_host.numpy().view(np.float64)
         self.nattempts += 1
         return float(e[0]), float(e[1]), float(e[2]), float(e[3])

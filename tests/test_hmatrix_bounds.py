@@ -1,0 +1,165 @@
+"""Host-side bounds check of every device access the H-matrix kernels make.
+
+HMatrix runs its real host pipeline on CPU tensors with the kernel layer
+replaced by a checker: for each launch it recomputes, from the same tables the
+kernel reads, the extreme index of every buffer access and asserts it lies
+inside the buffer (plus a random ACA rank outcome, including non-converged
+blocks). Mirrors the index arithmetic of pyquake3d_amd/csrc/hmat_kernels.hip."""
+import numpy as np
+import pytest
+import torch
+
+from pyquake3d_amd import _lib
+from pyquake3d_amd import hmatrix as hm
+from pyquake3d_amd.mesh import undulating_fault
+
+
+class Checker:
+    def __init__(self, rng, fail_frac=0.1):
+        self.reg = {}
+        self.rng = rng
+        self.fail_frac = fail_frac
+        self.aca_caps = {}
+        self.calls = []
+
+    def ptr(self, t):
+        if t is None:
+            return None
+        key = len(self.reg) + 1
+        self.reg[key] = t
+        return _lib.DevPtr(key)
+
+    def T(self, k):
+        return self.reg[k.value]
+
+    def call(self, name, *a):
+        # the real argument-kind check of the binding, then the bounds check
+        _lib._checked_args(name, getattr(_lib.load(), name), a)
+        self.calls.append(name)
+        getattr(self, name)(*a)
+
+    def pq3d_hmat_dense_assemble(self, tab, nb, recv, rec, mu, lam, pool, esize, stream):
+        tab = self.T(tab).numpy()
+        N = self.T(recv).shape[0]
+        assert self.T(rec).shape[0] == N and tab.shape == (nb, 6)
+        P = self.T(pool).numel()
+        for r0, m, c0, n, base, _ in tab:
+            assert 0 <= r0 and r0 + m <= N and 0 <= c0 and c0 + n <= N and m > 0 and n > 0
+            assert base >= 0 and base + 6 * m * n <= P
+
+    def pq3d_hmat_aca(self, tab, wsoff, nb, recv, rec, mu, lam, eps, kmax, wU, wV, flags, rank,
+                      stream):
+        tab_a = self.T(tab).numpy()
+        ws = self.T(wsoff).numpy()
+        N = self.T(recv).shape[0]
+        nU, nV, nF = self.T(wU).numel(), self.T(wV).numel(), self.T(flags).numel()
+        r = np.empty(nb, dtype=np.int64)
+        for b in range(nb):
+            r0, m, c0, n = tab_a[b, :4]
+            kcap = min(int(tab_a[b, 6]), kmax)
+            assert kcap >= 1
+            assert r0 + m <= N and c0 + n <= N
+            assert ws[b, 0] + kcap * 3 * m <= nU
+            assert ws[b, 1] + kcap * 2 * n <= nV
+            assert ws[b, 2] + 3 * m + 2 * n <= nF
+            r[b] = -1 if self.rng.random() < self.fail_frac else self.rng.integers(1, kcap + 1)
+            self.aca_caps[(int(r0), int(c0))] = kcap
+        self.T(rank)[:] = torch.from_numpy(r)
+
+    def pq3d_hmat_aca_pack(self, tab, wsoff, nb, wU, wV, pool, esize, stream):
+        tab_a = self.T(tab).numpy()
+        ws = self.T(wsoff).numpy()
+        P = self.T(pool).numel()
+        for b in range(nb):
+            r0, m, c0, n, ub, vb, k = tab_a[b, :7]
+            if k <= 0:
+                continue
+            assert k <= self.aca_caps[(int(r0), int(c0))]
+            assert ws[b, 0] + 3 * m * k <= self.T(wU).numel()
+            assert ws[b, 1] + 2 * n * k <= self.T(wV).numel()
+            assert 0 <= ub and ub + 3 * m * k <= P and 0 <= vb and vb + 2 * n * k <= P
+
+    def pq3d_hmat_matvec(self, vrows, nvr, tiles, tile_ptr, ntiles, items, pool, esize, v1, v2,
+                         t, y1, y2, y3, stream):
+        P = self.T(pool).numel()
+        N = self.T(v1).numel()
+        nt = self.T(t).numel()
+        if nvr:
+            for vb, n, c0, ti in self.T(vrows).numpy():
+                assert vb >= 0 and vb + 2 * n <= P and c0 + n <= N and 0 <= ti < nt
+        tl = self.T(tiles).numpy()
+        tp = self.T(tile_ptr).numpy()
+        it = self.T(items).numpy()
+        assert tl.shape == (ntiles, 2) and tp.shape == (ntiles + 1,) and tp[-1] == len(it)
+        for q in range(ntiles):
+            R0, mt = tl[q]
+            assert 0 < mt <= 64 and R0 + mt <= N
+            for kind, base, ld, plane, cnt, xo in it[tp[q]:tp[q + 1]]:
+                assert cnt > 0 and base >= 0
+                hi = base + (cnt - 1) * ld + (mt - 1)
+                if kind == 0:
+                    assert hi + 5 * plane < P and xo >= 0 and xo + cnt <= N
+                else:
+                    assert hi + 2 * plane < P and xo >= 0 and xo + cnt <= nt
+
+
+@pytest.mark.parametrize("leaf,eta,dtype", [(32, 2.0, torch.float32), (64, 2.0, torch.float64),
+                                            (32, 0.0, torch.float64)])
+def test_hmatrix_device_accesses_in_bounds(monkeypatch, leaf, eta, dtype):
+    chk = Checker(np.random.default_rng(leaf))
+    monkeypatch.setattr(_lib, "call", chk.call)
+    monkeypatch.setattr(_lib, "ptr", chk.ptr)
+    monkeypatch.setattr(_lib, "stream_handle", lambda s=None: None)
+    mesh = undulating_fault(length=40e3, depth=12e3, nx=44, nz=18, seed=4)
+    H = hm.HMatrix.from_mesh(mesh, 32e9, 32e9, leaf=leaf, eta=eta, eps=1e-4, dtype=dtype,
+                             device="cpu", aca_budget_gb=0.002)
+    x = torch.zeros(mesh.n, dtype=torch.float64)
+    H.matvec(x, x)
+    assert "pq3d_hmat_matvec" in chk.calls
+    if eta > 0:
+        assert chk.calls.count("pq3d_hmat_aca") > 1   # the workspace budget forced batches
+
+
+def test_tde_and_blocktree_call_argument_kinds(monkeypatch):
+    """Every Python path into the library passes pointers where the C-ABI
+    takes pointers and integers/doubles elsewhere (checked by the binding
+    itself), and the traction block's output bound matches the tensor."""
+    from pyquake3d_amd.blocktree import createHmatrix
+    from pyquake3d_amd.mesh import get_eleVec
+    from pyquake3d_amd.tde import DeviceMesh
+    seen = []
+    reg = {}
+
+    def ptr(t):
+        if t is None:
+            return None
+        reg[len(reg) + 1] = t
+        return _lib.DevPtr(len(reg))
+
+    def call(name, *a):
+        _lib._checked_args(name, getattr(_lib.load(), name), a)
+        seen.append(name)
+        if name.startswith("pq3d_tde_traction_block"):
+            out = reg[a[10].value]
+            m, n, si, sj, plane, out_len = a[4], a[7], a[11], a[12], a[13], a[14]
+            assert out_len == out.numel()
+            assert 5 * plane + (m - 1) * si + (n - 1) * sj < out_len
+        if name == "pq3d_hmat_aca":
+            reg[a[12].value][:] = 3
+
+    monkeypatch.setattr(_lib, "call", call)
+    monkeypatch.setattr(_lib, "ptr", ptr)
+    monkeypatch.setattr(_lib, "stream_handle", lambda s=None: None)
+    mesh = undulating_fault(length=12e3, depth=6e3, nx=12, nz=6, seed=1)
+    dm = DeviceMesh.from_mesh(mesh, 32e9, 32e9, device="cpu")
+    dm.traction_block()
+    dm.traction_block([0, 3, 5], [1, 2])
+    dm.traction_block(None, None, row0=2, m=7, col0=3, n=9, order="ji")
+    dm.traction_block([1, 2], [3], dtype=torch.float32)
+    ev, xg = get_eleVec(mesh.nodelst, mesh.elelst)
+    tree = createHmatrix(xg, mesh.nodelst, mesh.elelst, ev, 32e9, 32e9, False, mini_leaf=8,
+                         device="cpu")
+    tree.dm.traction_block = lambda r, c: torch.zeros((6, len(r), len(c)), dtype=torch.float64)
+    for b in tree.blocks_to_process:
+        tree.ACA_worker(b)
+    assert "pq3d_hmat_aca" in seen and seen.count("pq3d_tde_traction_block") == 3

@@ -1,0 +1,121 @@
+"""Host logic of the GPU stepper on the CPU: the controller and the kernel
+argument lists. The kernel layer is replaced by a checker that validates every
+pointer argument's buffer length against what the kernel indexes (per-cell
+arrays >= n, per-instance slots) and emulates the attempt with the oracle's
+arithmetic, so the accept/reject controller runs exactly as on the GPU."""
+import numpy as np
+import pytest
+import torch
+
+from oracle.qdsim_oracle import OracleSim
+from pyquake3d_amd import _lib
+from pyquake3d_amd.problem import planar_c1
+from pyquake3d_amd.qdsim import QDsim
+
+
+class IdentityOp:
+    def __init__(self, n, K):
+        self.perm = np.arange(n)
+        self.K = K
+
+    def matvec(self, v1, v2, out=None):
+        K = self.K
+        v1, v2 = v1.numpy(), v2.numpy()
+        y = np.stack([K["A1s"] @ v1 + K["A1d"] @ v2, K["A2s"] @ v1 + K["A2d"] @ v2,
+                      K["Bs"] @ v1 + K["Bd"] @ v2])
+        out.copy_(torch.from_numpy(y))
+        return out
+
+
+class StepChecker:
+    def __init__(self, ora):
+        self.reg = {}
+        self.ora = ora
+
+    def ptr(self, t):
+        if t is None:
+            return None
+        key = len(self.reg) + 1
+        self.reg[key] = t
+        return _lib.DevPtr(key)
+
+    def T(self, p):
+        return None if p is None or p == 0 else self.reg[p.value]
+
+    def call(self, name, *a):
+        _lib._checked_args(name, getattr(_lib.load(), name), a)
+        getattr(self, name)(*a)
+
+    def _cells(self, n, *ps):
+        for p in ps:
+            assert self.T(p).numel() >= n
+
+    def pq3d_deficit(self, n, s1, s2, p1, p2, x1, x2, stream):
+        self._cells(n, s1, s2, p1, p2, x1, x2)
+        self.T(x1)[:n] = self.T(s1)[:n] - self.T(p1)[:n]
+        self.T(x2)[:n] = self.T(s2)[:n] - self.T(p2)[:n]
+
+    def pq3d_add_loading(self, n, dsig, load, fix, stream):
+        self._cells(n, dsig, load)
+        if fix:
+            self.T(dsig)[:n] = self.T(load)[:n]
+        else:
+            self.T(dsig)[:n] += self.T(load)[:n]
+
+    def pq3d_rk45_attempt(self, n, inst, h, Tno, Tt1o, Tt2o, st, a, b, dc, f0, A1, A2, ds, sl, V,
+                          cp, y0, y1, y2, y3, err, a63, ninst, stream):
+        assert inst is None or inst == 0
+        self._cells(n, Tno, Tt1o, Tt2o, st, a, b, dc, f0, A1, A2, ds, sl, V, y0, y1, y2, y3)
+        assert self.T(h).numel() >= ninst and self.T(cp).numel() >= 3 * ninst
+        assert self.T(err).numel() >= 4 * ninst
+        o = self.ora
+        o.Tno, o.Tt1o, o.Tt2o, o.state = (self.T(x).numpy().copy() for x in (Tno, Tt1o, Tt2o, st))
+        o.AdotV1, o.AdotV2, o.dsigmadt = (self.T(x).numpy().copy() for x in (A1, A2, ds))
+        o.slipv = self.T(V).numpy().copy()
+        o.b63 = a63
+        hh = float(self.T(h)[0])
+        (s7, t17, t27, q7), c1, c2, _, _ = o.rk45_attempt(hh)
+        for dst, src in ((y0, s7), (y1, t17), (y2, t27), (y3, q7)):
+            self.T(dst)[:n] = torch.from_numpy(src)
+        # error slots as the kernel writes them: separate maxima, bit patterns
+        e = self._errs
+        self.T(err)[:4] = torch.from_numpy(np.array(e, dtype=np.float64).view(np.int64))
+
+    def pq3d_rk45_commit(self, n, inst, h, acc, a, cp, y0, y1, y2, y3, Tno, Tt1o, Tt2o, st, s1,
+                         s2, V, rake, sl1, sl2, vmax, ninst, stream):
+        self._cells(n, a, y0, y1, y2, y3, Tno, Tt1o, Tt2o, st, s1, s2, V, rake, sl1, sl2)
+        assert self.T(vmax).numel() >= ninst
+        for dst, src in ((Tno, y0), (Tt1o, y1), (Tt2o, y2), (st, y3)):
+            self.T(dst)[:n] = self.T(src)[:n]
+
+
+def test_stepper_argument_lists_and_controller(monkeypatch, oracle):
+    from oracle.qdsim_oracle import dense_operators
+    from pyquake3d_amd.mesh import get_eleVec
+    p = planar_c1(nx=6, nz=3)
+    ev, xg = get_eleVec(p.mesh.nodelst, p.mesh.elelst)
+    K = dense_operators(oracle, xg, ev, p.mesh.triangles(), p.mu, p.lambda_)
+    ora = OracleSim(K, p.a, p.b, p.dc, p.f0, p.V0, p.mu, p.Cs, p.Vpl, p.rake, p.initial_state())
+    chk = StepChecker(ora)
+    monkeypatch.setattr(_lib, "call", chk.call)
+    monkeypatch.setattr(_lib, "ptr", chk.ptr)
+    monkeypatch.setattr(_lib, "stream_handle", lambda s=None: None)
+    sim = QDsim(p, IdentityOp(p.n, K), device="cpu")
+    # scripted error maxima: reject twice (one NaN), then accept
+    script = [[np.nan, 1e-9, 1e-9, 1e-9], [1e-2, 1e-9, 1e-9, 1e-9], [1e-6, 1e-7, 2e-7, 1e-8]]
+    hs = []
+
+    def attempt_hook(h):
+        chk._errs = script[len(hs)]
+        hs.append(h)
+        return QDsim._attempt(sim, h)
+
+    sim._attempt = attempt_hook
+    h, dtn = sim.simu_forward(100.0)
+    assert hs[0] == 100.0 and hs[1] == 50.0            # NaN -> halve
+    r1, r2 = 1e-2 + 1e-10, 1e-9 + 1e-10                # second attempt rejected on psi
+    hnew = min(50.0 * 0.9 * (1e-4 / r1) ** 0.2, 50.0 * 0.9 * (1e-4 / r2) ** 0.2)
+    assert hs[2] == pytest.approx(max(25.0, hnew))
+    assert h == hs[2] and sim.nrejects == 2
+    assert dtn == pytest.approx(min(1.5 * h, h * 0.9 * (1e-4 / (1e-6 + 1e-10)) ** 0.2,
+                                    h * 0.9 * (1e-4 / (2e-7 + 1e-10)) ** 0.2))
