@@ -100,6 +100,12 @@ int pq3d_tde_traction_block_f32(const double* d_recv, const double* d_rec,
  * Stacked block operator: K[c*m+i, s*n+j] = op(c,s)[i,j] with
  * op(0,0)=A1s op(1,0)=A2s op(2,0)=Bs op(0,1)=A1d op(1,1)=A2d op(2,1)=Bd. */
 
+/* Row widths (int64 elements) of the tables the kernels below read, in the
+ * order: dense block, low-rank block, ACA workspace offsets, matvec item,
+ * V row, tile, receiver record, source record. Host-only; returns the number
+ * of values written (at most n). */
+int pq3d_layout_constants(int64_t* out, int64_t n);
+
 /* Near-field blocks (replaces ACA_worker's dense branch with
  * calc_stressfunc_fromCS/CD, src/Hmatrix.py:1049-1064, :425-511).
  * d_tab: nblocks x 6 int64 (r0, m, c0, n, pool base, unused); block data is

@@ -42,6 +42,7 @@ SIGNATURES = {
                                             c_void_p, c_int64, c_int64, c_int64, c_int64,
                                             c_void_p]),
     # H-matrix
+    "pq3d_layout_constants": (c_int, [_L, c_int64]),
     "pq3d_hmat_dense_assemble": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_double,
                                          c_double, c_void_p, c_int, c_void_p]),
     "pq3d_hmat_aca": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_double,
@@ -93,6 +94,31 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
     if path is None:
         _lib = lib
     return lib
+
+
+LAYOUT_NAMES = ("dense_block", "lowrank_block", "aca_ws", "item", "vrow", "tile", "recv", "rec")
+
+
+def layout_constants() -> dict:
+    """Row widths of the tables the kernels read (from the library itself)."""
+    import numpy as np
+    out = np.zeros(len(LAYOUT_NAMES), dtype=np.int64)
+    k = load().pq3d_layout_constants(out.ctypes.data_as(_L), len(out))
+    if k != len(out):
+        raise RuntimeError("pq3d_layout_constants returned %d values" % k)
+    return dict(zip(LAYOUT_NAMES, out.tolist()))
+
+
+def check_width(name: str, table) -> None:
+    """Raise unless ``table`` is a C-contiguous 2-D array/tensor whose row
+    width equals the kernel's for ``name``."""
+    w = layout_constants()[name]
+    shape = tuple(table.shape)
+    if len(shape) != 2 or shape[1] != w:
+        raise ValueError(f"{name} table has shape {shape}; the kernels read rows of {w}")
+    contiguous = table.is_contiguous() if hasattr(table, "is_contiguous") else table.flags["C_CONTIGUOUS"]
+    if not contiguous:
+        raise ValueError(f"{name} table is not contiguous")
 
 
 def exported(lib: ctypes.CDLL) -> dict:

@@ -138,6 +138,10 @@ class BlockTree:
         wV = torch.empty(kcap * 2 * n, dtype=torch.float64, device=dev)
         flags = torch.empty(3 * m + 2 * n, dtype=torch.uint8, device=dev)
         rank = torch.empty(1, dtype=torch.int64, device=dev)
+        _lib.check_width("lowrank_block", tab)
+        _lib.check_width("aca_ws", wsoff)
+        _lib.check_width("recv", recv)
+        _lib.check_width("rec", rec)
         _lib.call("pq3d_hmat_aca", _lib.ptr(tab), _lib.ptr(wsoff), 1, _lib.ptr(recv),
                   _lib.ptr(rec), self.mu_, self.lambda_, float(self.aca_eps), kcap, _lib.ptr(wU),
                   _lib.ptr(wV), _lib.ptr(flags), _lib.ptr(rank), _lib.stream_handle())

@@ -24,14 +24,22 @@
 
 namespace pq3d {
 
+// Table row widths are explicit and exported by pq3d_layout_constants() so the
+// host can check its tables against them.
 // dense block table: r0, m, c0, n, base (pool offset), unused
-enum { DB_R0 = 0, DB_M, DB_C0, DB_N, DB_BASE, DB_W };
+enum { DB_R0 = 0, DB_M = 1, DB_C0 = 2, DB_N = 3, DB_BASE = 4, DB_W = 6 };
 // low-rank block table: r0, m, c0, n, ubase, vbase, k, toff
-enum { LB_R0 = 0, LB_M, LB_C0, LB_N, LB_UB, LB_VB, LB_K, LB_TOFF, LB_W };
+enum { LB_R0 = 0, LB_M = 1, LB_C0 = 2, LB_N = 3, LB_UB = 4, LB_VB = 5, LB_K = 6, LB_TOFF = 7,
+       LB_W = 8 };
+// ACA workspace offsets per block: U, V, flags, unused
+enum { WS_U = 0, WS_V = 1, WS_F = 2, WS_W = 4 };
 // matvec work item: kind (0 dense, 1 low-rank), base, ld, plane, count, xoff
-enum { IT_KIND = 0, IT_BASE, IT_LD, IT_PLANE, IT_COUNT, IT_XOFF, IT_W };
+enum { IT_KIND = 0, IT_BASE = 1, IT_LD = 2, IT_PLANE = 3, IT_COUNT = 4, IT_XOFF = 5, IT_W = 6 };
 // V-row item: vbase (row start), n, c0, t index
-enum { VR_BASE = 0, VR_N, VR_C0, VR_T, VR_W };
+enum { VR_BASE = 0, VR_N = 1, VR_C0 = 2, VR_T = 3, VR_W = 4 };
+// tile: first row, rows
+enum { TL_W = 2 };
+constexpr int kRecvW = 12;  // receiver record: centroid + ev1, ev2, ev3
 
 __device__ inline void load_rec_g(const double* __restrict__ src, double* r) {
 #pragma unroll
@@ -53,7 +61,7 @@ __global__ void __launch_bounds__(256) k_dense_assemble(const int64_t* __restric
     const int64_t i = p % m, j = p / m;
     double r[kSrcRec];
     load_rec_g(rec + kSrcRec * (c0 + j), r);
-    const double* q = recv + 12 * (r0 + i);
+    const double* q = recv + kRecvW * (r0 + i);
     double ts[3], td[3];
     tde_pair_traction_sd(r, q, q + 3, mu, lam, nu, ts, td);
     T* o = pool + base + j * m + i;
@@ -131,9 +139,9 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
   const int64_t R = 3 * m, C = 2 * n;
   // per-block rank cap (the workspace of this block holds kcap columns/rows)
   const int kcap = (int)((b[LB_K] < kmax) ? b[LB_K] : kmax);
-  double* U = wU + wsoff[4 * bid + 0];
-  double* V = wV + wsoff[4 * bid + 1];
-  uint8_t* urow = flags + wsoff[4 * bid + 2];
+  double* U = wU + wsoff[WS_W * bid + WS_U];
+  double* V = wV + wsoff[WS_W * bid + WS_V];
+  uint8_t* urow = flags + wsoff[WS_W * bid + WS_F];
   uint8_t* ucol = urow + R;
   for (int64_t r = tid; r < R; r += kAcaThreads) urow[r] = 0;
   for (int64_t c = tid; c < C; c += kAcaThreads) ucol[c] = 0;
@@ -149,7 +157,7 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
     const int cs = (int)(rstar / m);
     const int64_t is = rstar % m;
     double* vk = V + (int64_t)k * C;
-    const double* q = recv + 12 * (r0 + is);
+    const double* q = recv + kRecvW * (r0 + is);
     ArgMax loc{-1.0, -1};
     for (int64_t j = tid; j < n; j += kAcaThreads) {
       double rr[kSrcRec];
@@ -199,7 +207,7 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
       double rr[kSrcRec];
       load_rec_g(rec + kSrcRec * (c0 + js), rr);
       for (int64_t i = tid; i < m; i += kAcaThreads) {
-        const double* qi = recv + 12 * (r0 + i);
+        const double* qi = recv + kRecvW * (r0 + i);
         double ts[3], td[3];
         tde_pair_traction_sd(rr, qi, qi + 3, mu, lam, nu, ts, td);
         for (int c = 0; c < 3; ++c) {
@@ -253,8 +261,8 @@ __global__ void __launch_bounds__(256) k_aca_pack(const int64_t* __restrict__ ta
   const int64_t* b = tab + bid * LB_W;
   const int64_t m = b[LB_M], n = b[LB_N], k = b[LB_K];
   if (k <= 0) return;
-  const double* U = wU + wsoff[4 * bid + 0];
-  const double* V = wV + wsoff[4 * bid + 1];
+  const double* U = wU + wsoff[WS_W * bid + WS_U];
+  const double* V = wV + wsoff[WS_W * bid + WS_V];
   T* pu = pool + b[LB_UB];
   T* pv = pool + b[LB_VB];
   const int64_t nu = 3 * m * k, nvv = 2 * n * k;
@@ -300,7 +308,7 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
     double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3) {
   __shared__ double red[kTileWaves][3][64];
   const int64_t tile = blockIdx.x;
-  const int64_t R0 = tiles[2 * tile], mt = tiles[2 * tile + 1];
+  const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const bool live = lane < mt;
@@ -352,6 +360,16 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
 using namespace pq3d;
 
 extern "C" {
+
+// Row widths of every table the H-matrix kernels read, in this order:
+// dense block, low-rank block, ACA workspace offsets, matvec item, V row,
+// tile, receiver record, source record. Returns the number written.
+int pq3d_layout_constants(int64_t* out, int64_t n) {
+  const int64_t v[8] = {DB_W, LB_W, WS_W, IT_W, VR_W, TL_W, kRecvW, kSrcRec};
+  int64_t k = 0;
+  for (; k < n && k < 8; ++k) out[k] = v[k];
+  return (int)k;
+}
 
 int pq3d_hmat_dense_assemble(const int64_t* d_tab, int64_t nblocks, const double* d_recv,
                              const double* d_rec, double mu, double lam, void* d_pool,

@@ -209,6 +209,9 @@ class HMatrix:
         dtab = np.stack([r0[dense_idx], d_m, c0[dense_idx], d_n, d_base,
                          np.zeros_like(d_base)], axis=1).astype(np.int64)
         self.dtab = torch.from_numpy(np.ascontiguousarray(dtab)).to(dev)
+        _lib.check_width("dense_block", self.dtab)
+        _lib.check_width("recv", self.recv)
+        _lib.check_width("rec", self.rec)
         _lib.call("pq3d_hmat_dense_assemble", _lib.ptr(self.dtab), len(dense_idx),
                   _lib.ptr(self.recv), _lib.ptr(self.rec), self.mu, self.lambda_,
                   _lib.ptr(self.pool), self.esize, stream)
@@ -229,6 +232,8 @@ class HMatrix:
                 ltab[q] = (r0[gb], m[gb], c0[gb], nn[gb], ub_of.get(gb, 0), vb_of.get(gb, 0),
                            kq if ok else 0, 0)
             lt = torch.from_numpy(ltab).to(dev)
+            _lib.check_width("lowrank_block", lt)
+            _lib.check_width("aca_ws", wsoff)
             _lib.call("pq3d_hmat_aca_pack", _lib.ptr(lt), _lib.ptr(wsoff), len(sel),
                       _lib.ptr(wsU), _lib.ptr(wsV), _lib.ptr(self.pool), self.esize, stream)
         _sync(dev)
@@ -272,6 +277,8 @@ class HMatrix:
         tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3] = r0[blocks], mb, c0[blocks], nbk
         tab[:, 6] = kcap
         tab_t = torch.from_numpy(tab).to(dev)
+        _lib.check_width("lowrank_block", tab_t)
+        _lib.check_width("aca_ws", wsoff)
         rank = torch.empty(nb, dtype=torch.int64, device=dev)
         _lib.call("pq3d_hmat_aca", _lib.ptr(tab_t), _lib.ptr(wsoff), nb, _lib.ptr(self.recv),
                   _lib.ptr(self.rec), self.mu, self.lambda_, float(eps), int(kcap.max()),
@@ -291,6 +298,9 @@ class HMatrix:
         self.vrows = torch.from_numpy(W["vrows"]).to(dev)
         self.nvrows = W["vrows"].shape[0]
         self.tbuf = torch.zeros(max(self.nvrows, 1), dtype=torch.float64, device=dev)
+        _lib.check_width("item", self.items)
+        _lib.check_width("vrow", self.vrows)
+        _lib.check_width("tile", self.tiles)
         # algorithmic bytes of one matvec: stored operators + 2 input + 3 output vectors
         self.matvec_bytes = int(self.pool.numel()) * self.esize + 5 * 8 * self.n
 

@@ -137,6 +137,8 @@ class DeviceMesh:
         fn = "pq3d_tde_traction_block" if dtype == torch.float64 else "pq3d_tde_traction_block_f32"
         if dtype not in (torch.float64, torch.float32):
             raise ValueError("dtype must be float64 or float32")
+        _lib.check_width("recv", self.recv)
+        _lib.check_width("rec", self.rec)
         _lib.call(fn, _lib.ptr(self.recv), _lib.ptr(self.rec), _lib.ptr(r), row0, m, _lib.ptr(c),
                   col0, n, self.mu, self.lambda_, _lib.ptr(out), si, sj, m * n, out.numel(),
                   _lib.stream_handle(stream))

@@ -14,6 +14,18 @@ from pyquake3d_amd import hmatrix as hm
 from pyquake3d_amd.mesh import undulating_fault
 
 
+LC = None
+
+
+def rows(t, name):
+    """Rows of a table as the kernel reads them: flat buffer, the library's
+    own row width, no reliance on the tensor's shape."""
+    w = LC[name]
+    flat = t.reshape(-1).numpy()
+    assert flat.size % w == 0, (name, flat.size, w)
+    return flat.reshape(-1, w)
+
+
 class Checker:
     def __init__(self, rng, fail_frac=0.1):
         self.reg = {}
@@ -38,21 +50,28 @@ class Checker:
         self.calls.append(name)
         getattr(self, name)(*a)
 
+    def _recs(self, recv, rec):
+        N = self.T(recv).numel() // LC["recv"]
+        assert self.T(rec).numel() // LC["rec"] == N
+        return N
+
     def pq3d_hmat_dense_assemble(self, tab, nb, recv, rec, mu, lam, pool, esize, stream):
-        tab = self.T(tab).numpy()
-        N = self.T(recv).shape[0]
-        assert self.T(rec).shape[0] == N and tab.shape == (nb, 6)
+        tab = rows(self.T(tab), "dense_block")
+        N = self._recs(recv, rec)
+        assert len(tab) >= nb
         P = self.T(pool).numel()
-        for r0, m, c0, n, base, _ in tab:
+        for b in range(nb):
+            r0, m, c0, n, base = tab[b, :5]
             assert 0 <= r0 and r0 + m <= N and 0 <= c0 and c0 + n <= N and m > 0 and n > 0
             assert base >= 0 and base + 6 * m * n <= P
 
     def pq3d_hmat_aca(self, tab, wsoff, nb, recv, rec, mu, lam, eps, kmax, wU, wV, flags, rank,
                       stream):
-        tab_a = self.T(tab).numpy()
-        ws = self.T(wsoff).numpy()
-        N = self.T(recv).shape[0]
+        tab_a = rows(self.T(tab), "lowrank_block")
+        ws = rows(self.T(wsoff), "aca_ws")
+        N = self._recs(recv, rec)
         nU, nV, nF = self.T(wU).numel(), self.T(wV).numel(), self.T(flags).numel()
+        assert self.T(rank).numel() >= nb
         r = np.empty(nb, dtype=np.int64)
         for b in range(nb):
             r0, m, c0, n = tab_a[b, :4]
@@ -64,11 +83,11 @@ class Checker:
             assert ws[b, 2] + 3 * m + 2 * n <= nF
             r[b] = -1 if self.rng.random() < self.fail_frac else self.rng.integers(1, kcap + 1)
             self.aca_caps[(int(r0), int(c0))] = kcap
-        self.T(rank)[:] = torch.from_numpy(r)
+        self.T(rank)[:nb] = torch.from_numpy(r)
 
     def pq3d_hmat_aca_pack(self, tab, wsoff, nb, wU, wV, pool, esize, stream):
-        tab_a = self.T(tab).numpy()
-        ws = self.T(wsoff).numpy()
+        tab_a = rows(self.T(tab), "lowrank_block")
+        ws = rows(self.T(wsoff), "aca_ws")
         P = self.T(pool).numel()
         for b in range(nb):
             r0, m, c0, n, ub, vb, k = tab_a[b, :7]
@@ -83,14 +102,17 @@ class Checker:
                          t, y1, y2, y3, stream):
         P = self.T(pool).numel()
         N = self.T(v1).numel()
+        assert self.T(v2).numel() == N and min(self.T(y).numel() for y in (y1, y2, y3)) >= N
         nt = self.T(t).numel()
         if nvr:
-            for vb, n, c0, ti in self.T(vrows).numpy():
+            vr = rows(self.T(vrows), "vrow")
+            assert len(vr) >= nvr
+            for vb, n, c0, ti in vr[:nvr]:
                 assert vb >= 0 and vb + 2 * n <= P and c0 + n <= N and 0 <= ti < nt
-        tl = self.T(tiles).numpy()
+        tl = rows(self.T(tiles), "tile")
         tp = self.T(tile_ptr).numpy()
-        it = self.T(items).numpy()
-        assert tl.shape == (ntiles, 2) and tp.shape == (ntiles + 1,) and tp[-1] == len(it)
+        it = rows(self.T(items), "item")
+        assert len(tl) >= ntiles and len(tp) >= ntiles + 1 and tp[ntiles] <= len(it)
         for q in range(ntiles):
             R0, mt = tl[q]
             assert 0 < mt <= 64 and R0 + mt <= N
@@ -106,6 +128,8 @@ class Checker:
 @pytest.mark.parametrize("leaf,eta,dtype", [(32, 2.0, torch.float32), (64, 2.0, torch.float64),
                                             (32, 0.0, torch.float64)])
 def test_hmatrix_device_accesses_in_bounds(monkeypatch, leaf, eta, dtype):
+    global LC
+    LC = _lib.layout_constants()
     chk = Checker(np.random.default_rng(leaf))
     monkeypatch.setattr(_lib, "call", chk.call)
     monkeypatch.setattr(_lib, "ptr", chk.ptr)
