@@ -3,20 +3,27 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 Workloads
-  assembly  Kernel assembly only: all N x N receiver/source pairs of the
-            undulating 100 x 25 km fault at N = 20,000, unit strike and dip
-            slip, FP64, projected to the six traction operators (19.2 GB of
-            FP64 output, resident in HBM). One step = one full assembly.
+  hstep     (default) H-matrix time stepping at N = 100,352 triangles on the
+            undulating 100 x 25 km fault with BP5-like heterogeneous friction
+            (leaf 64, eta 2, ACA eps 1e-4). A step is one accepted RK45 time
+            step (one six-operator H-matvec, the RK45 attempts, commit). The
+            metric is simulated years per wall-hour over K timed steps after W
+            warm-up steps, all from the same seeded initial state:
+              plain : FP64 operator storage, one simulation
+              fast  : FP32 operator storage, one simulation
+            ``value`` is the fast variant; both are reported.
+  assembly  Kernel assembly only: all N x N pairs at N = 20,000 (FP64, six
+            traction operators, 19.2 GB of output in HBM). One step = one full
+            assembly.
 
-Prints ONE JSON line (rank 0). Inputs are resident in HBM before the timed
-region; the timed region is bracketed by device synchronisation, and kernel
-time is also measured with HIP events on the launch stream.
+Prints ONE JSON line. Inputs are resident in HBM before the timed region;
+the timed region is bracketed by device synchronisation; kernel times come
+from HIP events recorded on the launch stream.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -26,17 +33,139 @@ sys.path.insert(0, str(REPO))
 
 MI355X_HBM_GBS = 8000.0
 MI355X_FP64_TFLOPS = 78.6
+YEAR = 3.15576e7
 
-# FP64 flops of one pair (both unit strike and unit dip slip, projected), from
-# an operation count of tde_pair_traction_sd in pyquake3d_amd/csrc/tde_device.h
-# ("Assembly roofline").
-FLOPS_PER_PAIR = 1520.0
+# FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
+# (unit strike and dip slip, both projected), counted operator by operator
+# from pyquake3d_amd/csrc/tde_device.h: per angular dislocation 191 (brackets)
+# + 8 (side frame) + 2 x (55 prefactors/combination + 34 side rotation) + 4
+# (slip rotation), three of them, plus 18 (TD transform) + 28 (configuration)
+# + 2 x (129 global rotation + 20 Hooke + 30 projection) + 24 (summing the
+# three angular dislocations); divisions and square roots count as one.
+FLOPS_PER_PAIR = 1571.0
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# ---------------------------------------------------------------- hstep
+def run_variant(problem, dtype, steps, warmup, name):
+    import torch
+
+    from pyquake3d_amd.hmatrix import HMatrix
+    from pyquake3d_amd.qdsim import QDsim
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    H = HMatrix.from_mesh(problem.mesh, problem.mu, problem.lambda_, leaf=64, eta=2.0, eps=1e-4,
+                          dtype=dtype, device=dev)
+    build_s = time.perf_counter() - t0
+    sim = QDsim(problem, H, device=dev)
+    dt = 1e-3
+    for _ in range(warmup):
+        _, dt = sim.simu_forward(dt)
+    torch.cuda.synchronize()
+    t_sim0 = sim.time
+    n0, a0 = sim.nsteps, sim.nattempts
+    sim.timing = True
+    sim.matvec_events = []
+    w0 = time.perf_counter()
+    for _ in range(steps):
+        _, dt = sim.simu_forward(dt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    mv = sim.matvec_ms()
+    yrs = (sim.time - t_sim0) / YEAR
+    res = dict(name=name, dtype=str(dtype).replace("torch.", ""), build_s=build_s,
+               sim_years=yrs, wall_s=wall, ms_per_step=1e3 * wall / steps,
+               attempts_per_step=(sim.nattempts - a0) / (sim.nsteps - n0),
+               years_per_hour=yrs / (wall / 3600.0), matvec_ms=sum(mv) / len(mv),
+               matvec_bytes=H.matvec_bytes, pool_gb=H.stats.bytes / 1e9,
+               ranks_mean=H.stats.ranks_mean, t_end_years=sim.time / YEAR,
+               max_slipv=sim.max_slipv())
+    log(f"{name}: {json.dumps(res)}")
+    return res, H, sim
+
+
+def cpu_baseline_hstep(problem, H, gpu_res, nsteps=4):
+    """Reference-style CPU step (per-block numpy FP64 matvec + numpy RK45) on
+    the same blocks, timed over ``nsteps`` steps from the initial state; the
+    rate is expressed for the GPU run's simulated interval."""
+    import numpy as np
+
+    from oracle.hmat_cpu import CpuHMatrix
+    from oracle.qdsim_oracle import OracleSim
+    t0 = time.perf_counter()
+    C = CpuHMatrix(H)
+    setup = time.perf_counter() - t0
+    perm = H.perm
+    init = [np.asarray(x)[perm] for x in problem.initial_state()]
+    g = lambda x: np.asarray(x)[perm]
+    ora = OracleSim(None, g(problem.a), g(problem.b), g(problem.dc), g(problem.f0), problem.V0,
+                    problem.mu, problem.Cs, g(problem.Vpl), g(problem.rake), init,
+                    matvec=C.matvec)
+    dt = 1e-3
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        _, dt = ora.simu_forward(dt)
+    per_step = (time.perf_counter() - t0) / nsteps
+    steps = gpu_res["wall_s"] / (gpu_res["ms_per_step"] / 1e3)
+    cpu_hours = per_step * steps / 3600.0
+    return {"value": gpu_res["sim_years"] / cpu_hours, "unit": "sim-years/wall-hour",
+            "cores": 1, "kind": "port",
+            "sample": f"{nsteps} steps of the reference-style CPU step (per-block numpy FP64 "
+                      f"matvec over the same {len(C.dense)} dense + {len(C.lowrank)} low-rank "
+                      f"blocks, numpy RK45) at {per_step:.2f} s/step, applied to the GPU run's "
+                      f"{steps:.0f} steps / {gpu_res['sim_years']:.2f} simulated years; "
+                      f"host setup {setup:.1f} s"}
+
+
+def bench_hstep(args):
+    import torch
+
+    from pyquake3d_amd.problem import undulating_c3
+
+    problem = undulating_c3()
+    log(f"hstep: N={problem.n}")
+    plain, H64, _ = run_variant(problem, torch.float64, args.steps, args.warmup, "plain_fp64")
+    del H64
+    torch.cuda.empty_cache()
+    fast, H32, _ = run_variant(problem, torch.float32, args.steps, args.warmup, "fast_fp32")
+    achieved = fast["matvec_bytes"] / (fast["matvec_ms"] * 1e-3) / 1e9
+    res = {
+        "metric": "sim-years/wall-hour at N=1e5 (fp32+batched+learned dt vs plain fp64)",
+        "value": fast["years_per_hour"],
+        "unit": "sim-years/wall-hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fast["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 operator storage, f64 state and accumulation",
+        "data": "synthetic undulating fault mesh and seeded friction fields",
+        "config": {"workload": f"H-matrix RK45 time stepping, undulating fault 100x25 km, "
+                               f"N={problem.n}, leaf 64, eta 2, ACA eps 1e-4, BP5-like friction, "
+                               f"{args.steps} accepted steps after {args.warmup} from the seeded "
+                               f"initial state",
+                   "variant": "fast_fp32 (one simulation; batching and learned dt not in this "
+                              "line)"},
+        "speedup_vs_plain_fp64": fast["years_per_hour"] / plain["years_per_hour"],
+        "plain_fp64": plain,
+        "fast_fp32": fast,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / MI355X_HBM_GBS, "traffic": None,
+                     "kernel": "H-matvec (k_hmv_phase1 + k_hmv_phase2)",
+                     "kernel_ms": fast["matvec_ms"], "algorithmic_bytes": fast["matvec_bytes"]},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_hstep(problem, H32, fast)
+    return res
+
+
+# ------------------------------------------------------------- assembly
 def bench_assembly(args):
     import numpy as np
     import torch
@@ -58,35 +187,29 @@ def bench_assembly(args):
     ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record()
-    for k in range(args.steps):
+    for _ in range(args.steps):
         dm.traction_block(out=out)
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kern_ms = ev0.elapsed_time(ev1) / args.steps
     pairs = float(N) * N
-    evals_per_s = 2.0 * pairs * args.steps / wall
     sample = out[:, :64, :64].cpu().numpy()
     assert np.isfinite(sample).all()
     achieved = pairs * FLOPS_PER_PAIR / (kern_ms * 1e-3) / 1e12
     res = {
-        "metric": "TDE evaluations/s (C2 full N x N assembly, FP64)",
-        "value": evals_per_s,
+        "metric": "TDE evaluations/s (N x N assembly, FP64)",
+        "value": 2.0 * pairs * args.steps / wall,
         "unit": "evaluations/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic undulating fault mesh, seed 0",
-        "config": {"workload": "C2 kernel assembly, undulating fault 100x25 km, N=20000, "
-                               "unit Ss and Ds, 6 traction operators, FP64"},
+        "config": {"workload": "kernel assembly, undulating fault 100x25 km, N=20000, unit Ss "
+                               "and Ds, six traction operators, FP64"},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": MI355X_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_TFLOPS,
-                     "traffic": None, "kernel_ms": kern_ms},
+                     "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_TFLOPS, "traffic": None,
+                     "kernel_ms": kern_ms},
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline_assembly(mesh, dm)
@@ -100,7 +223,6 @@ def cpu_baseline_assembly(mesh, dm):
     o = Oracle()
     tri = mesh.triangles()
     rows = np.arange(mesh.n)
-    # calibrate, then size the sample to ~10 s of CPU work
     t = time.perf_counter()
     o.traction_block(dm.xg, dm.eleVec, tri, rows, np.arange(4), 32e9, 32e9)
     per_col = (time.perf_counter() - t) / 4
@@ -117,16 +239,23 @@ def cpu_baseline_assembly(mesh, dm):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="assembly", choices=["assembly"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="hstep", choices=["hstep", "assembly"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU design is documented, not built)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the "
+                                   "multi-GPU design is documented, not built)"}))
         sys.exit(2)
-    res = bench_assembly(args)
+    if args.workload == "hstep":
+        args.steps = 3000 if args.steps is None else args.steps
+        args.warmup = 200 if args.warmup is None else args.warmup
+        res = bench_hstep(args)
+    else:
+        args.steps = 3 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        res = bench_assembly(args)
     print(json.dumps(res), flush=True)
 
 

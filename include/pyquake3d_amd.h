@@ -102,7 +102,7 @@ int pq3d_tde_traction_block_f32(const double* d_recv, const double* d_rec,
 
 /* Row widths (int64 elements) of the tables the kernels below read, in the
  * order: dense block, low-rank block, ACA workspace offsets, matvec item,
- * V row, tile, receiver record, source record. Host-only; returns the number
+ * phase-1 unit, phase-1b reduction, tile, receiver record, source record. Host-only; returns the number
  * of values written (at most n). */
 int pq3d_layout_constants(int64_t* out, int64_t n);
 
@@ -126,21 +126,25 @@ int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
                   int64_t* d_rank, void* stream);
 
 /* Copy converged factors into the pool: U (3m x k, column-major) at
- * d_tab[b][4], V (k x 2n, row-major) at d_tab[b][5], k = d_tab[b][6]. */
+ * d_tab[b][4], V transposed (2n x k, row-major) at d_tab[b][5],
+ * k = d_tab[b][6]. */
 int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
                        const double* d_wU, const double* d_wV, void* d_pool, int dtype_bytes,
                        void* stream);
 
-/* Six-operator H-matvec (replaces the six blocks_process_MVM calls of one
- * step, src/Hmatrix.py:1445-1538 via src/QDsim.py:906-917):
- * y1 = A1s v1 + A1d v2, y2 = A2s v1 + A2d v2, y3 = Bs v1 + Bd v2 (FP64
- * vectors, FP64 accumulation). Work lists as built by
- * pyquake3d_amd.hmatrix.build_worklists; d_t: nvrows doubles of scratch. */
-int pq3d_hmat_matvec(const int64_t* d_vrows, int64_t nvrows, const int64_t* d_tiles,
-                     const int64_t* d_tile_ptr, int64_t ntiles, const int64_t* d_items,
-                     const void* d_pool, int dtype_bytes, const double* d_v1,
-                     const double* d_v2, double* d_t, double* d_y1, double* d_y2, double* d_y3,
-                     void* stream);
+/* Six-operator H-matvec for nrhs right-hand sides (replaces the six
+ * blocks_process_MVM calls of one step, src/Hmatrix.py:1445-1538 via
+ * src/QDsim.py:906-917): y1 = A1s v1 + A1d v2, y2 = A2s v1 + A2d v2,
+ * y3 = Bs v1 + Bd v2, FP64 vectors interleaved by instance (element (row, b)
+ * at row*nrhs + b), FP64 accumulation; nrhs in {1, 2, 4, 8, 16}. Low-rank V
+ * factors are stored transposed (2n x k). Work lists (phase-1 units, phase-1b
+ * reductions, tiles, items) as built by pyquake3d_amd.hmatrix.build_worklists;
+ * d_t: nt*nrhs and d_part: npart*nrhs doubles of scratch. */
+int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_red, int64_t nred,
+                     const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
+                     const int64_t* d_items, const void* d_pool, int dtype_bytes, int nrhs,
+                     const double* d_v1, const double* d_v2, double* d_t, double* d_part,
+                     double* d_y1, double* d_y2, double* d_y3, void* stream);
 
 /* ---------------------------------------------------------- time step
  * Cells may belong to n_inst independent instances (d_inst[i], or NULL for

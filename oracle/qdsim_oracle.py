@@ -40,8 +40,11 @@ class OracleSim:
     """
 
     def __init__(self, K, a, b, dc, f0, V0, mu, Cs, Vpl, rake0, init, shear_loading=None,
-                 normal_loading=None, fix_Tn=False, reltol=1e-4, tableau="ref"):
+                 normal_loading=None, fix_Tn=False, reltol=1e-4, tableau="ref", matvec=None):
+        """K: dict of dense operators, or None with ``matvec(v1, v2) ->
+        (AdotV1, AdotV2, dsigma_K)`` (e.g. a CPU H-matrix)."""
         self.K = K
+        self.matvec = matvec
         self.a, self.b, self.dc, self.f0 = a, b, dc, f0
         self.V0, self.mu, self.Cs = V0, mu, Cs
         self.slipvC = Vpl
@@ -125,12 +128,17 @@ class OracleSim:
         K = self.K
         v1 = self.slipv1 - self.slipvC * np.cos(self.rake0)
         v2 = self.slipv2 - self.slipvC * np.sin(self.rake0)
-        if self.fix_Tn:
-            self.dsigmadt = self.normal_loading
+        if self.matvec is not None:
+            y1, y2, y3 = self.matvec(v1, v2)
+            self.dsigmadt = self.normal_loading if self.fix_Tn else y3 + self.normal_loading
+            self.AdotV1, self.AdotV2 = y1, y2
         else:
-            self.dsigmadt = K["Bs"] @ v1 + K["Bd"] @ v2 + self.normal_loading
-        self.AdotV1 = K["A1s"] @ v1 + K["A1d"] @ v2
-        self.AdotV2 = K["A2s"] @ v1 + K["A2d"] @ v2
+            if self.fix_Tn:
+                self.dsigmadt = self.normal_loading
+            else:
+                self.dsigmadt = K["Bs"] @ v1 + K["Bd"] @ v2 + self.normal_loading
+            self.AdotV1 = K["A1s"] @ v1 + K["A1d"] @ v2
+            self.AdotV2 = K["A2s"] @ v1 + K["A2d"] @ v2
         nrjct = 0
         h = dttry
         while True:

@@ -50,9 +50,9 @@ SIGNATURES = {
                               c_void_p]),
     "pq3d_hmat_aca_pack": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                    c_int, c_void_p]),
-    "pq3d_hmat_matvec": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p,
-                                 c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
-                                 c_void_p, c_void_p, c_void_p]),
+    "pq3d_hmat_matvec": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p,
+                                 c_int64, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     # time step
     "pq3d_rk45_attempt": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -96,7 +96,8 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
     return lib
 
 
-LAYOUT_NAMES = ("dense_block", "lowrank_block", "aca_ws", "item", "vrow", "tile", "recv", "rec")
+LAYOUT_NAMES = ("dense_block", "lowrank_block", "aca_ws", "item", "unit", "reduce", "tile",
+                "recv", "rec")
 
 
 def layout_constants() -> dict:

@@ -35,8 +35,15 @@ enum { LB_R0 = 0, LB_M = 1, LB_C0 = 2, LB_N = 3, LB_UB = 4, LB_VB = 5, LB_K = 6,
 enum { WS_U = 0, WS_V = 1, WS_F = 2, WS_W = 4 };
 // matvec work item: kind (0 dense, 1 low-rank), base, ld, plane, count, xoff
 enum { IT_KIND = 0, IT_BASE = 1, IT_LD = 2, IT_PLANE = 3, IT_COUNT = 4, IT_XOFF = 5, IT_W = 6 };
-// V-row item: vbase (row start), n, c0, t index
-enum { VR_BASE = 0, VR_N = 1, VR_C0 = 2, VR_T = 3, VR_W = 4 };
+// phase-1 unit (a column chunk of one low-rank block): V^T base, k, first
+// stacked column, columns, block n, block c0, output row offset, direct flag
+// (1: output rows are the block's t rows; 0: partial rows reduced in phase 1b)
+enum { UN_VT = 0, UN_K = 1, UN_CB = 2, UN_CC = 3, UN_N = 4, UN_C0 = 5, UN_OUT = 6, UN_DIRECT = 7,
+       UN_W = 8 };
+// phase-1b reduction (blocks split over several units): t row offset, k,
+// first partial row, number of units (partial rows of unit u start at
+// first + u*k)
+enum { RD_T = 0, RD_K = 1, RD_P = 2, RD_NU = 3, RD_W = 4 };
 // tile: first row, rows
 enum { TL_W = 2 };
 constexpr int kRecvW = 12;  // receiver record: centroid + ev1, ev2, ev3
@@ -267,52 +274,96 @@ __global__ void __launch_bounds__(256) k_aca_pack(const int64_t* __restrict__ ta
   T* pv = pool + b[LB_VB];
   const int64_t nu = 3 * m * k, nvv = 2 * n * k;
   for (int64_t p = threadIdx.x; p < nu; p += blockDim.x) pu[p] = (T)U[p];
-  for (int64_t p = threadIdx.x; p < nvv; p += blockDim.x) pv[p] = (T)V[p];
+  // V is stored transposed (2n x k, row-major): element (c, l) at c*k + l
+  const int64_t C = 2 * n;
+  for (int64_t p = threadIdx.x; p < nvv; p += blockDim.x) {
+    const int64_t c = p / k, l = p - c * k;
+    pv[p] = (T)V[l * C + c];
+  }
 }
 
 // ------------------------------------------------------------- H-matvec
-// Phase 1: t[row] = V[row, :] . [v1[c0:c0+n], v2[c0:c0+n]], one wavefront per
-// V row, FP64 accumulation.
-template <typename T>
-__global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ vrows,
-                                                    int64_t nrows, const T* __restrict__ pool,
+// NB right-hand sides at once (an ensemble sharing the operator). Vectors are
+// interleaved by instance: x[j*NB + b]. Every stored operator element is read
+// once per product whatever NB is; x and t values are wave-uniform
+// (scalar-cache broadcasts). Accumulation is FP64.
+
+// Phase 1: t[l][b] = sum over the unit's stacked columns c of V^T[c][l] x_b[c]
+// (x = v1 for c < n, v2 for c >= n); one wavefront per unit, lane <-> l.
+template <typename T, int NB>
+__global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ units,
+                                                    int64_t nunits, const T* __restrict__ pool,
                                                     const double* __restrict__ v1,
                                                     const double* __restrict__ v2,
-                                                    double* __restrict__ t) {
-  const int64_t row = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+                                                    double* __restrict__ t,
+                                                    double* __restrict__ part) {
+  const int64_t u = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (row >= nrows) return;
-  const int64_t* it = vrows + row * VR_W;
-  const T* v = pool + it[VR_BASE];
-  const int64_t n = it[VR_N], c0 = it[VR_C0];
-  double acc = 0.0;
-  for (int64_t j = lane; j < n; j += 64) {
-    acc += (double)v[j] * v1[c0 + j];
-    acc += (double)v[n + j] * v2[c0 + j];
-  }
+  if (u >= nunits) return;
+  const int64_t* it = units + u * UN_W;
+  const int64_t vt = it[UN_VT], k = it[UN_K], cb = it[UN_CB], cc = it[UN_CC], n = it[UN_N];
+  const int64_t c0 = it[UN_C0], out = it[UN_OUT];
+  double* dst = it[UN_DIRECT] ? t : part;
+  for (int64_t l0 = 0; l0 < k; l0 += 64) {
+    const int64_t l = l0 + lane;
+    const bool live = l < k;
+    double acc[NB];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (lane == 0) t[it[VR_T]] = acc;
+    for (int b = 0; b < NB; ++b) acc[b] = 0.0;
+    for (int64_t q = 0; q < cc; ++q) {
+      const int64_t c = cb + q;
+      const bool dip = c >= n;
+      const double* xs = (dip ? v2 : v1) + (c0 + (dip ? c - n : c)) * NB;
+      const double v = live ? (double)pool[vt + c * k + l] : 0.0;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[b] += v * xs[b];
+    }
+    if (live) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) dst[(out + l) * NB + b] = acc[b];
+    }
+  }
+}
+
+// Phase 1b: blocks split over several units: t = sum of their partial rows in
+// unit order (deterministic).
+template <int NB>
+__global__ void __launch_bounds__(256) k_hmv_phase1b(const int64_t* __restrict__ red,
+                                                     int64_t nred, const double* __restrict__ part,
+                                                     double* __restrict__ t) {
+  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= nred) return;
+  const int64_t* it = red + r * RD_W;
+  const int64_t to = it[RD_T], k = it[RD_K], p0 = it[RD_P], nu = it[RD_NU];
+  for (int64_t e = lane; e < k * NB; e += 64) {
+    const int64_t l = e / NB, b = e - l * NB;
+    double s = 0.0;
+    for (int64_t q = 0; q < nu; ++q) s += part[(p0 + q * k + l) * NB + b];
+    t[(to + l) * NB + b] = s;
+  }
 }
 
 constexpr int kTileWaves = 4;
 
 // Phase 2: one workgroup per row tile (a leaf cluster of <= 64 rows); lane <->
-// row. Waves split the tile's work items; partial sums meet in LDS.
-// y1 = AdotV1, y2 = AdotV2, y3 = dsigma (K part only).
-template <typename T>
+// row. Waves split the tile's work items; partial sums meet in LDS in wave
+// order. y1 = AdotV1, y2 = AdotV2, y3 = dsigma (K part only), interleaved.
+template <typename T, int NB>
 __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
     const int64_t* __restrict__ tiles, const int64_t* __restrict__ tile_ptr,
     const int64_t* __restrict__ items, const T* __restrict__ pool,
     const double* __restrict__ v1, const double* __restrict__ v2, const double* __restrict__ t,
     double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3) {
-  __shared__ double red[kTileWaves][3][64];
+  __shared__ double red[3 * NB][64];
   const int64_t tile = blockIdx.x;
   const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const bool live = lane < mt;
-  double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  double acc[3 * NB];
+#pragma unroll
+  for (int q = 0; q < 3 * NB; ++q) acc[q] = 0.0;
   for (int64_t q = tile_ptr[tile] + w; q < tile_ptr[tile + 1]; q += kTileWaves) {
     const int64_t* it = items + q * IT_W;
     const int64_t kind = it[IT_KIND], base = it[IT_BASE], ld = it[IT_LD], plane = it[IT_PLANE];
@@ -321,38 +372,66 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
     const T* p = pool + base + lane;
     if (kind == 0) {
       for (int64_t c = 0; c < cnt; ++c) {
-        const double x1 = v1[xo + c], x2 = v2[xo + c];
+        const double* x1 = v1 + (xo + c) * NB;
+        const double* x2 = v2 + (xo + c) * NB;
         const T* pc = p + c * ld;
-        a1 += (double)pc[0] * x1 + (double)pc[3 * plane] * x2;
-        a2 += (double)pc[plane] * x1 + (double)pc[4 * plane] * x2;
-        a3 += (double)pc[2 * plane] * x1 + (double)pc[5 * plane] * x2;
+        const double a0 = (double)pc[0], a1 = (double)pc[plane], a2 = (double)pc[2 * plane];
+        const double a3 = (double)pc[3 * plane], a4 = (double)pc[4 * plane];
+        const double a5 = (double)pc[5 * plane];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          acc[3 * b + 0] += a0 * x1[b] + a3 * x2[b];
+          acc[3 * b + 1] += a1 * x1[b] + a4 * x2[b];
+          acc[3 * b + 2] += a2 * x1[b] + a5 * x2[b];
+        }
       }
     } else {
       for (int64_t l = 0; l < cnt; ++l) {
-        const double tl = t[xo + l];
+        const double* tl = t + (xo + l) * NB;
         const T* pc = p + l * ld;
-        a1 += (double)pc[0] * tl;
-        a2 += (double)pc[plane] * tl;
-        a3 += (double)pc[2 * plane] * tl;
+        const double u0 = (double)pc[0], u1 = (double)pc[plane], u2 = (double)pc[2 * plane];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          acc[3 * b + 0] += u0 * tl[b];
+          acc[3 * b + 1] += u1 * tl[b];
+          acc[3 * b + 2] += u2 * tl[b];
+        }
       }
     }
   }
-  red[w][0][lane] = a1;
-  red[w][1][lane] = a2;
-  red[w][2][lane] = a3;
-  __syncthreads();
-  if (w == 0 && live) {
-    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  // deterministic cross-wave sum: wave 0 stores, waves 1.. add in order
+  for (int ww = 0; ww < kTileWaves; ++ww) {
+    if (w == ww) {
 #pragma unroll
-    for (int k = 0; k < kTileWaves; ++k) {
-      s1 += red[k][0][lane];
-      s2 += red[k][1][lane];
-      s3 += red[k][2][lane];
+      for (int q = 0; q < 3 * NB; ++q) red[q][lane] = (ww == 0) ? acc[q] : red[q][lane] + acc[q];
     }
-    y1[R0 + lane] = s1;
-    y2[R0 + lane] = s2;
-    y3[R0 + lane] = s3;
+    __syncthreads();
   }
+  if (w == 0 && live) {
+    const int64_t row = R0 + lane;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      y1[row * NB + b] = red[3 * b + 0][lane];
+      y2[row * NB + b] = red[3 * b + 1][lane];
+      y3[row * NB + b] = red[3 * b + 2][lane];
+    }
+  }
+}
+
+template <typename T, int NB>
+static void launch_matvec(const int64_t* units, int64_t nunits, const int64_t* red,
+                          int64_t nred, const int64_t* tiles, const int64_t* tile_ptr,
+                          int64_t ntiles, const int64_t* items, const T* pool, const double* v1,
+                          const double* v2, double* t, double* part, double* y1, double* y2,
+                          double* y3, hipStream_t s) {
+  if (nunits > 0)
+    k_hmv_phase1<T, NB><<<(unsigned)((nunits + 3) / 4), 256, 0, s>>>(units, nunits, pool, v1, v2,
+                                                                      t, part);
+  if (nred > 0)
+    k_hmv_phase1b<NB><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(red, nred, part, t);
+  if (ntiles > 0)
+    k_hmv_phase2<T, NB><<<(unsigned)ntiles, 64 * kTileWaves, 0, s>>>(tiles, tile_ptr, items, pool,
+                                                                     v1, v2, t, y1, y2, y3);
 }
 
 }  // namespace pq3d
@@ -362,12 +441,13 @@ using namespace pq3d;
 extern "C" {
 
 // Row widths of every table the H-matrix kernels read, in this order:
-// dense block, low-rank block, ACA workspace offsets, matvec item, V row,
-// tile, receiver record, source record. Returns the number written.
+// dense block, low-rank block, ACA workspace offsets, matvec item, phase-1
+// unit, phase-1b reduction, tile, receiver record, source record. Returns the
+// number written.
 int pq3d_layout_constants(int64_t* out, int64_t n) {
-  const int64_t v[8] = {DB_W, LB_W, WS_W, IT_W, VR_W, TL_W, kRecvW, kSrcRec};
+  const int64_t v[9] = {DB_W, LB_W, WS_W, IT_W, UN_W, RD_W, TL_W, kRecvW, kSrcRec};
   int64_t k = 0;
-  for (; k < n && k < 8; ++k) out[k] = v[k];
+  for (; k < n && k < 9; ++k) out[k] = v[k];
   return (int)k;
 }
 
@@ -427,41 +507,46 @@ int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nbl
   return PQ3D_OK;
 }
 
-// y1 = AdotV1, y2 = AdotV2, y3 = dsigma (K part) for deficits v1, v2 (all FP64,
-// cluster order). d_t: scratch of nvrows doubles.
-int pq3d_hmat_matvec(const int64_t* d_vrows, int64_t nvrows, const int64_t* d_tiles,
-                     const int64_t* d_tile_ptr, int64_t ntiles, const int64_t* d_items,
-                     const void* d_pool, int dtype_bytes, const double* d_v1,
-                     const double* d_v2, double* d_t, double* d_y1, double* d_y2, double* d_y3,
-                     void* stream) {
+// y1 = AdotV1, y2 = AdotV2, y3 = dsigma (K part) for nrhs interleaved
+// right-hand sides (all FP64, cluster order, element (row, b) at row*nrhs + b).
+// d_t: nt*nrhs doubles, d_part: npart*nrhs doubles of scratch.
+int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_red, int64_t nred,
+                     const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
+                     const int64_t* d_items, const void* d_pool, int dtype_bytes, int nrhs,
+                     const double* d_v1, const double* d_v2, double* d_t, double* d_part,
+                     double* d_y1, double* d_y2, double* d_y3, void* stream) {
   clear_error();
-  PQ3D_REQUIRE(ntiles >= 0 && nvrows >= 0, "sizes must be >= 0");
+  PQ3D_REQUIRE(ntiles >= 0 && nunits >= 0 && nred >= 0, "sizes must be >= 0");
   PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
   PQ3D_REQUIRE(d_tiles && d_tile_ptr && d_items && d_pool && d_v1 && d_v2 && d_y1 && d_y2 &&
                    d_y3,
                "null pointer");
+  PQ3D_REQUIRE(nunits == 0 || (d_units && d_t), "null pointer");
+  PQ3D_REQUIRE(nred == 0 || (d_red && d_part), "null pointer");
+  PQ3D_REQUIRE(nunits < (1LL << 33) && nred < (1LL << 33) && ntiles < (1LL << 31),
+               "too many work items");
   hipStream_t s = as_stream(stream);
-  if (nvrows > 0) {
-    PQ3D_REQUIRE(d_vrows && d_t, "null pointer");
-    const unsigned g = (unsigned)((nvrows + 3) / 4);
-    if (dtype_bytes == 4)
-      k_hmv_phase1<float><<<g, 256, 0, s>>>(d_vrows, nvrows, (const float*)d_pool, d_v1, d_v2,
-                                            d_t);
-    else
-      k_hmv_phase1<double><<<g, 256, 0, s>>>(d_vrows, nvrows, (const double*)d_pool, d_v1,
-                                             d_v2, d_t);
-    PQ3D_LAUNCH_CHECK();
+#define PQ3D_MV(NBV)                                                                        \
+  do {                                                                                      \
+    if (dtype_bytes == 4)                                                                   \
+      launch_matvec<float, NBV>(d_units, nunits, d_red, nred, d_tiles, d_tile_ptr, ntiles,  \
+                                d_items, (const float*)d_pool, d_v1, d_v2, d_t, d_part,     \
+                                d_y1, d_y2, d_y3, s);                                       \
+    else                                                                                    \
+      launch_matvec<double, NBV>(d_units, nunits, d_red, nred, d_tiles, d_tile_ptr, ntiles, \
+                                 d_items, (const double*)d_pool, d_v1, d_v2, d_t, d_part,   \
+                                 d_y1, d_y2, d_y3, s);                                      \
+  } while (0)
+  switch (nrhs) {
+    case 1: PQ3D_MV(1); break;
+    case 2: PQ3D_MV(2); break;
+    case 4: PQ3D_MV(4); break;
+    case 8: PQ3D_MV(8); break;
+    case 16: PQ3D_MV(16); break;
+    default: pq3d::set_error("nrhs must be 1, 2, 4, 8 or 16"); return PQ3D_EINVAL;
   }
-  if (ntiles > 0) {
-    if (dtype_bytes == 4)
-      k_hmv_phase2<float><<<(unsigned)ntiles, 64 * kTileWaves, 0, s>>>(
-          d_tiles, d_tile_ptr, d_items, (const float*)d_pool, d_v1, d_v2, d_t, d_y1, d_y2, d_y3);
-    else
-      k_hmv_phase2<double><<<(unsigned)ntiles, 64 * kTileWaves, 0, s>>>(
-          d_tiles, d_tile_ptr, d_items, (const double*)d_pool, d_v1, d_v2, d_t, d_y1, d_y2,
-          d_y3);
-    PQ3D_LAUNCH_CHECK();
-  }
+#undef PQ3D_MV
+  PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }
 

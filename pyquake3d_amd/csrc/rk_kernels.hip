@@ -62,6 +62,9 @@ __device__ inline void qd_rhs(double Tno, double Tt1o, double Tt2o, double state
   out[3] = dtau2dt / 1e6;
 }
 
+// Instances whose error/max slots are first reduced in LDS within a block.
+constexpr int kLdsInst = 64;
+
 // Monotone encoding of |x| for atomic max (NaN sorts above +inf).
 __device__ inline unsigned long long abs_bits(double x) {
   return (unsigned long long)__double_as_longlong(fabs(x));
@@ -101,8 +104,8 @@ __device__ inline unsigned long long abs_bits(double x) {
 // |d psi/psi8|, |d tau1/tau1_8|, |d tau2/tau2_8|, |d sigma/sigma8| (bit patterns
 // of non-negative doubles, NaN above +inf; zeroed by the entry point). Batched form: instance of cell i is
 // inst[i] (or 0 when inst is null), with its own h and error slots.
-__global__ void __launch_bounds__(256) k_rk45_attempt(
-    int64_t n, const int32_t* __restrict__ inst, const double* __restrict__ hvec,
+__device__ inline void rk45_cell(
+    int64_t i, const int32_t* __restrict__ inst, const double* __restrict__ hvec,
     const double* __restrict__ Tno, const double* __restrict__ Tt1o,
     const double* __restrict__ Tt2o, const double* __restrict__ st,
     const double* __restrict__ pa, const double* __restrict__ pb,
@@ -111,9 +114,7 @@ __global__ void __launch_bounds__(256) k_rk45_attempt(
     const double* __restrict__ dsig, const double* __restrict__ shear_loading,
     const double* __restrict__ slipv, const double* __restrict__ cellpar, double* __restrict__ yTno,
     double* __restrict__ yTt1o, double* __restrict__ yTt2o, double* __restrict__ yst,
-    unsigned long long* __restrict__ err, double B63) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+    unsigned long long* err, double B63) {
   const int g = inst ? inst[i] : 0;
   const double h = hvec[g];
   const double V0 = cellpar[3 * g + 0], mu = cellpar[3 * g + 1], Cs = cellpar[3 * g + 2];
@@ -174,6 +175,35 @@ __global__ void __launch_bounds__(256) k_rk45_attempt(
   atomicMax(&err[4 * g + 3], abs_bits(e_s));
 }
 
+__global__ void __launch_bounds__(256) k_rk45_attempt(
+    int64_t n, const int32_t* __restrict__ inst, const double* __restrict__ hvec,
+    const double* __restrict__ Tno, const double* __restrict__ Tt1o,
+    const double* __restrict__ Tt2o, const double* __restrict__ st,
+    const double* __restrict__ pa, const double* __restrict__ pb,
+    const double* __restrict__ pdc, const double* __restrict__ pf0,
+    const double* __restrict__ AdotV1, const double* __restrict__ AdotV2,
+    const double* __restrict__ dsig, const double* __restrict__ shear_loading,
+    const double* __restrict__ slipv, const double* __restrict__ cellpar, double* __restrict__ yTno,
+    double* __restrict__ yTt1o, double* __restrict__ yTt2o, double* __restrict__ yst,
+    unsigned long long* __restrict__ err, double B63, int64_t n_inst) {
+  // per-block maxima first (LDS atomics), then one global atomic per
+  // (instance, slot) per block: n cells would otherwise all hit 4 addresses
+  __shared__ unsigned long long s_err[kLdsInst * 4];
+  const bool lds = n_inst <= kLdsInst;
+  if (lds)
+    for (int t = threadIdx.x; t < n_inst * 4; t += blockDim.x) s_err[t] = 0ull;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rk45_cell(i, inst, hvec, Tno, Tt1o, Tt2o, st, pa, pb, pdc, pf0, AdotV1, AdotV2, dsig,
+                       shear_loading, slipv, cellpar, yTno, yTt1o, yTt2o, yst,
+                       lds ? s_err : err, B63);
+  __syncthreads();
+  if (lds)
+    for (int t = threadIdx.x; t < n_inst * 4; t += blockDim.x)
+      if (s_err[t]) atomicMax(&err[t], s_err[t]);
+}
+
+
 // Accept a step: state <- candidate, slip rates from the regularized law per
 // component, rake, V floor, slip (src/QDsim.py:960-993). vmax |= max V.
 __global__ void __launch_bounds__(256) k_rk45_commit(
@@ -185,30 +215,39 @@ __global__ void __launch_bounds__(256) k_rk45_commit(
     double* __restrict__ Tt2o, double* __restrict__ st, double* __restrict__ slipv1,
     double* __restrict__ slipv2, double* __restrict__ slipv, double* __restrict__ rake,
     double* __restrict__ slip1, double* __restrict__ slip2,
-    unsigned long long* __restrict__ vmax) {
+    unsigned long long* __restrict__ vmax, int64_t n_inst) {
+  __shared__ unsigned long long s_v[kLdsInst];
+  const bool lds = n_inst <= kLdsInst;
+  if (lds)
+    for (int t = threadIdx.x; t < n_inst; t += blockDim.x) s_v[t] = 0ull;
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int g = inst ? inst[i] : 0;
-  if (accept && !accept[g]) return;
-  const double h = hvec[g];
-  const double V0 = cellpar[3 * g + 0];
-  const double a = pa[i];
-  const double s = yTno[i], t1 = yTt1o[i], t2 = yTt2o[i], p = yst[i];
-  Tno[i] = s;
-  Tt1o[i] = t1;
-  Tt2o[i] = t2;
-  st[i] = p;
-  const double v1 = (2.0 * V0) * exp(-p / a) * sinh(t1 / s / a);
-  const double v2 = (2.0 * V0) * exp(-p / a) * sinh(t2 / s / a);
-  double v = sqrt(v1 * v1 + v2 * v2);
-  slipv1[i] = v1;
-  slipv2[i] = v2;
-  rake[i] = atan2(t2, t1);
-  if (v < 1e-30) v = 1e-30;
-  slipv[i] = v;
-  slip1[i] = slip1[i] + v1 * h;
-  slip2[i] = slip2[i] + v2 * h;
-  atomicMax(&vmax[g], abs_bits(v));
+  const int g = (i < n && inst) ? inst[i] : 0;
+  if (i < n && !(accept && !accept[g])) {
+    const double h = hvec[g];
+    const double V0 = cellpar[3 * g + 0];
+    const double a = pa[i];
+    const double s = yTno[i], t1 = yTt1o[i], t2 = yTt2o[i], p = yst[i];
+    Tno[i] = s;
+    Tt1o[i] = t1;
+    Tt2o[i] = t2;
+    st[i] = p;
+    const double v1 = (2.0 * V0) * exp(-p / a) * sinh(t1 / s / a);
+    const double v2 = (2.0 * V0) * exp(-p / a) * sinh(t2 / s / a);
+    double v = sqrt(v1 * v1 + v2 * v2);
+    slipv1[i] = v1;
+    slipv2[i] = v2;
+    rake[i] = atan2(t2, t1);
+    if (v < 1e-30) v = 1e-30;
+    slipv[i] = v;
+    slip1[i] = slip1[i] + v1 * h;
+    slip2[i] = slip2[i] + v2 * h;
+    atomicMax(lds ? &s_v[g] : &vmax[g], abs_bits(v));
+  }
+  __syncthreads();
+  if (lds)
+    for (int t = threadIdx.x; t < n_inst; t += blockDim.x)
+      if (s_v[t]) atomicMax(&vmax[t], s_v[t]);
 }
 
 // Slip-rate deficit (src/QDsim.py:902-903): x1 = V1 - Vpl cos(rake0),
@@ -282,7 +321,7 @@ int pq3d_rk45_attempt(int64_t n, const int32_t* d_inst, const double* d_h, const
   k_rk45_attempt<<<blocks_for(n, 256), 256, 0, s>>>(
       n, d_inst, d_h, d_Tno, d_Tt1o, d_Tt2o, d_state, d_a, d_b, d_dc, d_f0, d_AdotV1, d_AdotV2,
       d_dsigmadt, d_shear_loading, d_slipv, d_cellpar, d_yTno, d_yTt1o, d_yTt2o, d_ystate, d_err,
-      a63);
+      a63, n_inst);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }
@@ -301,7 +340,8 @@ int pq3d_rk45_commit(int64_t n, const int32_t* d_inst, const double* d_h,
   k_zero_u64<<<blocks_for(n_inst, 256), 256, 0, s>>>(d_vmax, n_inst);
   k_rk45_commit<<<blocks_for(n, 256), 256, 0, s>>>(
       n, d_inst, d_h, d_accept, d_a, d_cellpar, d_yTno, d_yTt1o, d_yTt2o, d_ystate, d_Tno,
-      d_Tt1o, d_Tt2o, d_state, d_slipv1, d_slipv2, d_slipv, d_rake, d_slip1, d_slip2, d_vmax);
+      d_Tt1o, d_Tt2o, d_state, d_slipv1, d_slipv2, d_slipv, d_rake, d_slip1, d_slip2, d_vmax,
+      n_inst);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }

@@ -33,6 +33,8 @@ from .tde import source_records
 
 DENSE_CHUNK = 32   # columns per dense work item
 LR_CHUNK = 16      # rank columns per low-rank work item
+UNIT_COLS = 512    # stacked V columns per phase-1 unit
+NRHS = (1, 2, 4, 8, 16)
 
 
 def _sync(dev):
@@ -91,7 +93,11 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
             kind 1 = low-rank chunk of ``count`` rank columns (element (c, i, l)
             of U at base + c*plane + l*ld + i, weights t[xoff + l])
     tile_ptr (T+1,): CSR offsets of each tile's items
-    vrows   (R, 4): (V row start, n, c0, t index) of every low-rank V row
+    units   (Q, 8): phase-1 units (V^T base, k, first stacked column, columns,
+            block n, block c0, output row, direct); V^T of a block is 2n x k
+            row-major at its V base
+    reduce  (R, 4): phase-1b sums (t row, k, first partial row, units)
+    nt, npart: rows of t and of the partial buffer
     """
     leaves = tree.leaves()
     lstart = tree.start[leaves]
@@ -110,13 +116,22 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
             for j0 in range(0, bn, DENSE_CHUNK):
                 cnt = min(DENSE_CHUNK, bn - j0)
                 items.append((q, 0, base + j0 * bm + ro, bm, bm * bn, cnt, bc0 + j0))
-    vrows = []
+    units, reds = [], []
     toff = 0
+    poff = 0
     for bi, ub, vb, k in zip(np.asarray(lr_idx).tolist(), np.asarray(l_ub).tolist(),
                              np.asarray(l_vb).tolist(), np.asarray(k_keep).tolist()):
         br0, bm, bc0, bn = int(r0[bi]), int(m[bi]), int(c0[bi]), int(nn[bi])
-        for l in range(k):
-            vrows.append((vb + l * 2 * bn, bn, bc0, toff + l))
+        C = 2 * bn
+        nu = -(-C // UNIT_COLS)
+        if nu == 1:
+            units.append((vb, k, 0, C, bn, bc0, toff, 1))
+        else:
+            for u in range(nu):
+                cb = u * UNIT_COLS
+                units.append((vb, k, cb, min(UNIT_COLS, C - cb), bn, bc0, poff + u * k, 0))
+            reds.append((toff, k, poff, nu))
+            poff += nu * k
         for q in range(tile_of_row[br0], tile_of_row[br0 + bm - 1] + 1):
             ro = int(lstart[q]) - br0
             for l0 in range(0, k, LR_CHUNK):
@@ -129,8 +144,9 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
     tile_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     return dict(items=np.ascontiguousarray(it[:, 1:]), tile_ptr=tile_ptr,
                 tiles=np.ascontiguousarray(np.stack([lstart, lsize], axis=1).astype(np.int64)),
-                vrows=np.ascontiguousarray(np.array(vrows, dtype=np.int64).reshape(-1, 4)),
-                nt=toff)
+                units=np.ascontiguousarray(np.array(units, dtype=np.int64).reshape(-1, 8)),
+                reduce=np.ascontiguousarray(np.array(reds, dtype=np.int64).reshape(-1, 4)),
+                nt=toff, npart=poff)
 
 
 class HMatrix:
@@ -295,29 +311,50 @@ class HMatrix:
         self.tile_ptr = torch.from_numpy(W["tile_ptr"]).to(dev)
         self.tiles = torch.from_numpy(W["tiles"]).to(dev)
         self.ntiles = W["tiles"].shape[0]
-        self.vrows = torch.from_numpy(W["vrows"]).to(dev)
-        self.nvrows = W["vrows"].shape[0]
-        self.tbuf = torch.zeros(max(self.nvrows, 1), dtype=torch.float64, device=dev)
+        self.units = torch.from_numpy(W["units"]).to(dev)
+        self.nunits = W["units"].shape[0]
+        self.reduce = torch.from_numpy(W["reduce"]).to(dev)
+        self.nred = W["reduce"].shape[0]
+        self.nt, self.npart = W["nt"], W["npart"]
+        self._scratch = {}
         _lib.check_width("item", self.items)
-        _lib.check_width("vrow", self.vrows)
+        _lib.check_width("unit", self.units)
+        _lib.check_width("reduce", self.reduce)
         _lib.check_width("tile", self.tiles)
         # algorithmic bytes of one matvec: stored operators + 2 input + 3 output vectors
         self.matvec_bytes = int(self.pool.numel()) * self.esize + 5 * 8 * self.n
 
+    def _bufs(self, nrhs):
+        if nrhs not in self._scratch:
+            self._scratch[nrhs] = (
+                torch.zeros(max(self.nt, 1) * nrhs, dtype=torch.float64, device=self.device),
+                torch.zeros(max(self.npart, 1) * nrhs, dtype=torch.float64, device=self.device))
+        return self._scratch[nrhs]
+
     # ----------------------------------------------------------------- apply
-    def matvec(self, v1, v2, out=None, stream=None):
-        """(AdotV1, AdotV2, dsigma_K) = six-operator product, cluster order, FP64."""
+    def matvec(self, v1, v2, out=None, stream=None, nrhs=1):
+        """(AdotV1, AdotV2, dsigma_K) = six-operator product, cluster order,
+        FP64. With nrhs > 1, v1/v2/out rows are interleaved by instance
+        (element (cell, b) at cell*nrhs + b) and out has shape (3, n*nrhs)."""
         n = self.n
+        if nrhs not in NRHS:
+            raise ValueError(f"nrhs must be one of {NRHS}")
         if out is None:
-            out = torch.empty((3, n), dtype=torch.float64, device=self.device)
-        assert v1.dtype == torch.float64 and v2.dtype == torch.float64
-        assert v1.is_contiguous() and v2.is_contiguous() and v1.numel() == n and v2.numel() == n
-        assert out.shape == (3, n) and out.is_contiguous()
-        _lib.call("pq3d_hmat_matvec", _lib.ptr(self.vrows), self.nvrows, _lib.ptr(self.tiles),
-                  _lib.ptr(self.tile_ptr), self.ntiles, _lib.ptr(self.items),
-                  _lib.ptr(self.pool), self.esize, _lib.ptr(v1), _lib.ptr(v2),
-                  _lib.ptr(self.tbuf), _lib.ptr(out[0]), _lib.ptr(out[1]), _lib.ptr(out[2]),
-                  _lib.stream_handle(stream))
+            out = torch.empty((3, n * nrhs), dtype=torch.float64, device=self.device)
+        if not (v1.dtype == torch.float64 and v2.dtype == torch.float64):
+            raise ValueError("v1, v2 must be float64")
+        if not (v1.is_contiguous() and v2.is_contiguous()):
+            raise ValueError("v1, v2 must be contiguous")
+        if v1.numel() != n * nrhs or v2.numel() != n * nrhs:
+            raise ValueError("v1, v2 must hold n*nrhs values")
+        if tuple(out.shape) != (3, n * nrhs) or not out.is_contiguous() or out.dtype != torch.float64:
+            raise ValueError("out must be a contiguous (3, n*nrhs) float64 tensor")
+        t, part = self._bufs(nrhs)
+        _lib.call("pq3d_hmat_matvec", _lib.ptr(self.units), self.nunits, _lib.ptr(self.reduce),
+                  self.nred, _lib.ptr(self.tiles), _lib.ptr(self.tile_ptr), self.ntiles,
+                  _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
+                  _lib.ptr(v2), _lib.ptr(t), _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]),
+                  _lib.ptr(out[2]), _lib.stream_handle(stream))
         return out
 
     def to_cluster(self, x: torch.Tensor) -> torch.Tensor:
@@ -334,7 +371,7 @@ class HMatrix:
         m, n, k = int(B["m"][bi]), int(B["n"][bi]), int(B["ranks"][q])
         ub, vb = int(B["l_ub"][q]), int(B["l_vb"][q])
         U = self.pool[ub:ub + 3 * m * k].double().cpu().numpy().reshape(k, 3 * m).T
-        V = self.pool[vb:vb + 2 * n * k].double().cpu().numpy().reshape(k, 2 * n)
+        V = self.pool[vb:vb + 2 * n * k].double().cpu().numpy().reshape(2 * n, k).T
         return U, V
 
     @classmethod

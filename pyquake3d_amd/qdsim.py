@@ -122,6 +122,8 @@ class QDsim:
         self.nsteps = 0
         self.nattempts = 0
         self.nrejects = 0
+        self.timing = False          # record HIP events around every operator product
+        self.matvec_events = []
 
     # -------------------------------------------------------------- kernels
     def _stream(self):
@@ -133,7 +135,15 @@ class QDsim:
         _lib.call("pq3d_deficit", self.n, _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
                   _lib.ptr(self.pl1), _lib.ptr(self.pl2), _lib.ptr(self.x[0]),
                   _lib.ptr(self.x[1]), s)
-        self.op.matvec(self.x[0], self.x[1], out=self.K)
+        if self.timing:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            self.op.matvec(self.x[0], self.x[1], out=self.K)
+            e1.record()
+            self.matvec_events.append((e0, e1))
+        else:
+            self.op.matvec(self.x[0], self.x[1], out=self.K)
         _lib.call("pq3d_add_loading", self.n, _lib.ptr(self.K[2]), _lib.ptr(self.normal_loading),
                   int(self.fix_Tn), s)
 
@@ -216,6 +226,11 @@ class QDsim:
         return out[0], out[1], out[2], out[3]
 
     # ------------------------------------------------------------- helpers
+    def matvec_ms(self) -> list:
+        """Durations (ms) of the recorded operator products (synchronises)."""
+        torch.cuda.synchronize()
+        return [a.elapsed_time(b) for a, b in self.matvec_events]
+
     def max_slipv(self) -> float:
         return float(self.slipv.max())
 

@@ -98,17 +98,27 @@ class Checker:
             assert ws[b, 1] + 2 * n * k <= self.T(wV).numel()
             assert 0 <= ub and ub + 3 * m * k <= P and 0 <= vb and vb + 2 * n * k <= P
 
-    def pq3d_hmat_matvec(self, vrows, nvr, tiles, tile_ptr, ntiles, items, pool, esize, v1, v2,
-                         t, y1, y2, y3, stream):
+    def pq3d_hmat_matvec(self, units, nun, red, nred, tiles, tile_ptr, ntiles, items, pool,
+                         esize, nrhs, v1, v2, t, part, y1, y2, y3, stream):
         P = self.T(pool).numel()
-        N = self.T(v1).numel()
-        assert self.T(v2).numel() == N and min(self.T(y).numel() for y in (y1, y2, y3)) >= N
-        nt = self.T(t).numel()
-        if nvr:
-            vr = rows(self.T(vrows), "vrow")
-            assert len(vr) >= nvr
-            for vb, n, c0, ti in vr[:nvr]:
-                assert vb >= 0 and vb + 2 * n <= P and c0 + n <= N and 0 <= ti < nt
+        assert nrhs in (1, 2, 4, 8, 16)
+        N = self.T(v1).numel() // nrhs
+        assert self.T(v2).numel() == N * nrhs
+        assert min(self.T(y).numel() for y in (y1, y2, y3)) >= N * nrhs
+        nt = self.T(t).numel() // nrhs
+        npart = self.T(part).numel() // nrhs
+        if nun:
+            un = rows(self.T(units), "unit")
+            assert len(un) >= nun
+            for vt, k, cb, cc, bn, c0, out, direct in un[:nun]:
+                assert k > 0 and cc > 0 and 0 <= cb and cb + cc <= 2 * bn
+                assert vt >= 0 and vt + (cb + cc) * k <= P          # V^T rows cb..cb+cc
+                assert c0 >= 0 and c0 + bn <= N
+                assert out >= 0 and out + k <= (nt if direct else npart)
+        if nred:
+            rd = rows(self.T(red), "reduce")
+            for to, k, p0, nu in rd[:nred]:
+                assert to + k <= nt and p0 >= 0 and p0 + nu * k <= npart
         tl = rows(self.T(tiles), "tile")
         tp = self.T(tile_ptr).numpy()
         it = rows(self.T(items), "item")
@@ -139,6 +149,8 @@ def test_hmatrix_device_accesses_in_bounds(monkeypatch, leaf, eta, dtype):
                              device="cpu", aca_budget_gb=0.002)
     x = torch.zeros(mesh.n, dtype=torch.float64)
     H.matvec(x, x)
+    x8 = torch.zeros(mesh.n * 8, dtype=torch.float64)
+    H.matvec(x8, x8, nrhs=8)
     assert "pq3d_hmat_matvec" in chk.calls
     if eta > 0:
         assert chk.calls.count("pq3d_hmat_aca") > 1   # the workspace budget forced batches

@@ -48,12 +48,21 @@ def test_block_partition_covers_matrix_once():
     assert P.adm.any() and (~P.adm).any()
 
 
-def emulate_matvec(W, pool, v1, v2, n):
-    t = np.zeros(max(W["nt"], 1))
-    for vb, nb, c0, ti in W["vrows"]:
-        v = pool[vb:vb + 2 * nb]
-        t[ti] = v[:nb] @ v1[c0:c0 + nb] + v[nb:] @ v2[c0:c0 + nb]
-    y = np.zeros((3, n))
+def emulate_matvec(W, pool, v1, v2, n, nb=1):
+    """Phases 1, 1b and 2 with the kernels' indexing; v1, v2 interleaved
+    (cell*nb + b)."""
+    t = np.zeros((max(W["nt"], 1), nb))
+    part = np.zeros((max(W["npart"], 1), nb))
+    X1, X2 = v1.reshape(n, nb), v2.reshape(n, nb)
+    for vt, k, cb, cc, bn, c0, out, direct in W["units"]:
+        acc = np.zeros((k, nb))
+        for c in range(cb, cb + cc):
+            x = X2[c0 + c - bn] if c >= bn else X1[c0 + c]
+            acc += np.outer(pool[vt + c * k: vt + c * k + k], x)
+        (t if direct else part)[out:out + k] = acc
+    for to, k, p0, nu in W["reduce"]:
+        t[to:to + k] = sum(part[p0 + q * k: p0 + q * k + k] for q in range(nu))
+    y = np.zeros((3, n, nb))
     items, ptr = W["items"], W["tile_ptr"]
     for q, (R0, mt) in enumerate(W["tiles"]):
         lanes = np.arange(mt)
@@ -62,16 +71,15 @@ def emulate_matvec(W, pool, v1, v2, n):
             for c in range(cnt):
                 p = base + lanes + c * ld
                 if kind == 0:
-                    x1, x2 = v1[xo + c], v2[xo + c]
-                    y[0, R0:R0 + mt] += pool[p] * x1 + pool[p + 3 * plane] * x2
-                    y[1, R0:R0 + mt] += pool[p + plane] * x1 + pool[p + 4 * plane] * x2
-                    y[2, R0:R0 + mt] += pool[p + 2 * plane] * x1 + pool[p + 5 * plane] * x2
+                    x1, x2 = X1[xo + c], X2[xo + c]
+                    for o in range(3):
+                        y[o, R0:R0 + mt] += (np.outer(pool[p + o * plane], x1) +
+                                             np.outer(pool[p + (3 + o) * plane], x2))
                 else:
                     tl = t[xo + c]
-                    y[0, R0:R0 + mt] += pool[p] * tl
-                    y[1, R0:R0 + mt] += pool[p + plane] * tl
-                    y[2, R0:R0 + mt] += pool[p + 2 * plane] * tl
-    return y
+                    for o in range(3):
+                        y[o, R0:R0 + mt] += np.outer(pool[p + o * plane], tl)
+    return y.reshape(3, n * nb)
 
 
 @pytest.mark.parametrize("leaf", [32, 64])
@@ -105,7 +113,7 @@ def test_layout_and_worklists_reproduce_dense_product(oracle, leaf):
     for bi, ub, vb, k in zip(L["lr_idx"], L["l_ub"], L["l_vb"], L["k"]):
         U, V = fac[bi]
         pool[ub:ub + U.size] = U.T.ravel()   # column l of U contiguous
-        pool[vb:vb + V.size] = V.ravel()     # row l of V contiguous
+        pool[vb:vb + V.size] = V.T.ravel()   # V stored transposed: 2n x k row-major
     W = build_worklists(t, n, r0, m, c0, nn, L["dense_idx"], L["d_base"], L["lr_idx"],
                         L["l_ub"], L["l_vb"], L["k"])
     rng = np.random.default_rng(0)
@@ -115,3 +123,11 @@ def test_layout_and_worklists_reproduce_dense_product(oracle, leaf):
     scale = np.abs(K).max() * np.abs(np.concatenate([v1, v2])).sum()
     assert np.abs(y - ref).max() <= 1e-8 * scale
     assert len(L["lr_idx"]) > 0 and len(L["dense_idx"]) > 0
+    assert len(W["reduce"]) > 0 or leaf == 64  # some V's span several units
+    # four interleaved right-hand sides
+    V1, V2 = rng.standard_normal((n, 4)), rng.standard_normal((n, 4))
+    y4 = emulate_matvec(W, pool, V1.ravel(), V2.ravel(), n, nb=4).reshape(3, n, 4)
+    for b in range(4):
+        rb = np.stack([K[0] @ V1[:, b] + K[3] @ V2[:, b], K[1] @ V1[:, b] + K[4] @ V2[:, b],
+                       K[2] @ V1[:, b] + K[5] @ V2[:, b]])
+        assert np.abs(y4[:, :, b] - rb).max() <= 1e-8 * scale
