@@ -194,6 +194,36 @@ int pq3d_rk45_commit(int64_t n, const int32_t* d_inst, const double* d_h,
                      double* d_rake, double* d_slip1, double* d_slip2,
                      unsigned long long* d_vmax, int64_t n_inst, void* stream);
 
+/* ------------------------------------------------- learned step size
+ * 6 -> 64 -> 64 -> 64 -> 1 tanh MLP in FP32. Parameters packed as
+ * W1 64x6 | b1 | W2 64x64 | b2 | W3 64x64 | b3 | W4 64 | b4 (row-major).
+ * No reference counterpart (the reference uses only its controller,
+ * src/QDsim.py:931-958); the proposal only seeds the first RK45 attempt. */
+
+/* Number of packed parameters (8833). */
+int pq3d_mlp_num_params(void);
+
+/* Forward over nsamp inputs d_X (nsamp x 6); predictions to d_out when not
+ * NULL; with d_grad != NULL also backward of 0.5 (y - T)^2: chunk samples per
+ * workgroup, per-workgroup gradient sums to d_grad (ceil(nsamp/chunk) x P)
+ * and loss sums to d_loss. */
+int pq3d_mlp_forward_backward(const float* d_prm, const float* d_X, const float* d_T,
+                              int64_t nsamp, int64_t chunk, float* d_out, float* d_grad,
+                              float* d_loss, void* stream);
+
+/* Sum the nblk gradient slots in order, divide by nsamp, write the gradient
+ * to d_gout (optional) and, if lr > 0, take Adam step t on d_prm. */
+int pq3d_mlp_adam(float* d_prm, const float* d_grad, int64_t nblk, int64_t nsamp, float* d_m,
+                  float* d_v, float lr, float beta1, float beta2, float eps, int t, float* d_gout,
+                  void* stream);
+
+/* Per-instance solver features over ne = n*nb cells (cell-major ensemble
+ * layout): d_acc[4b..4b+3] = max(log10 V + 1000), max(1000 - log10 V),
+ * sum log10(V theta / Dc), count(V > 1e-3), theta = Dc/V0 exp((psi-f0)/b). */
+int pq3d_features(int64_t ne, int nb, const double* d_slipv, const double* d_psi,
+                  const double* d_b, const double* d_f0, double V0, double* d_acc,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

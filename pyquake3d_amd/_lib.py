@@ -69,6 +69,14 @@ SIGNATURES = {
                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_double, c_double, c_double, c_void_p, c_void_p]),
     "pq3d_add_loading": (c_int, [c_int64, c_void_p, c_void_p, c_int, c_void_p]),
+    # learned step size
+    "pq3d_mlp_num_params": (c_int, []),
+    "pq3d_mlp_forward_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                                          c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pq3d_mlp_adam": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_float,
+                              c_float, c_float, c_float, c_int, c_void_p, c_void_p]),
+    "pq3d_features": (c_int, [c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                              c_void_p, c_void_p]),
 }
 
 _lib = None

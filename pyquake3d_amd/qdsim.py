@@ -256,3 +256,154 @@ class QDsim:
                 rows.append((self.nsteps, h, self.time, self.max_slipv()))
         self.htry = dt
         return rows
+
+
+class EnsembleQDsim:
+    """NB independent simulations on one fault mesh (different friction
+    fields / initial states), stepped together: one multi-RHS operator product
+    per step serves all of them (the operator is read once), and every
+    instance keeps its own step size and the reference's accept/reject
+    controller. Element (cell i, instance b) is stored at i*NB + b.
+
+    A lockstep iteration ends when every instance has accepted its step; an
+    instance that accepted early is recomputed with the same h on the retry
+    rounds of the others (same inputs, same candidate)."""
+
+    def __init__(self, problems, op, reltol: float = 1e-4, device="cuda", tableau: str = "ref"):
+        nb = len(problems)
+        if nb not in (1, 2, 4, 8, 16):
+            raise ValueError("ensemble size must be 1, 2, 4, 8 or 16")
+        n = problems[0].n
+        for p in problems:
+            if p.n != n:
+                raise ValueError("all instances must share the mesh of the operator")
+        if tableau not in ("ref", "dp5"):
+            raise ValueError("tableau must be 'ref' or 'dp5'")
+        self.a63 = -46732. / 5247. if tableau == "ref" else 46732. / 5247.
+        self.nb, self.n, self.ne = nb, n, n * nb
+        self.op = op
+        self.reltol = float(reltol)
+        dev = torch.device(device)
+        self.device = dev
+        perm = np.asarray(op.perm)
+        self.perm = perm
+
+        def inter(arrs):
+            # (nb, n) original order -> (n*nb,) cluster order, cell-major
+            A = np.stack([np.asarray(x, dtype=np.float64)[perm] for x in arrs], axis=1)
+            return torch.from_numpy(np.ascontiguousarray(A.reshape(-1))).to(dev)
+
+        inits = [p.initial_state() for p in problems]
+        (self.Tno, self.Tt1o, self.Tt2o, self.state, self.slipv1, self.slipv2,
+         self.slipv) = (inter([s[k] for s in inits]) for k in range(7))
+        self.rake = inter([p.rake for p in problems])
+        self.slip1 = torch.zeros(self.ne, dtype=torch.float64, device=dev)
+        self.slip2 = torch.zeros(self.ne, dtype=torch.float64, device=dev)
+        self.a = inter([p.a for p in problems])
+        self.b = inter([p.b for p in problems])
+        self.dc = inter([p.dc for p in problems])
+        self.f0 = inter([p.f0 for p in problems])
+        self.pl1 = inter([p.Vpl * np.cos(p.rake) for p in problems])
+        self.pl2 = inter([p.Vpl * np.sin(p.rake) for p in problems])
+        self.shear_loading = inter([p.shear_loading for p in problems])
+        self.normal_loading = inter([p.normal_loading for p in problems])
+        if len({bool(p.fix_Tn) for p in problems}) != 1:
+            raise ValueError("fix_Tn must be the same for all instances")
+        self.fix_Tn = bool(problems[0].fix_Tn)
+        self.inst = torch.from_numpy(np.tile(np.arange(nb, dtype=np.int32), n)).to(dev)
+        self.cellpar = torch.tensor([[p.V0, p.mu, p.Cs] for p in problems],
+                                    dtype=torch.float64, device=dev).reshape(-1)
+        self.y = torch.empty((4, self.ne), dtype=torch.float64, device=dev)
+        self.x = torch.empty((2, self.ne), dtype=torch.float64, device=dev)
+        self.K = torch.empty((3, self.ne), dtype=torch.float64, device=dev)
+        self.hdev = torch.zeros(nb, dtype=torch.float64, device=dev)
+        self.err = torch.zeros(4 * nb, dtype=torch.int64, device=dev)
+        self.vmax = torch.zeros(nb, dtype=torch.int64, device=dev)
+        self.err_host = torch.zeros(4 * nb, dtype=torch.int64)
+        self.vmax_host = torch.zeros(nb, dtype=torch.int64)
+        if dev.type == "cuda":
+            self.err_host = self.err_host.pin_memory()
+            self.vmax_host = self.vmax_host.pin_memory()
+        self.time = np.zeros(nb)
+        self.dt = np.full(nb, 1e-3)
+        self.nsteps = 0
+        self.nattempts = 0
+        self.nrejects = np.zeros(nb, dtype=np.int64)
+
+    def compute_K(self):
+        s = _lib.stream_handle()
+        _lib.call("pq3d_deficit", self.ne, _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
+                  _lib.ptr(self.pl1), _lib.ptr(self.pl2), _lib.ptr(self.x[0]),
+                  _lib.ptr(self.x[1]), s)
+        self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nb)
+        _lib.call("pq3d_add_loading", self.ne, _lib.ptr(self.K[2]),
+                  _lib.ptr(self.normal_loading), int(self.fix_Tn), s)
+
+    def _attempt(self, h):
+        self.hdev.copy_(torch.from_numpy(np.asarray(h, dtype=np.float64)))
+        _lib.call("pq3d_rk45_attempt", self.ne, _lib.ptr(self.inst), _lib.ptr(self.hdev),
+                  _lib.ptr(self.Tno), _lib.ptr(self.Tt1o), _lib.ptr(self.Tt2o),
+                  _lib.ptr(self.state), _lib.ptr(self.a), _lib.ptr(self.b), _lib.ptr(self.dc),
+                  _lib.ptr(self.f0), _lib.ptr(self.K[0]), _lib.ptr(self.K[1]),
+                  _lib.ptr(self.K[2]), _lib.ptr(self.shear_loading), _lib.ptr(self.slipv),
+                  _lib.ptr(self.cellpar), _lib.ptr(self.y[0]), _lib.ptr(self.y[1]),
+                  _lib.ptr(self.y[2]), _lib.ptr(self.y[3]), _lib.ptr(self.err), self.a63,
+                  self.nb, _lib.stream_handle())
+        self.err_host.copy_(self.err, non_blocking=True)
+        if self.device.type == "cuda":
+            torch.cuda.current_stream().synchronize()
+        self.nattempts += 1
+        return self.err_host.numpy().view(np.float64).reshape(self.nb, 4)
+
+    def step(self):
+        """One accepted step of every instance (the reference controller per
+        instance); returns the accepted h per instance."""
+        self.compute_K()
+        h = self.dt.copy()
+        done = np.zeros(self.nb, dtype=bool)
+        nrj = np.zeros(self.nb, dtype=np.int64)
+        dtnext = np.zeros(self.nb)
+        tol = self.reltol
+        while not done.all():
+            e = self._attempt(h)
+            for b in np.where(~done)[0]:
+                R1 = np.float64(e[b, 0]) + 1e-10
+                R2 = max(np.float64(e[b, 1]), np.float64(e[b, 2]), np.float64(e[b, 3])) + 1e-10
+                c1, c2 = R1 / tol, R2 / tol
+                hn1 = h[b] * 0.9 * (tol / R1) ** 0.2
+                hn2 = h[b] * 0.9 * (tol / R2) ** 0.2
+                if max(c1, c2) < 1.0 and not (np.isnan(c1) or np.isnan(c2)):
+                    dtnext[b] = min(min(hn1, hn2), 1.5 * h[b])
+                    done[b] = True
+                else:
+                    nrj[b] += 1
+                    h[b] = max(0.5 * h[b], min(hn1, hn2))
+                    if h[b] < 1e-15 or nrj[b] > 20:
+                        raise StepAborted(f"instance {b}: dt is too small (h={h[b]:g})")
+        self.nrejects += nrj
+        self.hdev.copy_(torch.from_numpy(h))
+        _lib.call("pq3d_rk45_commit", self.ne, _lib.ptr(self.inst), _lib.ptr(self.hdev), 0,
+                  _lib.ptr(self.a), _lib.ptr(self.cellpar), _lib.ptr(self.y[0]),
+                  _lib.ptr(self.y[1]), _lib.ptr(self.y[2]), _lib.ptr(self.y[3]),
+                  _lib.ptr(self.Tno), _lib.ptr(self.Tt1o), _lib.ptr(self.Tt2o),
+                  _lib.ptr(self.state), _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
+                  _lib.ptr(self.slipv), _lib.ptr(self.rake), _lib.ptr(self.slip1),
+                  _lib.ptr(self.slip2), _lib.ptr(self.vmax), self.nb, _lib.stream_handle())
+        self.time += h
+        self.dt = dtnext
+        self.nsteps += 1
+        return h
+
+    def max_slipv(self) -> np.ndarray:
+        """Max slip rate of every instance after the last commit."""
+        self.vmax_host.copy_(self.vmax)
+        if self.device.type == "cuda":
+            torch.cuda.current_stream().synchronize()
+        return self.vmax_host.numpy().view(np.float64).copy()
+
+    def instance(self, name: str, b: int) -> np.ndarray:
+        """Field ``name`` of instance b in original element order."""
+        x = getattr(self, name).reshape(self.n, self.nb)[:, b].cpu().numpy()
+        out = np.empty(self.n)
+        out[self.perm] = x
+        return out

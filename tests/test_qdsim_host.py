@@ -18,12 +18,13 @@ class IdentityOp:
         self.perm = np.arange(n)
         self.K = K
 
-    def matvec(self, v1, v2, out=None):
+    def matvec(self, v1, v2, out=None, nrhs=1):
         K = self.K
-        v1, v2 = v1.numpy(), v2.numpy()
+        n = K["A1s"].shape[0]
+        v1, v2 = v1.numpy().reshape(n, nrhs), v2.numpy().reshape(n, nrhs)
         y = np.stack([K["A1s"] @ v1 + K["A1d"] @ v2, K["A2s"] @ v1 + K["A2d"] @ v2,
                       K["Bs"] @ v1 + K["Bd"] @ v2])
-        out.copy_(torch.from_numpy(y))
+        out.copy_(torch.from_numpy(y.reshape(3, n * nrhs)))
         return out
 
 
@@ -64,8 +65,11 @@ class StepChecker:
 
     def pq3d_rk45_attempt(self, n, inst, h, Tno, Tt1o, Tt2o, st, a, b, dc, f0, A1, A2, ds, sl, V,
                           cp, y0, y1, y2, y3, err, a63, ninst, stream):
-        assert inst is None or inst == 0
         self._cells(n, Tno, Tt1o, Tt2o, st, a, b, dc, f0, A1, A2, ds, sl, V, y0, y1, y2, y3)
+        if inst is not None and inst != 0:
+            ii = self.T(inst)
+            assert ii.numel() >= n and int(ii.max()) < ninst and int(ii.min()) >= 0
+            return self._ensemble_attempt(n, ii, ninst, err)
         assert self.T(h).numel() >= ninst and self.T(cp).numel() >= 3 * ninst
         assert self.T(err).numel() >= 4 * ninst
         o = self.ora
@@ -80,6 +84,10 @@ class StepChecker:
         # error slots as the kernel writes them: separate maxima, bit patterns
         e = self._errs
         self.T(err)[:4] = torch.from_numpy(np.array(e, dtype=np.float64).view(np.int64))
+
+    def _ensemble_attempt(self, n, inst, ninst, err):
+        assert self.T(err).numel() >= 4 * ninst
+        self.T(err)[:4 * ninst] = 0
 
     def pq3d_rk45_commit(self, n, inst, h, acc, a, cp, y0, y1, y2, y3, Tno, Tt1o, Tt2o, st, s1,
                          s2, V, rake, sl1, sl2, vmax, ninst, stream):
@@ -119,3 +127,21 @@ def test_stepper_argument_lists_and_controller(monkeypatch, oracle):
     assert h == hs[2] and sim.nrejects == 2
     assert dtn == pytest.approx(min(1.5 * h, h * 0.9 * (1e-4 / (1e-6 + 1e-10)) ** 0.2,
                                     h * 0.9 * (1e-4 / (2e-7 + 1e-10)) ** 0.2))
+
+
+def test_ensemble_argument_lists(monkeypatch, oracle):
+    from oracle.qdsim_oracle import dense_operators
+    from pyquake3d_amd.mesh import get_eleVec
+    from pyquake3d_amd.qdsim import EnsembleQDsim
+    p = planar_c1(nx=6, nz=3)
+    ev, xg = get_eleVec(p.mesh.nodelst, p.mesh.elelst)
+    K = dense_operators(oracle, xg, ev, p.mesh.triangles(), p.mu, p.lambda_)
+    ora = OracleSim(K, p.a, p.b, p.dc, p.f0, p.V0, p.mu, p.Cs, p.Vpl, p.rake, p.initial_state())
+    chk = StepChecker(ora)
+    monkeypatch.setattr(_lib, "call", chk.call)
+    monkeypatch.setattr(_lib, "ptr", chk.ptr)
+    monkeypatch.setattr(_lib, "stream_handle", lambda s=None: None)
+    ens = EnsembleQDsim([p] * 4, IdentityOp(p.n, K), device="cpu")
+    h = ens.step()      # zero errors -> every instance accepts at once
+    assert np.all(h == 1e-3) and ens.nattempts == 1
+    assert ens.max_slipv().shape == (4,)
