@@ -286,10 +286,52 @@ __global__ void __launch_bounds__(256) k_aca_pack(const int64_t* __restrict__ ta
 // NB right-hand sides at once (an ensemble sharing the operator). Vectors are
 // interleaved by instance: x[j*NB + b]. Every stored operator element is read
 // once per product whatever NB is; x and t values are wave-uniform
-// (scalar-cache broadcasts). Accumulation is FP64.
+// (scalar-cache broadcasts). Accumulation is FP64 by fused multiply-adds in an
+// order that does not depend on NB, so every right-hand side of a batched
+// product equals the single-vector product bit for bit.
+
+// Index of the wavefront within its workgroup, as a wave-uniform value: table
+// rows and vector offsets derived from it then go through scalar loads and
+// scalar address arithmetic.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
+// Column loops are unrolled so that a group's loads are all issued before its
+// multiply-adds (several loads in flight per wave). A larger NB means more
+// arithmetic and more live registers per loaded value, hence less unrolling.
+template <int NB>
+constexpr int p1_unroll() { return NB <= 2 ? 8 : NB <= 4 ? 4 : NB <= 8 ? 2 : 1; }
+template <int NB>
+constexpr int p2_unroll() { return NB <= 2 ? 8 : NB <= 4 ? 4 : NB <= 8 ? 2 : 1; }
 
 // Phase 1: t[l][b] = sum over the unit's stacked columns c of V^T[c][l] x_b[c]
-// (x = v1 for c < n, v2 for c >= n); one wavefront per unit, lane <-> l.
+// (x = v1 for c < n, v2 for c >= n); one wavefront per unit, lane <-> l; the
+// terms of every (l, b) are added in order of c.
+template <typename T, int NB>
+__device__ __forceinline__ void p1_columns(const T* __restrict__ vc, int lc, int64_t k,
+                                           int64_t len, const double* __restrict__ xs,
+                                           double (&acc)[NB]) {
+  constexpr int U = p1_unroll<NB>();
+  int64_t q = 0;
+  for (; q + U <= len; q += U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = (vc + (q + u) * k)[lc];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double vu = (double)v[u];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[b] = fma(vu, xs[(q + u) * NB + b], acc[b]);
+    }
+  }
+  for (; q < len; ++q) {
+    const double vu = (double)(vc + q * k)[lc];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = fma(vu, xs[q * NB + b], acc[b]);
+  }
+}
+
 template <typename T, int NB>
 __global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ units,
                                                     int64_t nunits, const T* __restrict__ pool,
@@ -297,28 +339,28 @@ __global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ 
                                                     const double* __restrict__ v2,
                                                     double* __restrict__ t,
                                                     double* __restrict__ part) {
-  const int64_t u = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int64_t u = (int64_t)blockIdx.x * (blockDim.x / 64) + wave_id();
   const int lane = threadIdx.x & 63;
   if (u >= nunits) return;
   const int64_t* it = units + u * UN_W;
   const int64_t vt = it[UN_VT], k = it[UN_K], cb = it[UN_CB], cc = it[UN_CC], n = it[UN_N];
   const int64_t c0 = it[UN_C0], out = it[UN_OUT];
   double* dst = it[UN_DIRECT] ? t : part;
+  const int64_t ce = cb + cc;
+  const int64_t e1 = ce < n ? ce : n;   // strike columns [cb, e1) read v1
+  const int64_t s2 = cb > n ? cb : n;   // dip columns [s2, ce) read v2
   for (int64_t l0 = 0; l0 < k; l0 += 64) {
     const int64_t l = l0 + lane;
-    const bool live = l < k;
+    const int lc = (int)((l < k ? l : k - 1) - l0);  // idle lanes re-read row k-1, never stored
+    const T* vb = pool + vt + l0;
     double acc[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = 0.0;
-    for (int64_t q = 0; q < cc; ++q) {
-      const int64_t c = cb + q;
-      const bool dip = c >= n;
-      const double* xs = (dip ? v2 : v1) + (c0 + (dip ? c - n : c)) * NB;
-      const double v = live ? (double)pool[vt + c * k + l] : 0.0;
-#pragma unroll
-      for (int b = 0; b < NB; ++b) acc[b] += v * xs[b];
-    }
-    if (live) {
+    if (cb < e1)
+      p1_columns<T, NB>(vb + cb * k, lc, k, e1 - cb, v1 + (c0 + cb) * NB, acc);
+    if (s2 < ce)
+      p1_columns<T, NB>(vb + s2 * k, lc, k, ce - s2, v2 + (c0 + s2 - n) * NB, acc);
+    if (l < k) {
 #pragma unroll
       for (int b = 0; b < NB; ++b) dst[(out + l) * NB + b] = acc[b];
     }
@@ -326,41 +368,58 @@ __global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ 
 }
 
 // Phase 1b: blocks split over several units: t = sum of their partial rows in
-// unit order (deterministic).
+// unit order (deterministic). Element e = l*NB + b of unit q sits at
+// part[(p0 + q*k)*NB + e]; loads are issued eight units at a time.
 template <int NB>
 __global__ void __launch_bounds__(256) k_hmv_phase1b(const int64_t* __restrict__ red,
                                                      int64_t nred, const double* __restrict__ part,
                                                      double* __restrict__ t) {
-  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + wave_id();
   const int lane = threadIdx.x & 63;
   if (r >= nred) return;
   const int64_t* it = red + r * RD_W;
   const int64_t to = it[RD_T], k = it[RD_K], p0 = it[RD_P], nu = it[RD_NU];
-  for (int64_t e = lane; e < k * NB; e += 64) {
-    const int64_t l = e / NB, b = e - l * NB;
+  const int64_t qs = k * NB;
+  for (int64_t e = lane; e < qs; e += 64) {
+    const double* pp = part + p0 * NB + e;
     double s = 0.0;
-    for (int64_t q = 0; q < nu; ++q) s += part[(p0 + q * k + l) * NB + b];
-    t[(to + l) * NB + b] = s;
+    int64_t q = 0;
+    for (; q + 8 <= nu; q += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = pp[(q + u) * qs];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; q < nu; ++q) s += pp[q * qs];
+    t[to * NB + e] = s;
   }
 }
 
 constexpr int kTileWaves = 4;
 
 // Phase 2: one workgroup per row tile (a leaf cluster of <= 64 rows); lane <->
-// row. Waves split the tile's work items; partial sums meet in LDS in wave
-// order. y1 = AdotV1, y2 = AdotV2, y3 = dsigma (K part only), interleaved.
+// row. Every item is three row planes (outputs y1, y2, y3) of cnt columns
+// times weights w[xoff + j] (kind 0: v1, dense strike operators; kind 2: v2,
+// dense dip operators; kind 1: t, low-rank U). Waves split the tile's items;
+// partial sums meet in LDS in wave order. y1 = AdotV1, y2 = AdotV2,
+// y3 = dsigma (K part only), interleaved.
 template <typename T, int NB>
 __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
     const int64_t* __restrict__ tiles, const int64_t* __restrict__ tile_ptr,
     const int64_t* __restrict__ items, const T* __restrict__ pool,
     const double* __restrict__ v1, const double* __restrict__ v2, const double* __restrict__ t,
     double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3) {
+  constexpr int U = p2_unroll<NB>();
   __shared__ double red[3 * NB][64];
   const int64_t tile = blockIdx.x;
   const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_id();
   const int lane = threadIdx.x & 63;
   const bool live = lane < mt;
+  // lanes past the tile's last row repeat that row (in bounds, never stored),
+  // which keeps the item loop free of divergent branches
+  const int row = live ? lane : (int)mt - 1;
   double acc[3 * NB];
 #pragma unroll
   for (int q = 0; q < 3 * NB; ++q) acc[q] = 0.0;
@@ -368,34 +427,37 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
     const int64_t* it = items + q * IT_W;
     const int64_t kind = it[IT_KIND], base = it[IT_BASE], ld = it[IT_LD], plane = it[IT_PLANE];
     const int64_t cnt = it[IT_COUNT], xo = it[IT_XOFF];
-    if (!live) continue;
-    const T* p = pool + base + lane;
-    if (kind == 0) {
-      for (int64_t c = 0; c < cnt; ++c) {
-        const double* x1 = v1 + (xo + c) * NB;
-        const double* x2 = v2 + (xo + c) * NB;
-        const T* pc = p + c * ld;
-        const double a0 = (double)pc[0], a1 = (double)pc[plane], a2 = (double)pc[2 * plane];
-        const double a3 = (double)pc[3 * plane], a4 = (double)pc[4 * plane];
-        const double a5 = (double)pc[5 * plane];
+    const T* p = pool + base;  // wave-uniform; lanes add their row
+    const double* xs = (kind == 0 ? v1 : kind == 1 ? t : v2) + xo * NB;
+    int64_t c = 0;
+    for (; c + U <= cnt; c += U) {
+      T a[U][3];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) a[u][o] = (p + (c + u) * ld + o * plane)[row];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double a0 = (double)a[u][0], a1 = (double)a[u][1], a2 = (double)a[u][2];
+        const double* x = xs + (c + u) * NB;
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          acc[3 * b + 0] += a0 * x1[b] + a3 * x2[b];
-          acc[3 * b + 1] += a1 * x1[b] + a4 * x2[b];
-          acc[3 * b + 2] += a2 * x1[b] + a5 * x2[b];
+          acc[3 * b + 0] = fma(a0, x[b], acc[3 * b + 0]);
+          acc[3 * b + 1] = fma(a1, x[b], acc[3 * b + 1]);
+          acc[3 * b + 2] = fma(a2, x[b], acc[3 * b + 2]);
         }
       }
-    } else {
-      for (int64_t l = 0; l < cnt; ++l) {
-        const double* tl = t + (xo + l) * NB;
-        const T* pc = p + l * ld;
-        const double u0 = (double)pc[0], u1 = (double)pc[plane], u2 = (double)pc[2 * plane];
+    }
+    for (; c < cnt; ++c) {
+      const double a0 = (double)(p + c * ld)[row];
+      const double a1 = (double)(p + c * ld + plane)[row];
+      const double a2 = (double)(p + c * ld + 2 * plane)[row];
+      const double* x = xs + c * NB;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          acc[3 * b + 0] += u0 * tl[b];
-          acc[3 * b + 1] += u1 * tl[b];
-          acc[3 * b + 2] += u2 * tl[b];
-        }
+      for (int b = 0; b < NB; ++b) {
+        acc[3 * b + 0] = fma(a0, x[b], acc[3 * b + 0]);
+        acc[3 * b + 1] = fma(a1, x[b], acc[3 * b + 1]);
+        acc[3 * b + 2] = fma(a2, x[b], acc[3 * b + 2]);
       }
     }
   }
@@ -408,12 +470,12 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
     __syncthreads();
   }
   if (w == 0 && live) {
-    const int64_t row = R0 + lane;
+    const int64_t r = R0 + lane;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      y1[row * NB + b] = red[3 * b + 0][lane];
-      y2[row * NB + b] = red[3 * b + 1][lane];
-      y3[row * NB + b] = red[3 * b + 2][lane];
+      y1[r * NB + b] = red[3 * b + 0][lane];
+      y2[r * NB + b] = red[3 * b + 1][lane];
+      y3[r * NB + b] = red[3 * b + 2][lane];
     }
   }
 }

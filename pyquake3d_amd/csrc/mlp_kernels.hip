@@ -111,11 +111,9 @@ __global__ void __launch_bounds__(64) k_mlp(const float* __restrict__ prm,
     for (int i = 0; i < kH; ++i) g[oW3 + i * kH + j] = gW3[i];
     g[oB3 + j] = gb3;
     g[oW4 + j] = gW4;
-    const float b4 = wave_sum(gb4) / 64.f;  // every lane holds the same gb4
-    const float ls = wave_sum(lsum) / 64.f;
-    if (j == 0) {
-      g[oB4] = b4;
-      loss[blockIdx.x] = ls;
+    if (j == 0) {  // every lane holds the same gb4 and lsum
+      g[oB4] = gb4;
+      loss[blockIdx.x] = lsum;
     }
   }
 }
@@ -146,32 +144,52 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float* __restrict__ prm,
 
 // Per-instance solver-state features over the cells of each instance
 // (element e belongs to instance e % nb, cell-major layout):
-//   f[b] = {max log10 V, min log10 V, mean log10(V theta/Dc), frac(V > 1e-3)}
+//   {max log10 V, min log10 V, sum log10(V theta/Dc), count(V > 1e-3)}
 // with theta = Dc/V0 exp((psi - f0)/b) (src/QDsim.py:642 steady-state law).
-// Accumulated as (max, max of -x, sum, count) per instance in FP64 slots.
+// Integer accumulation keeps the result independent of the order of the
+// atomics: slots 0/1 hold the largest bit pattern of (1000 + log10 V) and
+// (1000 - log10 V) (positive doubles order like their bit patterns; log10 V is
+// clamped to [-999, 999]), slot 2 the sum of log10(V theta/Dc) in fixed point
+// (2^-32 units), slot 3 the count. Grid-stride over elements, LDS per block,
+// one global atomic per block and slot.
+constexpr int kFeatMaxInst = 1024;
+constexpr double kFeatScale = 4294967296.0;  // 2^32
+
 __global__ void __launch_bounds__(256) k_features(int64_t ne, int nb,
                                                   const double* __restrict__ slipv,
                                                   const double* __restrict__ psi,
                                                   const double* __restrict__ pb,
                                                   const double* __restrict__ pf0, double V0,
-                                                  double* __restrict__ acc) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const int b = (int)(e % nb);
-  const double V = slipv[e];
-  const double lv = log10(V);
-  // log10(V theta / Dc) = log10(V/V0) + (psi - f0)/(b ln 10)
-  const double lvt = log10(V / V0) + (psi[e] - pf0[e]) / (pb[e] * 2.302585092994046);
-  // max of lv and of -lv via monotone integer encoding of (lv + 1000) >= 0
-  atomicMax((unsigned long long*)&acc[4 * b + 0], (unsigned long long)__double_as_longlong(lv + 1000.0));
-  atomicMax((unsigned long long*)&acc[4 * b + 1], (unsigned long long)__double_as_longlong(1000.0 - lv));
-  atomicAdd(&acc[4 * b + 2], lvt);
-  if (V > 1e-3) atomicAdd(&acc[4 * b + 3], 1.0);
+                                                  unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long sh[4 * kFeatMaxInst];
+  for (int q = threadIdx.x; q < 4 * nb; q += blockDim.x) sh[q] = 0ull;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
+    const int b = (int)(e % nb);
+    const double V = slipv[e];
+    double lv = log10(V);
+    lv = fmax(fmin(lv, 999.0), -999.0);
+    // log10(V theta / Dc) = log10(V/V0) + (psi - f0)/(b ln 10)
+    const double lvt = log10(V / V0) + (psi[e] - pf0[e]) / (pb[e] * 2.302585092994046);
+    atomicMax(&sh[4 * b + 0], (unsigned long long)__double_as_longlong(1000.0 + lv));
+    atomicMax(&sh[4 * b + 1], (unsigned long long)__double_as_longlong(1000.0 - lv));
+    atomicAdd(&sh[4 * b + 2], (unsigned long long)llrint(lvt * kFeatScale));
+    if (V > 1e-3) atomicAdd(&sh[4 * b + 3], 1ull);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 4 * nb; q += blockDim.x) {
+    const int slot = q & 3;
+    if (slot < 2)
+      atomicMax(&acc[q], sh[q]);
+    else
+      atomicAdd(&acc[q], sh[q]);
+  }
 }
 
-__global__ void k_zero_f64(double* p, int64_t n) {
+__global__ void k_feat_clear(unsigned long long* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
+  if (i < n) p[i] = 0ull;
 }
 
 }  // namespace pq3d
@@ -216,18 +234,23 @@ int pq3d_mlp_adam(float* d_prm, const float* d_grad, int64_t nblk, int64_t nsamp
 }
 
 // Per-instance features of an ensemble state (see k_features); d_acc: 4*nb
-// doubles, overwritten. Host finishing: max = acc0 - 1000, min = 1000 - acc1,
-// mean = acc2 / n_cells, frac = acc3 / n_cells (acc0/acc1 hold bit patterns).
+// 64-bit integers, overwritten. Host finishing: max = bits(acc0) - 1000,
+// min = 1000 - bits(acc1), mean = (int64)acc2 / 2^32 / n_cells,
+// frac = acc3 / n_cells.
 int pq3d_features(int64_t ne, int nb, const double* d_slipv, const double* d_psi,
-                  const double* d_b, const double* d_f0, double V0, double* d_acc,
+                  const double* d_b, const double* d_f0, double V0, uint64_t* d_acc,
                   void* stream) {
   clear_error();
-  PQ3D_REQUIRE(ne >= 0 && nb >= 1, "bad sizes");
+  PQ3D_REQUIRE(ne >= 0 && nb >= 1 && nb <= kFeatMaxInst, "nb must be in [1, 1024]");
   PQ3D_REQUIRE(d_slipv && d_psi && d_b && d_f0 && d_acc, "null pointer");
   hipStream_t s = as_stream(stream);
-  k_zero_f64<<<blocks_for(4 * nb, 256), 256, 0, s>>>(d_acc, 4 * nb);
-  if (ne > 0)
-    k_features<<<blocks_for(ne, 256), 256, 0, s>>>(ne, nb, d_slipv, d_psi, d_b, d_f0, V0, d_acc);
+  k_feat_clear<<<blocks_for(4 * nb, 256), 256, 0, s>>>((unsigned long long*)d_acc, 4 * nb);
+  if (ne > 0) {
+    const int64_t want = (ne + 255) / 256;
+    const unsigned grid = (unsigned)(want < 512 ? want : 512);
+    k_features<<<grid, 256, 0, s>>>(ne, nb, d_slipv, d_psi, d_b, d_f0, V0,
+                                    (unsigned long long*)d_acc);
+  }
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }

@@ -88,10 +88,11 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
 
     tiles   (T, 2): (first row, rows) of every leaf cluster (<= 64 rows)
     items   (I, 6): (kind, base, ld, plane, count, xoff), sorted by tile;
-            kind 0 = dense chunk of ``count`` columns starting at column
-            ``xoff`` (element (op, i, j) at base + op*plane + j*ld + i),
-            kind 1 = low-rank chunk of ``count`` rank columns (element (c, i, l)
-            of U at base + c*plane + l*ld + i, weights t[xoff + l])
+            every item is three row planes (outputs y1, y2, y3) of ``count``
+            columns, element (o, i, j) at base + o*plane + j*ld + i, times
+            weights w[xoff + j]: kind 0 = strike operators of a dense chunk
+            (w = v1), kind 2 = its dip operators (w = v2), kind 1 = low-rank
+            chunk of U (w = t)
     tile_ptr (T+1,): CSR offsets of each tile's items
     units   (Q, 8): phase-1 units (V^T base, k, first stacked column, columns,
             block n, block c0, output row, direct); V^T of a block is 2n x k
@@ -116,6 +117,7 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
             for j0 in range(0, bn, DENSE_CHUNK):
                 cnt = min(DENSE_CHUNK, bn - j0)
                 items.append((q, 0, base + j0 * bm + ro, bm, bm * bn, cnt, bc0 + j0))
+                items.append((q, 2, base + 3 * bm * bn + j0 * bm + ro, bm, bm * bn, cnt, bc0 + j0))
     units, reds = [], []
     toff = 0
     poff = 0

@@ -25,6 +25,18 @@ from . import _lib
 from .qdsim import EnsembleQDsim, StepAborted
 
 NFEAT = 6
+FEAT_SCALE = 2.0 ** 32
+
+
+def decode_features(acc: np.ndarray, ncell: int):
+    """Finish the integer slots written by pq3d_features (nb x 4 int64):
+    max log10 V, min log10 V, mean log10(V theta/Dc), fraction V > 1e-3."""
+    acc = np.ascontiguousarray(acc, dtype=np.int64)
+    vmax = acc[:, 0].view(np.float64) - 1000.0
+    vmin = 1000.0 - acc[:, 1].view(np.float64)
+    mean = acc[:, 2].astype(np.float64) / FEAT_SCALE / ncell
+    frac = acc[:, 3].astype(np.float64) / ncell
+    return vmax, vmin, mean, frac
 
 
 class StepMLP:
@@ -130,7 +142,7 @@ class FeatureEnsemble(EnsembleQDsim):
         super().__init__(problems, op, **kw)
         self.mlp = mlp
         self.growth_cap = growth_cap
-        self.acc = torch.zeros(4 * self.nb, dtype=torch.float64, device=self.device)
+        self.acc = torch.zeros(4 * self.nb, dtype=torch.int64, device=self.device)
         self.V0 = float(problems[0].V0)
         self.last_dt = np.full(self.nb, 1e-3)
         self.last_ratio = np.full(self.nb, 1e-2)
@@ -144,11 +156,7 @@ class FeatureEnsemble(EnsembleQDsim):
                   _lib.ptr(self.b), _lib.ptr(self.f0), self.V0, _lib.ptr(self.acc),
                   _lib.stream_handle())
         a = self.acc.cpu().numpy().reshape(self.nb, 4)
-        # slots 0/1 hold max(log10 V + 1000) and max(1000 - log10 V) (exact bits)
-        vmax = a[:, 0] - 1000.0
-        vmin = 1000.0 - a[:, 1]
-        mean = a[:, 2] / self.n
-        frac = a[:, 3] / self.n
+        vmax, vmin, mean, frac = decode_features(a, self.n)
         return np.stack([vmax, vmin, mean, np.log10(self.last_dt), np.log10(self.last_ratio),
                          frac], axis=1)
 

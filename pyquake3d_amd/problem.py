@@ -120,14 +120,19 @@ def gaussian_field(xg, corr, seed, grid=256):
 
 
 def undulating_c3(nx=448, nz=112, seed=0, nucleation=True, mu=32e9, nu=0.25, Vs=3464.0,
-                  sigma=25.0, Vpl=1e-9) -> Problem:
+                  sigma=25.0, Vpl=1e-9, friction_seed=None, mesh=None) -> Problem:
     """BP5-like heterogeneous friction on the undulating 100 x 25 km fault.
     a, b, Dc: BP5 values (a_VW 0.004, a_VS 0.04, b 0.03, Dc 0.14 m) with a
     VW region x in [-30, 30] km, depth 3..18 km (2 km tapers), perturbed by
     seeded Gaussian fields (5 km correlation): a * (1 + 0.1 g1),
-    b * (1 + 0.05 g2), Dc * exp(0.2 g3)."""
-    mesh = undulating_fault(length=100e3, depth=25e3, nx=nx, nz=nz, amplitude=300.0,
-                            wavelength=5e3, seed=seed)
+    b * (1 + 0.05 g2), Dc * exp(0.2 g3). The mesh follows ``seed``; the
+    friction draw and nucleation patch follow ``friction_seed`` (default
+    ``seed``), so that draws can share one mesh (pass it as ``mesh``)."""
+    if mesh is None:
+        mesh = undulating_fault(length=100e3, depth=25e3, nx=nx, nz=nz, amplitude=300.0,
+                                wavelength=5e3, seed=seed)
+    if friction_seed is not None:
+        seed = friction_seed
     lam = 2 * mu * nu / (1 - 2 * nu)
     _, xg = get_eleVec(mesh.nodelst, mesh.elelst)
     n = mesh.n

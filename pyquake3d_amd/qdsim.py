@@ -329,13 +329,23 @@ class EnsembleQDsim:
         self.nsteps = 0
         self.nattempts = 0
         self.nrejects = np.zeros(nb, dtype=np.int64)
+        self.timing = False          # record HIP events around every operator product
+        self.matvec_events = []
 
     def compute_K(self):
         s = _lib.stream_handle()
         _lib.call("pq3d_deficit", self.ne, _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
                   _lib.ptr(self.pl1), _lib.ptr(self.pl2), _lib.ptr(self.x[0]),
                   _lib.ptr(self.x[1]), s)
-        self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nb)
+        if self.timing:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nb)
+            e1.record()
+            self.matvec_events.append((e0, e1))
+        else:
+            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nb)
         _lib.call("pq3d_add_loading", self.ne, _lib.ptr(self.K[2]),
                   _lib.ptr(self.normal_loading), int(self.fix_Tn), s)
 
@@ -393,6 +403,11 @@ class EnsembleQDsim:
         self.dt = dtnext
         self.nsteps += 1
         return h
+
+    def matvec_ms(self) -> list:
+        """Durations (ms) of the recorded operator products (synchronises)."""
+        torch.cuda.synchronize()
+        return [a.elapsed_time(b) for a, b in self.matvec_events]
 
     def max_slipv(self) -> np.ndarray:
         """Max slip rate of every instance after the last commit."""

@@ -1,0 +1,105 @@
+"""Step-size MLP kernels (forward, backward, Adam) and the feature reduction
+against the numpy oracle of the same network and numpy reductions."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp_oracle
+from pyquake3d_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _mlp(gpu, seed=0):
+    from pyquake3d_amd.learned_dt import StepMLP
+    return StepMLP(device=gpu, seed=seed)
+
+
+def test_parameter_count_matches_oracle(gpu, pq3d_lib):
+    assert _mlp(gpu).P == mlp_oracle.P == 8833
+
+
+@pytest.mark.parametrize("nsamp,chunk", [(1, 64), (300, 64), (257, 16)])
+def test_forward_backward_match_oracle(gpu, pq3d_lib, nsamp, chunk):
+    """FP32 kernels vs FP64 numpy: loss and every gradient entry within 1e-4
+    of the gradient's largest magnitude (ragged last chunk included)."""
+    net = _mlp(gpu, seed=1)
+    rng = np.random.default_rng(nsamp)
+    X = rng.standard_normal((nsamp, 6)).astype(np.float32)
+    T = rng.standard_normal(nsamp).astype(np.float32)
+    p = net.prm.cpu().numpy()
+    # non-zero biases so that their gradients are exercised through tanh'
+    p = p + rng.normal(0, 0.1, p.shape).astype(np.float32)
+    net.prm.copy_(torch.from_numpy(p))
+    Xd, Td = torch.from_numpy(X).to(gpu), torch.from_numpy(T).to(gpu)
+    loss, g = net.loss_and_grad(Xd, Td, chunk=chunk)
+    ref_loss, ref_g = mlp_oracle.loss_and_grad(p.astype(np.float64), X, T.astype(np.float64))
+    assert loss == pytest.approx(ref_loss, rel=1e-4)
+    g = g.cpu().numpy().astype(np.float64)
+    assert np.abs(g - ref_g).max() <= 1e-4 * np.abs(ref_g).max()
+    y = net.predict_raw(Xd).cpu().numpy()
+    ref_y, _ = mlp_oracle.forward(p, X.astype(np.float64))
+    assert np.abs(y - ref_y).max() <= 1e-5 * max(1.0, np.abs(ref_y).max())
+
+
+def test_adam_step_matches_numpy(gpu, pq3d_lib):
+    """Three Adam steps: the kernel's update from its own summed gradient
+    equals numpy's Adam applied to that gradient."""
+    net = _mlp(gpu, seed=2)
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((200, 6)).astype(np.float32)
+    T = rng.standard_normal(200).astype(np.float32)
+    m = np.zeros(net.P)
+    v = np.zeros(net.P)
+    lr, b1, b2, eps = 1e-2, 0.9, 0.999, 1e-8
+    Xd, Td = torch.from_numpy(X).to(gpu), torch.from_numpy(T).to(gpu)
+    for t in range(1, 4):
+        p = net.prm.cpu().numpy().astype(np.float64)
+        _, gd = net.loss_and_grad(Xd, Td)  # lr = 0: gradient only, moments untouched
+        g = gd.cpu().numpy().astype(np.float64)
+        grad, _, nblk = net._fb(Xd, Td)
+        _lib.call("pq3d_mlp_adam", _lib.ptr(net.prm), _lib.ptr(grad), nblk, 200,
+                  _lib.ptr(net.m), _lib.ptr(net.v), lr, b1, b2, eps, t, None,
+                  _lib.stream_handle())
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        p = p - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+        got = net.prm.cpu().numpy().astype(np.float64)
+        assert np.abs(got - p).max() <= 1e-3 * lr, t
+
+
+def test_training_reduces_loss(gpu, pq3d_lib):
+    """Fit a smooth function of the six inputs; the kernels' Adam loop must
+    reduce the normalised loss by an order of magnitude."""
+    net = _mlp(gpu, seed=3)
+    rng = np.random.default_rng(7)
+    X = rng.uniform(-1, 1, (4096, 6)).astype(np.float32)
+    T = (np.sin(2 * X[:, 0]) + X[:, 1] * X[:, 2] - 0.5 * X[:, 3] ** 2).astype(np.float32)
+    hist = net.fit(X, T, epochs=40, batch=512, lr=3e-3)
+    assert hist[-1] < 0.1 * hist[0]
+    pred = net.predict_log_dt(X)
+    assert np.sqrt(np.mean((pred - T) ** 2)) < 0.3 * T.std()
+
+
+def test_features_match_numpy(gpu, pq3d_lib):
+    from pyquake3d_amd.learned_dt import decode_features
+    rng = np.random.default_rng(11)
+    n, nb = 5003, 7
+    V = 10.0 ** rng.uniform(-14, 0.5, (n, nb))
+    psi = rng.uniform(0.5, 0.8, (n, nb))
+    b = rng.uniform(0.01, 0.02, (n, nb))
+    f0 = np.full((n, nb), 0.6)
+    V0 = 1e-6
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x).ravel()).to(gpu)
+    acc = torch.zeros(4 * nb, dtype=torch.int64, device=gpu)
+    dV, dpsi, db, df0 = d(V), d(psi), d(b), d(f0)  # held until the kernel has run
+    _lib.call("pq3d_features", n * nb, nb, _lib.ptr(dV), _lib.ptr(dpsi), _lib.ptr(db),
+              _lib.ptr(df0), V0, _lib.ptr(acc), _lib.stream_handle())
+    vmax, vmin, mean, frac = decode_features(acc.cpu().numpy().reshape(nb, 4), n)
+    lv = np.log10(V)
+    # 1000 + log10 V is stored as a double: resolution 2^-43 * 1000
+    assert np.abs(vmax - lv.max(0)).max() <= 1e-12
+    assert np.abs(vmin - lv.min(0)).max() <= 1e-12
+    lvt = np.log10(V / V0) + (psi - f0) / (b * np.log(10.0))
+    assert np.abs(mean - lvt.mean(0)).max() <= 1e-8
+    assert np.array_equal(frac, (V > 1e-3).sum(0) / n)

@@ -129,10 +129,8 @@ class Checker:
             for kind, base, ld, plane, cnt, xo in it[tp[q]:tp[q + 1]]:
                 assert cnt > 0 and base >= 0
                 hi = base + (cnt - 1) * ld + (mt - 1)
-                if kind == 0:
-                    assert hi + 5 * plane < P and xo >= 0 and xo + cnt <= N
-                else:
-                    assert hi + 2 * plane < P and xo >= 0 and xo + cnt <= nt
+                assert kind in (0, 1, 2) and hi + 2 * plane < P and xo >= 0
+                assert xo + cnt <= (nt if kind == 1 else N)
 
 
 @pytest.mark.parametrize("leaf,eta,dtype", [(32, 2.0, torch.float32), (64, 2.0, torch.float64),

@@ -68,17 +68,11 @@ def emulate_matvec(W, pool, v1, v2, n, nb=1):
         lanes = np.arange(mt)
         for it in items[ptr[q]:ptr[q + 1]]:
             kind, base, ld, plane, cnt, xo = it
+            w = {0: X1, 1: t, 2: X2}[kind]
             for c in range(cnt):
                 p = base + lanes + c * ld
-                if kind == 0:
-                    x1, x2 = X1[xo + c], X2[xo + c]
-                    for o in range(3):
-                        y[o, R0:R0 + mt] += (np.outer(pool[p + o * plane], x1) +
-                                             np.outer(pool[p + (3 + o) * plane], x2))
-                else:
-                    tl = t[xo + c]
-                    for o in range(3):
-                        y[o, R0:R0 + mt] += np.outer(pool[p + o * plane], tl)
+                for o in range(3):
+                    y[o, R0:R0 + mt] += np.outer(pool[p + o * plane], w[xo + c])
     return y.reshape(3, n * nb)
 
 
