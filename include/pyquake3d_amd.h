@@ -38,11 +38,11 @@ int pq3d_version(void);
 /* ------------------------------------------- reference drop-in symbols */
 
 /* Replaces `TDstressFS_C` (reference src/TDstressFS_C.cpp:2048-2095, bound by
- * ctypes at src/Hmatrix.py:25-38): full-space stress and strain at n receivers
+ * ctypes at src/Hmatrix.py:25-38): stress and strain at n receivers
  * (X[i],Y[i],Z[i]) from one triangle (P1,P2,P3, 3 doubles each) with Burgers
- * vector (Ss, Ds, Ts). Outputs n x 6 row-major. The half-space branch
- * (jud_halfspace = true) is not provided: outputs become NaN and
- * pq3d_last_error() says so. */
+ * vector (Ss, Ds, Ts), in the full space or, with jud_halfspace, the half
+ * space z <= 0. Outputs n x 6 row-major. On an error the outputs become NaN
+ * and pq3d_last_error() says why. */
 void TDstressFS_C(const double* X, const double* Y, const double* Z, size_t n,
                   const double* P1, const double* P2, const double* P3, double Ss, double Ds,
                   double Ts, double mu, double lambda_, bool jud_halfspace, double* stress,
@@ -51,7 +51,7 @@ void TDstressFS_C(const double* X, const double* Y, const double* Z, size_t n,
 /* Replaces `TDstressEachSourceAtReceiver_C` (reference
  * src/TDstressFS_C.cpp:2098-2143, bound at src/Hmatrix.py:40-49): one receiver
  * (x,y,z) and n_sources triangles given as flat n_sources x 3 arrays. Outputs
- * n_sources x 6 row-major. Same half-space behaviour as above. */
+ * n_sources x 6 row-major. Half space and errors as above. */
 void TDstressEachSourceAtReceiver_C(double x, double y, double z, const double* P1_flat,
                                     const double* P2_flat, const double* P3_flat,
                                     size_t n_sources, double Ss, double Ds, double Ts, double mu,
@@ -94,6 +94,38 @@ int pq3d_tde_traction_block_f32(const double* d_recv, const double* d_rec,
                                 double lambda_, float* d_out, int64_t si, int64_t sj,
                                 int64_t plane, int64_t out_len, void* stream);
 
+/* ------------------------------------------------ half space (z <= 0)
+ * TDstressHS (src/TDstressFS_C.cpp:1911-1999): main + free-surface
+ * correction (TDstress_HarFunc, :1766-1908, with the reference's reuse of
+ * the previous side's values for a vertical side) + image. The correction
+ * strain is the gradient of the correction displacement
+ * (src/DH_greenfunction.py:313-399) by forward-mode differentiation: it
+ * agrees with the reference to rounding, not bit for bit.
+ * pq3d_tde_hs_records: host, 104 doubles per triangle (tri n x 9). */
+int pq3d_tde_hs_records(const double* tri, int64_t n, double* hrec);
+/* TDstressFS_C with jud_halfspace = true, device pointers. */
+int pq3d_tde_stress_points_hs(const double* d_X, const double* d_Y, const double* d_Z,
+                              int64_t n, const double* d_rec1, const double* d_hrec1, double Ss,
+                              double Ds, double Ts, double mu, double lambda_, double* d_stress,
+                              double* d_strain, void* stream);
+/* TDstressEachSourceAtReceiver_C with jud_halfspace = true, device pointers. */
+int pq3d_tde_stress_sources_hs(double x, double y, double z, const double* d_rec,
+                               const double* d_hrec, int64_t ns, double Ss, double Ds, double Ts,
+                               double mu, double lambda_, double* d_stress, double* d_strain,
+                               void* stream);
+/* pq3d_tde_traction_block in the half space (the evaluators of
+ * src/Hmatrix.py:318-511 with halfspace_jud = True). */
+int pq3d_tde_traction_block_hs(const double* d_recv, const double* d_rec, const double* d_hrec,
+                               const int64_t* d_rows, int64_t row0, int64_t m,
+                               const int64_t* d_cols, int64_t col0, int64_t n, double mu,
+                               double lambda_, double* d_out, int64_t si, int64_t sj,
+                               int64_t plane, int64_t out_len, void* stream);
+int pq3d_tde_traction_block_hs_f32(const double* d_recv, const double* d_rec,
+                                   const double* d_hrec, const int64_t* d_rows, int64_t row0,
+                                   int64_t m, const int64_t* d_cols, int64_t col0, int64_t n,
+                                   double mu, double lambda_, float* d_out, int64_t si,
+                                   int64_t sj, int64_t plane, int64_t out_len, void* stream);
+
 /* ------------------------------------------------------------ H-matrix
  * Index space: cluster order (the cluster tree's permutation); blocks are
  * contiguous ranges. Pool element type: float (dtype_bytes 4) or double (8).
@@ -102,17 +134,20 @@ int pq3d_tde_traction_block_f32(const double* d_recv, const double* d_rec,
 
 /* Row widths (int64 elements) of the tables the kernels below read, in the
  * order: dense block, low-rank block, ACA workspace offsets, matvec item,
- * phase-1 unit, phase-1b reduction, tile, receiver record, source record. Host-only; returns the number
- * of values written (at most n). */
+ * phase-1 unit, phase-1b reduction, tile, receiver record, source record,
+ * half-space source record. Host-only; returns the number of values written
+ * (at most n). */
 int pq3d_layout_constants(int64_t* out, int64_t n);
 
 /* Near-field blocks (replaces ACA_worker's dense branch with
  * calc_stressfunc_fromCS/CD, src/Hmatrix.py:1049-1064, :425-511).
  * d_tab: nblocks x 6 int64 (r0, m, c0, n, pool base, unused); block data is
- * six column-major m x n planes at d_pool[base + op*m*n + j*m + i]. */
+ * six column-major m x n planes at d_pool[base + op*m*n + j*m + i].
+ * d_hrec: half-space records (pq3d_tde_hs_records) for the half space, or
+ * NULL for the full space (also for pq3d_hmat_aca). */
 int pq3d_hmat_dense_assemble(const int64_t* d_tab, int64_t nblocks, const double* d_recv,
-                             const double* d_rec, double mu, double lambda_, void* d_pool,
-                             int dtype_bytes, void* stream);
+                             const double* d_rec, const double* d_hrec, double mu,
+                             double lambda_, void* d_pool, int dtype_bytes, void* stream);
 
 /* Far-field ACA (replaces ACA_plus_full_traceS/D, src/Hmatrix.py:581-1011,
  * with a relative Frobenius stop): one workgroup per block of the stacked
@@ -121,9 +156,9 @@ int pq3d_hmat_dense_assemble(const int64_t* d_tab, int64_t nblocks, const double
  * offsets into d_wU (cap*3m doubles per block), d_wV (cap*2n) and d_flags
  * (3m+2n bytes). d_rank[b] = rank, or -1 if the cap was reached. */
 int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
-                  const double* d_recv, const double* d_rec, double mu, double lambda_,
-                  double eps, int kmax, double* d_wU, double* d_wV, uint8_t* d_flags,
-                  int64_t* d_rank, void* stream);
+                  const double* d_recv, const double* d_rec, const double* d_hrec, double mu,
+                  double lambda_, double eps, int kmax, double* d_wU, double* d_wV,
+                  uint8_t* d_flags, int64_t* d_rank, void* stream);
 
 /* Copy converged factors into the pool: U (3m x k, column-major) at
  * d_tab[b][4], V transposed (2n x k, row-major) at d_tab[b][5],
@@ -173,7 +208,9 @@ int pq3d_qd_rhs(int64_t n, const double* d_Tno, const double* d_Tt1o, const doub
  * RungeKutte_solve_Dormand_Prince, src/QDsim.py:1001-1134): candidate state
  * to d_y*, error maxima (psi, tau1, tau2, sigma) to d_err. a63 is the
  * tableau's a63 (the reference uses -46732/5247, the published value is
- * +46732/5247). */
+ * +46732/5247). An instance with d_h[inst] < 0 is left out: its d_y* cells
+ * keep their values (the retry rounds of an ensemble skip the instances
+ * that have already accepted). */
 int pq3d_rk45_attempt(int64_t n, const int32_t* d_inst, const double* d_h, const double* d_Tno,
                       const double* d_Tt1o, const double* d_Tt2o, const double* d_state,
                       const double* d_a, const double* d_b, const double* d_dc,
@@ -184,7 +221,7 @@ int pq3d_rk45_attempt(int64_t n, const int32_t* d_inst, const double* d_h, const
                       unsigned long long* d_err, double a63, int64_t n_inst, void* stream);
 
 /* Accept the candidate (src/QDsim.py:960-993) for instances with
- * d_accept[inst] != 0 (all when NULL): state, slip rates from the
+ * d_accept[inst] == 1 (all when NULL): state, slip rates from the
  * regularized law per component, rake, V floor 1e-30, slip; d_vmax = max V. */
 int pq3d_rk45_commit(int64_t n, const int32_t* d_inst, const double* d_h,
                      const uint8_t* d_accept, const double* d_a, const double* d_cellpar,
@@ -193,6 +230,36 @@ int pq3d_rk45_commit(int64_t n, const int32_t* d_inst, const double* d_h,
                      double* d_state, double* d_slipv1, double* d_slipv2, double* d_slipv,
                      double* d_rake, double* d_slip1, double* d_slip2,
                      unsigned long long* d_vmax, int64_t n_inst, void* stream);
+
+/* Device-driven form of the accept/reject loop of simu_forward
+ * (src/QDsim.py:924-958): `rounds` times, one attempt of every instance with
+ * d_flag 0 at d_h, then the reference's decision on device (accept: flag 1,
+ * d_dtnext = min(hn1, hn2, 1.5 h); reject: h = max(0.5 h, min(hn1, hn2));
+ * flag 2 and d_status 1 when h < 1e-15 or after 20 rejections). No host
+ * synchronisation; the operator products must be current for the step. */
+int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
+                     int32_t* d_nrj, double* d_dtnext, int32_t* d_status, int rounds,
+                     double tol, const double* d_Tno, const double* d_Tt1o, const double* d_Tt2o,
+                     const double* d_state, const double* d_a, const double* d_b,
+                     const double* d_dc, const double* d_f0, const double* d_AdotV1,
+                     const double* d_AdotV2, const double* d_dsigmadt,
+                     const double* d_shear_loading, const double* d_slipv,
+                     const double* d_cellpar, double* d_yTno, double* d_yTt1o, double* d_yTt2o,
+                     double* d_ystate, unsigned long long* d_err, double a63, int64_t n_inst,
+                     void* stream);
+
+/* Commit the instances with d_flag 1 (as pq3d_rk45_commit, the state update
+ * of src/QDsim.py:960-993), then t += h, h = dtnext, rejection count 0,
+ * accepted-step count + 1, flag 0. Instances still attempting keep state, h
+ * and count; they continue in the next pq3d_rk45_rounds of the same step. */
+int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
+                      int32_t* d_nrj, const double* d_dtnext, double* d_t, int64_t* d_nacc,
+                      const double* d_a, const double* d_cellpar, const double* d_yTno,
+                      const double* d_yTt1o, const double* d_yTt2o, const double* d_ystate,
+                      double* d_Tno, double* d_Tt1o, double* d_Tt2o, double* d_state,
+                      double* d_slipv1, double* d_slipv2, double* d_slipv, double* d_rake,
+                      double* d_slip1, double* d_slip2, unsigned long long* d_vmax,
+                      int64_t n_inst, void* stream);
 
 /* ------------------------------------------------- learned step size
  * 6 -> 64 -> 64 -> 64 -> 1 tanh MLP in FP32. Parameters packed as

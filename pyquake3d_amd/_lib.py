@@ -41,13 +41,28 @@ SIGNATURES = {
                                             c_void_p, c_int64, c_int64, c_double, c_double,
                                             c_void_p, c_int64, c_int64, c_int64, c_int64,
                                             c_void_p]),
+    "pq3d_tde_hs_records": (c_int, [_D, c_int64, _D]),
+    "pq3d_tde_stress_points_hs": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                          c_void_p, c_double, c_double, c_double, c_double,
+                                          c_double, c_void_p, c_void_p, c_void_p]),
+    "pq3d_tde_stress_sources_hs": (c_int, [c_double, c_double, c_double, c_void_p, c_void_p,
+                                           c_int64, c_double, c_double, c_double, c_double,
+                                           c_double, c_void_p, c_void_p, c_void_p]),
+    "pq3d_tde_traction_block_hs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                                           c_int64, c_void_p, c_int64, c_int64, c_double,
+                                           c_double, c_void_p, c_int64, c_int64, c_int64,
+                                           c_int64, c_void_p]),
+    "pq3d_tde_traction_block_hs_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                                               c_int64, c_void_p, c_int64, c_int64, c_double,
+                                               c_double, c_void_p, c_int64, c_int64, c_int64,
+                                               c_int64, c_void_p]),
     # H-matrix
     "pq3d_layout_constants": (c_int, [_L, c_int64]),
-    "pq3d_hmat_dense_assemble": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_double,
-                                         c_double, c_void_p, c_int, c_void_p]),
-    "pq3d_hmat_aca": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_double,
-                              c_double, c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
-                              c_void_p]),
+    "pq3d_hmat_dense_assemble": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                         c_double, c_double, c_void_p, c_int, c_void_p]),
+    "pq3d_hmat_aca": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                              c_double, c_double, c_double, c_int, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p]),
     "pq3d_hmat_aca_pack": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                    c_int, c_void_p]),
     "pq3d_hmat_matvec": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p,
@@ -63,6 +78,11 @@ SIGNATURES = {
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    "pq3d_rk45_rounds": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_int, c_double] + [c_void_p] * 19 +
+                         [c_double, c_int64, c_void_p]),
+    "pq3d_rk45_advance": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p] + [c_void_p] * 17 + [c_int64, c_void_p]),
     "pq3d_deficit": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p]),
     "pq3d_qd_rhs": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -105,7 +125,7 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
 
 
 LAYOUT_NAMES = ("dense_block", "lowrank_block", "aca_ws", "item", "unit", "reduce", "tile",
-                "recv", "rec")
+                "recv", "rec", "hrec")
 
 
 def layout_constants() -> dict:

@@ -61,8 +61,6 @@ class Block:
 class BlockTree:
     def __init__(self, root_block, nodelst, elelst, eleVec, mu_, lambda_, xg, halfspace_jud,
                  mini_leaf, device="cuda"):
-        if halfspace_jud:
-            raise NotImplementedError("half-space Green's functions are not provided")
         self.root_block = root_block
         self.nodelst = np.asarray(nodelst, dtype=np.float64)
         self.elelst = np.asarray(elelst, dtype=np.int64)
@@ -75,7 +73,8 @@ class BlockTree:
         self.thread_svd = mini_leaf
         self.aca_eps = 1e-4
         self.device = torch.device(device)
-        self.dm = DeviceMesh(self.nodelst, self.elelst, self.mu_, self.lambda_, device=self.device)
+        self.dm = DeviceMesh(self.nodelst, self.elelst, self.mu_, self.lambda_, device=self.device,
+                             halfspace=bool(halfspace_jud))
         # receivers use the caller's frames and centroids (as the reference does)
         rv = np.concatenate([self.xg, self.eleVec], axis=1)
         self.dm.recv = torch.from_numpy(np.ascontiguousarray(rv)).to(self.device)
@@ -131,6 +130,8 @@ class BlockTree:
         m, n = len(rows), len(cols)
         recv = self.dm.recv[torch.from_numpy(rows).to(dev)].contiguous()
         rec = self.dm.rec[torch.from_numpy(cols).to(dev)].contiguous()
+        hrec = (self.dm.hrec[torch.from_numpy(cols).to(dev)].contiguous() if self.dm.halfspace
+                else None)
         kcap = int(min(kmax, 3 * m, 2 * n))
         tab = torch.tensor([[0, m, 0, n, 0, 0, kcap, 0]], dtype=torch.int64, device=dev)
         wsoff = torch.zeros((1, 4), dtype=torch.int64, device=dev)
@@ -143,7 +144,7 @@ class BlockTree:
         _lib.check_width("recv", recv)
         _lib.check_width("rec", rec)
         _lib.call("pq3d_hmat_aca", _lib.ptr(tab), _lib.ptr(wsoff), 1, _lib.ptr(recv),
-                  _lib.ptr(rec), self.mu_, self.lambda_, float(self.aca_eps), kcap, _lib.ptr(wU),
+                  _lib.ptr(rec), _lib.ptr(hrec), self.mu_, self.lambda_, float(self.aca_eps), kcap, _lib.ptr(wU),
                   _lib.ptr(wV), _lib.ptr(flags), _lib.ptr(rank), _lib.stream_handle())
         k = int(rank.item())
         if k < 0:

@@ -21,6 +21,7 @@
 
 #include "pq3d_common.h"
 #include "tde_device.h"
+#include "tde_hs_device.h"
 
 namespace pq3d {
 
@@ -53,12 +54,26 @@ __device__ inline void load_rec_g(const double* __restrict__ src, double* r) {
   for (int k = 0; k < kSrcRec; ++k) r[k] = src[k];
 }
 
+// Unit strike and dip traction of one (receiver, source) pair, full space or
+// (HS, with the source's half-space record hrec) half space.
+template <bool HS>
+__device__ __forceinline__ void pair_traction(const double* r, const double* __restrict__ hrec,
+                                              int64_t src, const double* q, double mu,
+                                              double lam, double nu, double ts[3],
+                                              double td[3]) {
+  if constexpr (HS)
+    tde_pair_traction_sd_hs(r, hrec + (int64_t)kHsRec * src, q, q + 3, mu, lam, nu, ts, td);
+  else
+    tde_pair_traction_sd(r, q, q + 3, mu, lam, nu, ts, td);
+}
+
 // ------------------------------------------------------ near-field assembly
 // One workgroup per dense block; lane <-> receiver row (column-major writes).
-template <typename T>
+template <typename T, bool HS>
 __global__ void __launch_bounds__(256) k_dense_assemble(const int64_t* __restrict__ tab,
                                                         const double* __restrict__ recv,
                                                         const double* __restrict__ rec,
+                                                        const double* __restrict__ hrec,
                                                         double mu, double lam, double nu,
                                                         T* __restrict__ pool) {
   const int64_t* b = tab + (int64_t)blockIdx.x * DB_W;
@@ -70,7 +85,7 @@ __global__ void __launch_bounds__(256) k_dense_assemble(const int64_t* __restric
     load_rec_g(rec + kSrcRec * (c0 + j), r);
     const double* q = recv + kRecvW * (r0 + i);
     double ts[3], td[3];
-    tde_pair_traction_sd(r, q, q + 3, mu, lam, nu, ts, td);
+    pair_traction<HS>(r, hrec, c0 + j, q, mu, lam, nu, ts, td);
     T* o = pool + base + j * m + i;
     o[0 * mn] = (T)ts[0];
     o[1 * mn] = (T)ts[1];
@@ -132,9 +147,11 @@ __device__ inline double block_sum(double v, double* sv) {
 // workspace size). out_rank[b] = rank, or -1 when the cap was reached without
 // convergence (the caller then stores that block densely).
 // Bit masks of used rows/cols live in global scratch (flags, 1 byte each).
+template <bool HS>
 __global__ void __launch_bounds__(kAcaThreads) k_aca(
     const int64_t* __restrict__ tab, const int64_t* __restrict__ wsoff,
-    const double* __restrict__ recv, const double* __restrict__ rec, double mu, double lam,
+    const double* __restrict__ recv, const double* __restrict__ rec,
+    const double* __restrict__ hrec, double mu, double lam,
     double nu, double eps, int kmax, double* __restrict__ wU, double* __restrict__ wV,
     uint8_t* __restrict__ flags, int64_t* __restrict__ out_rank) {
   __shared__ double sv[kAcaThreads];
@@ -170,7 +187,7 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
       double rr[kSrcRec];
       load_rec_g(rec + kSrcRec * (c0 + j), rr);
       double ts[3], td[3];
-      tde_pair_traction_sd(rr, q, q + 3, mu, lam, nu, ts, td);
+      pair_traction<HS>(rr, hrec, c0 + j, q, mu, lam, nu, ts, td);
       double a = ts[cs], d = td[cs];
       for (int l = 0; l < k; ++l) {
         const double ul = U[(int64_t)l * R + rstar];
@@ -216,7 +233,7 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
       for (int64_t i = tid; i < m; i += kAcaThreads) {
         const double* qi = recv + kRecvW * (r0 + i);
         double ts[3], td[3];
-        tde_pair_traction_sd(rr, qi, qi + 3, mu, lam, nu, ts, td);
+        pair_traction<HS>(rr, hrec, c0 + js, qi, mu, lam, nu, ts, td);
         for (int c = 0; c < 3; ++c) {
           double v = ss ? td[c] : ts[c];
           const int64_t row = c * m + i;
@@ -300,10 +317,28 @@ __device__ __forceinline__ int wave_id() {
 // Column loops are unrolled so that a group's loads are all issued before its
 // multiply-adds (several loads in flight per wave). A larger NB means more
 // arithmetic and more live registers per loaded value, hence less unrolling.
+// Build-time tuning knobs for 8 and 16 right-hand sides (defaults: the best
+// of a sweep on MI355X at N = 100,352, FP32 and FP64 storage).
+#ifndef PQ3D_P1_U8
+#define PQ3D_P1_U8 4
+#endif
+#ifndef PQ3D_P2_U8
+#define PQ3D_P2_U8 4
+#endif
+#ifndef PQ3D_P1_U16
+#define PQ3D_P1_U16 4
+#endif
+#ifndef PQ3D_P2_U16
+#define PQ3D_P2_U16 2
+#endif
 template <int NB>
-constexpr int p1_unroll() { return NB <= 2 ? 8 : NB <= 4 ? 4 : NB <= 8 ? 2 : 1; }
+constexpr int p1_unroll() {
+  return NB <= 2 ? 8 : NB <= 4 ? 4 : NB <= 8 ? PQ3D_P1_U8 : PQ3D_P1_U16;
+}
 template <int NB>
-constexpr int p2_unroll() { return NB <= 2 ? 8 : NB <= 4 ? 4 : NB <= 8 ? 2 : 1; }
+constexpr int p2_unroll() {
+  return NB <= 2 ? 8 : NB <= 4 ? 4 : NB <= 8 ? PQ3D_P2_U8 : PQ3D_P2_U16;
+}
 
 // Phase 1: t[l][b] = sum over the unit's stacked columns c of V^T[c][l] x_b[c]
 // (x = v1 for c < n, v2 for c >= n); one wavefront per unit, lane <-> l; the
@@ -504,18 +539,18 @@ extern "C" {
 
 // Row widths of every table the H-matrix kernels read, in this order:
 // dense block, low-rank block, ACA workspace offsets, matvec item, phase-1
-// unit, phase-1b reduction, tile, receiver record, source record. Returns the
-// number written.
+// unit, phase-1b reduction, tile, receiver record, source record, half-space
+// source record. Returns the number written.
 int pq3d_layout_constants(int64_t* out, int64_t n) {
-  const int64_t v[9] = {DB_W, LB_W, WS_W, IT_W, UN_W, RD_W, TL_W, kRecvW, kSrcRec};
+  const int64_t v[10] = {DB_W, LB_W, WS_W, IT_W, UN_W, RD_W, TL_W, kRecvW, kSrcRec, kHsRec};
   int64_t k = 0;
-  for (; k < n && k < 9; ++k) out[k] = v[k];
+  for (; k < n && k < 10; ++k) out[k] = v[k];
   return (int)k;
 }
 
 int pq3d_hmat_dense_assemble(const int64_t* d_tab, int64_t nblocks, const double* d_recv,
-                             const double* d_rec, double mu, double lam, void* d_pool,
-                             int dtype_bytes, void* stream) {
+                             const double* d_rec, const double* d_hrec, double mu, double lam,
+                             void* d_pool, int dtype_bytes, void* stream) {
   clear_error();
   PQ3D_REQUIRE(nblocks >= 0, "nblocks must be >= 0");
   if (nblocks == 0) return PQ3D_OK;
@@ -523,18 +558,30 @@ int pq3d_hmat_dense_assemble(const int64_t* d_tab, int64_t nblocks, const double
   PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
   PQ3D_REQUIRE(nblocks < (1LL << 31), "too many blocks");
   const double nu = 1.0 / (1.0 + lam / mu) / 2.0;
-  if (dtype_bytes == 4)
-    k_dense_assemble<float><<<(unsigned)nblocks, 256, 0, as_stream(stream)>>>(
-        d_tab, d_recv, d_rec, mu, lam, nu, (float*)d_pool);
-  else
-    k_dense_assemble<double><<<(unsigned)nblocks, 256, 0, as_stream(stream)>>>(
-        d_tab, d_recv, d_rec, mu, lam, nu, (double*)d_pool);
+  hipStream_t s = as_stream(stream);
+  const unsigned g = (unsigned)nblocks;
+  if (dtype_bytes == 4) {
+    if (d_hrec)
+      k_dense_assemble<float, true><<<g, 256, 0, s>>>(d_tab, d_recv, d_rec, d_hrec, mu, lam, nu,
+                                                      (float*)d_pool);
+    else
+      k_dense_assemble<float, false><<<g, 256, 0, s>>>(d_tab, d_recv, d_rec, nullptr, mu, lam,
+                                                       nu, (float*)d_pool);
+  } else {
+    if (d_hrec)
+      k_dense_assemble<double, true><<<g, 256, 0, s>>>(d_tab, d_recv, d_rec, d_hrec, mu, lam, nu,
+                                                       (double*)d_pool);
+    else
+      k_dense_assemble<double, false><<<g, 256, 0, s>>>(d_tab, d_recv, d_rec, nullptr, mu, lam,
+                                                        nu, (double*)d_pool);
+  }
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }
 
 int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
-                  const double* d_recv, const double* d_rec, double mu, double lam, double eps,
+                  const double* d_recv, const double* d_rec, const double* d_hrec, double mu,
+                  double lam, double eps,
                   int kmax, double* d_wU, double* d_wV, uint8_t* d_flags, int64_t* d_rank,
                   void* stream) {
   clear_error();
@@ -546,8 +593,14 @@ int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
   PQ3D_REQUIRE(eps > 0.0, "eps must be > 0");
   PQ3D_REQUIRE(nblocks < (1LL << 31), "too many blocks");
   const double nu = 1.0 / (1.0 + lam / mu) / 2.0;
-  k_aca<<<(unsigned)nblocks, kAcaThreads, 0, as_stream(stream)>>>(
-      d_tab, d_wsoff, d_recv, d_rec, mu, lam, nu, eps, kmax, d_wU, d_wV, d_flags, d_rank);
+  if (d_hrec)
+    k_aca<true><<<(unsigned)nblocks, kAcaThreads, 0, as_stream(stream)>>>(
+        d_tab, d_wsoff, d_recv, d_rec, d_hrec, mu, lam, nu, eps, kmax, d_wU, d_wV, d_flags,
+        d_rank);
+  else
+    k_aca<false><<<(unsigned)nblocks, kAcaThreads, 0, as_stream(stream)>>>(
+        d_tab, d_wsoff, d_recv, d_rec, nullptr, mu, lam, nu, eps, kmax, d_wU, d_wV, d_flags,
+        d_rank);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }

@@ -114,9 +114,12 @@ __device__ inline void rk45_cell(
     const double* __restrict__ dsig, const double* __restrict__ shear_loading,
     const double* __restrict__ slipv, const double* __restrict__ cellpar, double* __restrict__ yTno,
     double* __restrict__ yTt1o, double* __restrict__ yTt2o, double* __restrict__ yst,
-    unsigned long long* err, double B63) {
+    unsigned long long* err, double B63, const uint8_t* __restrict__ flag) {
   const int g = inst ? inst[i] : 0;
   const double h = hvec[g];
+  // a negative step size, or a nonzero flag (accepted or aborted), leaves the
+  // instance out of this attempt: its candidate state and error slots stay
+  if (h < 0.0 || (flag && flag[g])) return;
   const double V0 = cellpar[3 * g + 0], mu = cellpar[3 * g + 1], Cs = cellpar[3 * g + 2];
   const double a = pa[i], b = pb[i], dc = pdc[i], f0 = pf0[i];
   const double A1 = AdotV1[i], A2 = AdotV2[i], ds = dsig[i], sl = shear_loading[i], V = slipv[i];
@@ -185,7 +188,8 @@ __global__ void __launch_bounds__(256) k_rk45_attempt(
     const double* __restrict__ dsig, const double* __restrict__ shear_loading,
     const double* __restrict__ slipv, const double* __restrict__ cellpar, double* __restrict__ yTno,
     double* __restrict__ yTt1o, double* __restrict__ yTt2o, double* __restrict__ yst,
-    unsigned long long* __restrict__ err, double B63, int64_t n_inst) {
+    unsigned long long* __restrict__ err, double B63, int64_t n_inst,
+    const uint8_t* __restrict__ flag) {
   // per-block maxima first (LDS atomics), then one global atomic per
   // (instance, slot) per block: n cells would otherwise all hit 4 addresses
   __shared__ unsigned long long s_err[kLdsInst * 4];
@@ -196,7 +200,7 @@ __global__ void __launch_bounds__(256) k_rk45_attempt(
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) rk45_cell(i, inst, hvec, Tno, Tt1o, Tt2o, st, pa, pb, pdc, pf0, AdotV1, AdotV2, dsig,
                        shear_loading, slipv, cellpar, yTno, yTt1o, yTt2o, yst,
-                       lds ? s_err : err, B63);
+                       lds ? s_err : err, B63, flag);
   __syncthreads();
   if (lds)
     for (int t = threadIdx.x; t < n_inst * 4; t += blockDim.x)
@@ -223,7 +227,7 @@ __global__ void __launch_bounds__(256) k_rk45_commit(
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int g = (i < n && inst) ? inst[i] : 0;
-  if (i < n && !(accept && !accept[g])) {
+  if (i < n && !(accept && accept[g] != 1)) {
     const double h = hvec[g];
     const double V0 = cellpar[3 * g + 0];
     const double a = pa[i];
@@ -299,6 +303,71 @@ __global__ void k_zero_u64(unsigned long long* p, int64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
+// Python's max(a, b) and min(a, b) for floats (the reference's controller is
+// Python): the first argument is kept unless the second compares greater
+// (smaller), so a NaN in the second position is dropped.
+__device__ inline double py_max(double a, double b) { return b > a ? b : a; }
+__device__ inline double py_min(double a, double b) { return b < a ? b : a; }
+
+// Accept/reject decision of the reference controller (src/QDsim.py:931-958)
+// per instance still attempting (flag 0), from the error maxima of the last
+// attempt: accept -> flag 1 and dtnext = min(hn1, hn2, 1.5 h); reject ->
+// h = max(0.5 h, min(hn1, hn2)), and flag 2 with status 1 when h < 1e-15 or
+// after more than 20 rejections. hn_i = 0.9 h (tol / R_i)^0.2,
+// R1 = err_psi + 1e-10, R2 = max(err_tau1, err_tau2, err_sigma) + 1e-10.
+__global__ void k_rk45_control(int64_t n_inst, const unsigned long long* __restrict__ err,
+                               double tol, double* __restrict__ hcur, uint8_t* __restrict__ flag,
+                               int32_t* __restrict__ nrj, double* __restrict__ dtnext,
+                               int32_t* __restrict__ status) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_inst || flag[b]) return;
+  const double e0 = __longlong_as_double((long long)err[4 * b + 0]);
+  const double e1 = __longlong_as_double((long long)err[4 * b + 1]);
+  const double e2 = __longlong_as_double((long long)err[4 * b + 2]);
+  const double e3 = __longlong_as_double((long long)err[4 * b + 3]);
+  const double R1 = e0 + 1e-10;
+  const double R2 = py_max(py_max(e1, e2), e3) + 1e-10;
+  const double c1 = R1 / tol, c2 = R2 / tol;
+  const double h = hcur[b];
+  const double hn1 = h * 0.9 * pow(tol / R1, 0.2);
+  const double hn2 = h * 0.9 * pow(tol / R2, 0.2);
+  if (py_max(c1, c2) < 1.0 && !isnan(c1) && !isnan(c2)) {
+    dtnext[b] = py_min(py_min(hn1, hn2), 1.5 * h);
+    flag[b] = 1;
+    return;
+  }
+  const int32_t r = nrj[b] + 1;
+  nrj[b] = r;
+  const double hn = py_max(0.5 * h, py_min(hn1, hn2));
+  hcur[b] = hn;
+  if (hn < 1e-15 || r > 20) {
+    flag[b] = 2;
+    status[b] = 1;
+  }
+}
+
+// Before the commit of the accepted instances: clear their slip-rate maxima.
+__global__ void k_clear_accepted(int64_t n_inst, const uint8_t* __restrict__ flag,
+                                 unsigned long long* __restrict__ vmax) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < n_inst && flag[b] == 1) vmax[b] = 0ull;
+}
+
+// After the commit: accepted instances advance t += h, take h = dtnext for
+// their next step and start it with a clear rejection count.
+__global__ void k_rk45_advance(int64_t n_inst, uint8_t* __restrict__ flag,
+                               double* __restrict__ hcur, const double* __restrict__ dtnext,
+                               int32_t* __restrict__ nrj, double* __restrict__ t,
+                               int64_t* __restrict__ nacc) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_inst || flag[b] != 1) return;
+  t[b] = t[b] + hcur[b];
+  hcur[b] = dtnext[b];
+  nrj[b] = 0;
+  nacc[b] = nacc[b] + 1;
+  flag[b] = 0;
+}
+
 }  // namespace pq3d
 
 using namespace pq3d;
@@ -321,7 +390,7 @@ int pq3d_rk45_attempt(int64_t n, const int32_t* d_inst, const double* d_h, const
   k_rk45_attempt<<<blocks_for(n, 256), 256, 0, s>>>(
       n, d_inst, d_h, d_Tno, d_Tt1o, d_Tt2o, d_state, d_a, d_b, d_dc, d_f0, d_AdotV1, d_AdotV2,
       d_dsigmadt, d_shear_loading, d_slipv, d_cellpar, d_yTno, d_yTt1o, d_yTt2o, d_ystate, d_err,
-      a63, n_inst);
+      a63, n_inst, nullptr);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }
@@ -379,6 +448,68 @@ int pq3d_add_loading(int64_t n, double* d_dsig, const double* d_load, int fix_tn
   PQ3D_REQUIRE(n >= 0, "bad size");
   if (n == 0) return PQ3D_OK;
   k_add_loading<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(n, d_dsig, d_load, fix_tn);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+// Device-driven controller: `rounds` x (attempt of every instance with
+// flag 0 at its h, then the reference's accept/reject decision), with no host
+// synchronisation. The operator products must be current for the step.
+// Per-instance arrays (n_inst): d_h (attempt step size), d_flag (0 attempting,
+// 1 accepted, 2 aborted), d_nrj (rejections this step), d_dtnext, d_status
+// (1 = aborted: dt < 1e-15 or more than 20 rejections).
+int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
+                     int32_t* d_nrj, double* d_dtnext, int32_t* d_status, int rounds,
+                     double tol, const double* d_Tno, const double* d_Tt1o, const double* d_Tt2o,
+                     const double* d_state, const double* d_a, const double* d_b,
+                     const double* d_dc, const double* d_f0, const double* d_AdotV1,
+                     const double* d_AdotV2, const double* d_dsigmadt,
+                     const double* d_shear_loading, const double* d_slipv,
+                     const double* d_cellpar, double* d_yTno, double* d_yTt1o, double* d_yTt2o,
+                     double* d_ystate, unsigned long long* d_err, double a63, int64_t n_inst,
+                     void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0 && n_inst >= 1 && rounds >= 1, "bad sizes");
+  PQ3D_REQUIRE(tol > 0.0, "tol must be > 0");
+  if (n == 0) return PQ3D_OK;
+  hipStream_t s = as_stream(stream);
+  for (int r = 0; r < rounds; ++r) {
+    k_zero_u64<<<blocks_for(4 * n_inst, 256), 256, 0, s>>>(d_err, 4 * n_inst);
+    k_rk45_attempt<<<blocks_for(n, 256), 256, 0, s>>>(
+        n, d_inst, d_h, d_Tno, d_Tt1o, d_Tt2o, d_state, d_a, d_b, d_dc, d_f0, d_AdotV1,
+        d_AdotV2, d_dsigmadt, d_shear_loading, d_slipv, d_cellpar, d_yTno, d_yTt1o, d_yTt2o,
+        d_ystate, d_err, a63, n_inst, d_flag);
+    k_rk45_control<<<blocks_for(n_inst, 64), 64, 0, s>>>(n_inst, d_err, tol, d_h, d_flag, d_nrj,
+                                                          d_dtnext, d_status);
+  }
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+// Commit the instances with flag 1 (as pq3d_rk45_commit), then advance them:
+// t += h, h = dtnext, rejection count 0, accepted-step count + 1, flag 0.
+// Instances still attempting keep their state, h and count, and continue in
+// the next call of pq3d_rk45_rounds (after the operator products are
+// recomputed from the same, unchanged state).
+int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
+                      int32_t* d_nrj, const double* d_dtnext, double* d_t, int64_t* d_nacc,
+                      const double* d_a, const double* d_cellpar, const double* d_yTno,
+                      const double* d_yTt1o, const double* d_yTt2o, const double* d_ystate,
+                      double* d_Tno, double* d_Tt1o, double* d_Tt2o, double* d_state,
+                      double* d_slipv1, double* d_slipv2, double* d_slipv, double* d_rake,
+                      double* d_slip1, double* d_slip2, unsigned long long* d_vmax,
+                      int64_t n_inst, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0 && n_inst >= 1, "bad sizes");
+  if (n == 0) return PQ3D_OK;
+  hipStream_t s = as_stream(stream);
+  k_clear_accepted<<<blocks_for(n_inst, 64), 64, 0, s>>>(n_inst, d_flag, d_vmax);
+  k_rk45_commit<<<blocks_for(n, 256), 256, 0, s>>>(
+      n, d_inst, d_h, d_flag, d_a, d_cellpar, d_yTno, d_yTt1o, d_yTt2o, d_ystate, d_Tno,
+      d_Tt1o, d_Tt2o, d_state, d_slipv1, d_slipv2, d_slipv, d_rake, d_slip1, d_slip2, d_vmax,
+      n_inst);
+  k_rk45_advance<<<blocks_for(n_inst, 64), 64, 0, s>>>(n_inst, d_flag, d_h, d_dtnext, d_nrj, d_t,
+                                                        d_nacc);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }

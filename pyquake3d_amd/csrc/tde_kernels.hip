@@ -10,6 +10,9 @@
 //                              drop-in host-pointer entry points with the
 //                              reference's signatures (src/TDstressFS_C.cpp:2048,
 //                              :2098) built on the kernels above.
+//  * *_hs variants             the same for the half space (free surface z = 0,
+//                              src/TDstressFS_C.cpp:1911-1999), with a second
+//                              per-triangle record (pq3d_tde_hs_records).
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdarg>
@@ -20,6 +23,7 @@
 
 #include "pq3d_common.h"
 #include "tde_device.h"
+#include "tde_hs_device.h"
 
 namespace pq3d {
 
@@ -40,37 +44,50 @@ __device__ inline void load_rec(const double* __restrict__ src, double* r) {
   for (int k = 0; k < kSrcRec; ++k) r[k] = src[k];
 }
 
-// Reference symbol 1: n receivers (X,Y,Z) x one source record.
+// Reference symbol 1: n receivers (X,Y,Z) x one source record; with HS, the
+// half-space stress (hrec: the triangle's half-space record).
+template <bool HS>
 __global__ void __launch_bounds__(256) k_stress_points(const double* __restrict__ X,
                                                        const double* __restrict__ Y,
                                                        const double* __restrict__ Z, int64_t n,
-                                                       const double* __restrict__ rec1, double Ss,
-                                                       double Ds, double Ts, double mu, double lam,
-                                                       double nu, double* __restrict__ stress,
+                                                       const double* __restrict__ rec1,
+                                                       const double* __restrict__ hrec1,
+                                                       double Ss, double Ds, double Ts, double mu,
+                                                       double lam, double nu,
+                                                       double* __restrict__ stress,
                                                        double* __restrict__ strain) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double r[kSrcRec];
   load_rec(rec1, r);
   double s[6], e[6];
-  tde_point(r, X[i], Y[i], Z[i], Ss, Ds, Ts, mu, lam, nu, s, e);
+  if constexpr (HS)
+    tde_point_hs(r, hrec1, X[i], Y[i], Z[i], Ss, Ds, Ts, mu, lam, nu, s, e);
+  else
+    tde_point(r, X[i], Y[i], Z[i], Ss, Ds, Ts, mu, lam, nu, s, e);
 #pragma unroll
   for (int k = 0; k < 6; ++k) { stress[6 * i + k] = s[k]; strain[6 * i + k] = e[k]; }
 }
 
-// Reference symbol 2: one receiver x n source records.
+// Reference symbol 2: one receiver x n source records (HS: and their
+// half-space records).
+template <bool HS>
 __global__ void __launch_bounds__(256) k_stress_sources(double x, double y, double z,
-                                                        const double* __restrict__ rec, int64_t ns,
-                                                        double Ss, double Ds, double Ts, double mu,
-                                                        double lam, double nu,
-                                                        double* __restrict__ stress,
+                                                        const double* __restrict__ rec,
+                                                        const double* __restrict__ hrec,
+                                                        int64_t ns, double Ss, double Ds,
+                                                        double Ts, double mu, double lam,
+                                                        double nu, double* __restrict__ stress,
                                                         double* __restrict__ strain) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   double r[kSrcRec];
   load_rec(rec + (int64_t)kSrcRec * i, r);
   double s[6], e[6];
-  tde_point(r, x, y, z, Ss, Ds, Ts, mu, lam, nu, s, e);
+  if constexpr (HS)
+    tde_point_hs(r, hrec + (int64_t)kHsRec * i, x, y, z, Ss, Ds, Ts, mu, lam, nu, s, e);
+  else
+    tde_point(r, x, y, z, Ss, Ds, Ts, mu, lam, nu, s, e);
 #pragma unroll
   for (int k = 0; k < 6; ++k) { stress[6 * i + k] = s[k]; strain[6 * i + k] = e[k]; }
 }
@@ -82,9 +99,10 @@ constexpr int kTbRows = 8;    // receivers per thread
 // thread walks kTbRows receivers whose 12 doubles (point + frame) are uniform
 // across the workgroup. Output: out[op * plane + i * si + j * sj], op order
 // A1s, A2s, Bs, A1d, A2d, Bd. rows/cols may be null (identity + offset).
-template <typename OutT>
+template <typename OutT, bool HS>
 __global__ void __launch_bounds__(256) k_traction_block(
     const double* __restrict__ recv, const double* __restrict__ rec,
+    const double* __restrict__ hrec,
     const int64_t* __restrict__ rows, int64_t row0, int64_t m, const int64_t* __restrict__ cols,
     int64_t col0, int64_t n, double mu, double lam, double nu, OutT* __restrict__ out, int64_t si,
     int64_t sj, int64_t plane) {
@@ -99,7 +117,10 @@ __global__ void __launch_bounds__(256) k_traction_block(
     const int64_t rcv = rows ? rows[i] : row0 + i;
     const double* p = recv + 12 * rcv;
     double ts[3], td[3];
-    tde_pair_traction_sd(r, p, p + 3, mu, lam, nu, ts, td);
+    if constexpr (HS)
+      tde_pair_traction_sd_hs(r, hrec + (int64_t)kHsRec * src, p, p + 3, mu, lam, nu, ts, td);
+    else
+      tde_pair_traction_sd(r, p, p + 3, mu, lam, nu, ts, td);
     OutT* o = out + i * si + j * sj;
     o[0 * plane] = (OutT)ts[0];
     o[1 * plane] = (OutT)ts[1];
@@ -166,16 +187,67 @@ int pq3d_tde_source_records(const double* tri, int64_t n, double* rec) {
   return PQ3D_OK;
 }
 
-int pq3d_tde_stress_points(const double* d_X, const double* d_Y, const double* d_Z, int64_t n,
-                           const double* d_rec1, double Ss, double Ds, double Ts, double mu,
-                           double lam, double* d_stress, double* d_strain, void* stream) {
+int pq3d_tde_hs_records(const double* tri, int64_t n, double* hrec) {
   clear_error();
+  PQ3D_REQUIRE(n >= 0, "n must be >= 0");
+  if (n == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(tri && hrec, "null pointer");
+  for (int64_t i = 0; i < n; ++i)
+    tde_hs_record_host(tri + 9 * i, tri + 9 * i + 3, tri + 9 * i + 6, hrec + kHsRec * i);
+  return PQ3D_OK;
+}
+
+static int stress_points_launch(const double* d_X, const double* d_Y, const double* d_Z,
+                                int64_t n, const double* d_rec1, const double* d_hrec1, double Ss,
+                                double Ds, double Ts, double mu, double lam, double* d_stress,
+                                double* d_strain, void* stream) {
   PQ3D_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return PQ3D_OK;
   PQ3D_REQUIRE(d_X && d_Y && d_Z && d_rec1 && d_stress && d_strain, "null pointer");
   PQ3D_REQUIRE(n <= (int64_t)1 << 40, "n too large");
-  k_stress_points<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
-      d_X, d_Y, d_Z, n, d_rec1, Ss, Ds, Ts, mu, lam, poisson(mu, lam), d_stress, d_strain);
+  if (d_hrec1)
+    k_stress_points<true><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
+        d_X, d_Y, d_Z, n, d_rec1, d_hrec1, Ss, Ds, Ts, mu, lam, poisson(mu, lam), d_stress,
+        d_strain);
+  else
+    k_stress_points<false><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
+        d_X, d_Y, d_Z, n, d_rec1, nullptr, Ss, Ds, Ts, mu, lam, poisson(mu, lam), d_stress,
+        d_strain);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_tde_stress_points(const double* d_X, const double* d_Y, const double* d_Z, int64_t n,
+                           const double* d_rec1, double Ss, double Ds, double Ts, double mu,
+                           double lam, double* d_stress, double* d_strain, void* stream) {
+  clear_error();
+  return stress_points_launch(d_X, d_Y, d_Z, n, d_rec1, nullptr, Ss, Ds, Ts, mu, lam, d_stress,
+                              d_strain, stream);
+}
+
+int pq3d_tde_stress_points_hs(const double* d_X, const double* d_Y, const double* d_Z,
+                              int64_t n, const double* d_rec1, const double* d_hrec1, double Ss,
+                              double Ds, double Ts, double mu, double lam, double* d_stress,
+                              double* d_strain, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(d_hrec1, "null half-space record");
+  return stress_points_launch(d_X, d_Y, d_Z, n, d_rec1, d_hrec1, Ss, Ds, Ts, mu, lam, d_stress,
+                              d_strain, stream);
+}
+
+static int stress_sources_launch(double x, double y, double z, const double* d_rec,
+                                 const double* d_hrec, int64_t ns, double Ss, double Ds,
+                                 double Ts, double mu, double lam, double* d_stress,
+                                 double* d_strain, void* stream) {
+  PQ3D_REQUIRE(ns >= 0, "n_sources must be >= 0");
+  if (ns == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_rec && d_stress && d_strain, "null pointer");
+  if (d_hrec)
+    k_stress_sources<true><<<blocks_for(ns, 256), 256, 0, as_stream(stream)>>>(
+        x, y, z, d_rec, d_hrec, ns, Ss, Ds, Ts, mu, lam, poisson(mu, lam), d_stress, d_strain);
+  else
+    k_stress_sources<false><<<blocks_for(ns, 256), 256, 0, as_stream(stream)>>>(
+        x, y, z, d_rec, nullptr, ns, Ss, Ds, Ts, mu, lam, poisson(mu, lam), d_stress, d_strain);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }
@@ -184,17 +256,23 @@ int pq3d_tde_stress_sources(double x, double y, double z, const double* d_rec, i
                             double Ss, double Ds, double Ts, double mu, double lam,
                             double* d_stress, double* d_strain, void* stream) {
   clear_error();
-  PQ3D_REQUIRE(ns >= 0, "n_sources must be >= 0");
-  if (ns == 0) return PQ3D_OK;
-  PQ3D_REQUIRE(d_rec && d_stress && d_strain, "null pointer");
-  k_stress_sources<<<blocks_for(ns, 256), 256, 0, as_stream(stream)>>>(
-      x, y, z, d_rec, ns, Ss, Ds, Ts, mu, lam, poisson(mu, lam), d_stress, d_strain);
-  PQ3D_LAUNCH_CHECK();
-  return PQ3D_OK;
+  return stress_sources_launch(x, y, z, d_rec, nullptr, ns, Ss, Ds, Ts, mu, lam, d_stress,
+                               d_strain, stream);
+}
+
+int pq3d_tde_stress_sources_hs(double x, double y, double z, const double* d_rec,
+                               const double* d_hrec, int64_t ns, double Ss, double Ds, double Ts,
+                               double mu, double lam, double* d_stress, double* d_strain,
+                               void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(d_hrec, "null half-space records");
+  return stress_sources_launch(x, y, z, d_rec, d_hrec, ns, Ss, Ds, Ts, mu, lam, d_stress,
+                               d_strain, stream);
 }
 
 static int traction_block_launch(const double* d_recv, const double* d_rec,
-                                 const int64_t* d_rows, int64_t row0, int64_t m,
+                                 const double* d_hrec, const int64_t* d_rows, int64_t row0,
+                                 int64_t m,
                                  const int64_t* d_cols, int64_t col0, int64_t n, double mu,
                                  double lam, void* d_out, int64_t si, int64_t sj, int64_t plane,
                                  int64_t out_len, int f32, void* stream) {
@@ -216,16 +294,28 @@ static int traction_block_launch(const double* d_recv, const double* d_rec,
     const int64_t r0 = y0 * kTbRows;
     const int64_t mm = (m - r0 < ny * kTbRows) ? m - r0 : ny * kTbRows;
     dim3 grid((unsigned)gx, (unsigned)ny);
+    const int64_t* rr = d_rows ? d_rows + r0 : nullptr;
+    hipStream_t st = as_stream(stream);
     if (f32) {
       float* o = (float*)d_out + r0 * si;
-      k_traction_block<float><<<grid, kTbCols, 0, as_stream(stream)>>>(
-          d_recv, d_rec, d_rows ? d_rows + r0 : nullptr, row0 + r0, mm, d_cols, col0, n, mu, lam,
-          nu, o, si, sj, plane);
+      if (d_hrec)
+        k_traction_block<float, true><<<grid, kTbCols, 0, st>>>(
+            d_recv, d_rec, d_hrec, rr, row0 + r0, mm, d_cols, col0, n, mu, lam, nu, o, si, sj,
+            plane);
+      else
+        k_traction_block<float, false><<<grid, kTbCols, 0, st>>>(
+            d_recv, d_rec, nullptr, rr, row0 + r0, mm, d_cols, col0, n, mu, lam, nu, o, si, sj,
+            plane);
     } else {
       double* o = (double*)d_out + r0 * si;
-      k_traction_block<double><<<grid, kTbCols, 0, as_stream(stream)>>>(
-          d_recv, d_rec, d_rows ? d_rows + r0 : nullptr, row0 + r0, mm, d_cols, col0, n, mu, lam,
-          nu, o, si, sj, plane);
+      if (d_hrec)
+        k_traction_block<double, true><<<grid, kTbCols, 0, st>>>(
+            d_recv, d_rec, d_hrec, rr, row0 + r0, mm, d_cols, col0, n, mu, lam, nu, o, si, sj,
+            plane);
+      else
+        k_traction_block<double, false><<<grid, kTbCols, 0, st>>>(
+            d_recv, d_rec, nullptr, rr, row0 + r0, mm, d_cols, col0, n, mu, lam, nu, o, si, sj,
+            plane);
     }
     PQ3D_LAUNCH_CHECK();
   }
@@ -237,8 +327,8 @@ int pq3d_tde_traction_block(const double* d_recv, const double* d_rec, const int
                             int64_t n, double mu, double lam, double* d_out, int64_t si,
                             int64_t sj, int64_t plane, int64_t out_len, void* stream) {
   clear_error();
-  return traction_block_launch(d_recv, d_rec, d_rows, row0, m, d_cols, col0, n, mu, lam, d_out,
-                               si, sj, plane, out_len, 0, stream);
+  return traction_block_launch(d_recv, d_rec, nullptr, d_rows, row0, m, d_cols, col0, n, mu, lam,
+                               d_out, si, sj, plane, out_len, 0, stream);
 }
 
 int pq3d_tde_traction_block_f32(const double* d_recv, const double* d_rec,
@@ -247,8 +337,30 @@ int pq3d_tde_traction_block_f32(const double* d_recv, const double* d_rec,
                                 double lam, float* d_out, int64_t si, int64_t sj, int64_t plane,
                                 int64_t out_len, void* stream) {
   clear_error();
-  return traction_block_launch(d_recv, d_rec, d_rows, row0, m, d_cols, col0, n, mu, lam, d_out,
-                               si, sj, plane, out_len, 1, stream);
+  return traction_block_launch(d_recv, d_rec, nullptr, d_rows, row0, m, d_cols, col0, n, mu, lam,
+                               d_out, si, sj, plane, out_len, 1, stream);
+}
+
+int pq3d_tde_traction_block_hs(const double* d_recv, const double* d_rec, const double* d_hrec,
+                               const int64_t* d_rows, int64_t row0, int64_t m,
+                               const int64_t* d_cols, int64_t col0, int64_t n, double mu,
+                               double lam, double* d_out, int64_t si, int64_t sj, int64_t plane,
+                               int64_t out_len, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(d_hrec, "null half-space records");
+  return traction_block_launch(d_recv, d_rec, d_hrec, d_rows, row0, m, d_cols, col0, n, mu, lam,
+                               d_out, si, sj, plane, out_len, 0, stream);
+}
+
+int pq3d_tde_traction_block_hs_f32(const double* d_recv, const double* d_rec,
+                                   const double* d_hrec, const int64_t* d_rows, int64_t row0,
+                                   int64_t m, const int64_t* d_cols, int64_t col0, int64_t n,
+                                   double mu, double lam, float* d_out, int64_t si, int64_t sj,
+                                   int64_t plane, int64_t out_len, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(d_hrec, "null half-space records");
+  return traction_block_launch(d_recv, d_rec, d_hrec, d_rows, row0, m, d_cols, col0, n, mu, lam,
+                               d_out, si, sj, plane, out_len, 1, stream);
 }
 
 // ------------------------------------------------ drop-in host entry points
@@ -260,11 +372,11 @@ static void fill_nan(double* p, size_t n) {
 
 static int host_stress_points(const double* X, const double* Y, const double* Z, size_t n,
                               const double* P1, const double* P2, const double* P3, double Ss,
-                              double Ds, double Ts, double mu, double lam, double* stress,
-                              double* strain) {
+                              double Ds, double Ts, double mu, double lam, bool hs,
+                              double* stress, double* strain) {
   std::lock_guard<std::mutex> lk(g_pool.mu);
   const size_t nb = n * sizeof(double);
-  const size_t total = 3 * nb + 9 * sizeof(double) + kSrcRec * sizeof(double) + 12 * nb;
+  const size_t total = 3 * nb + (kSrcRec + kHsRec) * sizeof(double) + 12 * nb;
   void* base = nullptr;
   int rc = pool_reserve(total, &base);
   if (rc) return rc;
@@ -272,19 +384,21 @@ static int host_stress_points(const double* X, const double* Y, const double* Z,
   double* dX = (double*)b;
   double* dY = (double*)(b + nb);
   double* dZ = (double*)(b + 2 * nb);
-  double* dT = (double*)(b + 3 * nb);
-  double* dR = dT + 9;
-  double* dS = dR + kSrcRec;
+  double* dR = (double*)(b + 3 * nb);
+  double* dH = dR + kSrcRec;
+  double* dS = dH + kHsRec;
   double* dE = dS + 6 * n;
   hipStream_t s = g_pool.stream;
-  double tri[9] = {P1[0], P1[1], P1[2], P2[0], P2[1], P2[2], P3[0], P3[1], P3[2]};
   PQ3D_HIP_CHECK(hipMemcpyAsync(dX, X, nb, hipMemcpyHostToDevice, s));
   PQ3D_HIP_CHECK(hipMemcpyAsync(dY, Y, nb, hipMemcpyHostToDevice, s));
   PQ3D_HIP_CHECK(hipMemcpyAsync(dZ, Z, nb, hipMemcpyHostToDevice, s));
-  double rec[kSrcRec];
-  tde_source_record_host(tri, tri + 3, tri + 6, rec);
-  PQ3D_HIP_CHECK(hipMemcpyAsync(dR, rec, sizeof(rec), hipMemcpyHostToDevice, s));
-  rc = pq3d_tde_stress_points(dX, dY, dZ, (int64_t)n, dR, Ss, Ds, Ts, mu, lam, dS, dE, s);
+  double rec[kSrcRec + kHsRec];
+  tde_source_record_host(P1, P2, P3, rec);
+  if (hs) tde_hs_record_host(P1, P2, P3, rec + kSrcRec);
+  PQ3D_HIP_CHECK(hipMemcpyAsync(dR, rec, (hs ? kSrcRec + kHsRec : kSrcRec) * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+  rc = stress_points_launch(dX, dY, dZ, (int64_t)n, dR, hs ? dH : nullptr, Ss, Ds, Ts, mu, lam,
+                            dS, dE, s);
   if (rc) return rc;
   PQ3D_HIP_CHECK(hipMemcpyAsync(stress, dS, 6 * nb, hipMemcpyDeviceToHost, s));
   PQ3D_HIP_CHECK(hipMemcpyAsync(strain, dE, 6 * nb, hipMemcpyDeviceToHost, s));
@@ -294,23 +408,29 @@ static int host_stress_points(const double* X, const double* Y, const double* Z,
 
 static int host_stress_sources(double x, double y, double z, const double* P1f,
                                const double* P2f, const double* P3f, size_t ns, double Ss,
-                               double Ds, double Ts, double mu, double lam, double* stress,
-                               double* strain) {
+                               double Ds, double Ts, double mu, double lam, bool hs,
+                               double* stress, double* strain) {
   std::lock_guard<std::mutex> lk(g_pool.mu);
-  const size_t total = (kSrcRec * ns + 12 * ns) * sizeof(double);
+  const size_t nrec = (size_t)kSrcRec * ns, nh = hs ? (size_t)kHsRec * ns : 0;
+  const size_t total = (nrec + nh + 12 * ns) * sizeof(double);
   void* base = nullptr;
   int rc = pool_reserve(total, &base);
   if (rc) return rc;
   double* dR = (double*)base;
-  double* dS = dR + (size_t)kSrcRec * ns;
+  double* dH = dR + nrec;
+  double* dS = dH + nh;
   double* dE = dS + 6 * ns;
   hipStream_t s = g_pool.stream;
-  std::vector<double> recs((size_t)kSrcRec * ns);
-  for (size_t i = 0; i < ns; ++i)
+  std::vector<double> recs(nrec + nh);
+  for (size_t i = 0; i < ns; ++i) {
     tde_source_record_host(P1f + 3 * i, P2f + 3 * i, P3f + 3 * i, recs.data() + kSrcRec * i);
+    if (hs)
+      tde_hs_record_host(P1f + 3 * i, P2f + 3 * i, P3f + 3 * i, recs.data() + nrec + kHsRec * i);
+  }
   PQ3D_HIP_CHECK(hipMemcpyAsync(dR, recs.data(), recs.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
-  rc = pq3d_tde_stress_sources(x, y, z, dR, (int64_t)ns, Ss, Ds, Ts, mu, lam, dS, dE, s);
+  rc = stress_sources_launch(x, y, z, dR, hs ? dH : nullptr, (int64_t)ns, Ss, Ds, Ts, mu, lam, dS,
+                             dE, s);
   if (rc) return rc;
   PQ3D_HIP_CHECK(hipMemcpyAsync(stress, dS, 6 * ns * sizeof(double), hipMemcpyDeviceToHost, s));
   PQ3D_HIP_CHECK(hipMemcpyAsync(strain, dE, 6 * ns * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -319,22 +439,17 @@ static int host_stress_sources(double x, double y, double z, const double* P1f,
 }
 
 // Same signature and output layout as the reference (src/TDstressFS_C.cpp:2048-2095):
-// n receivers x 1 triangle, stress and strain n x 6 row-major (xx,yy,zz,xy,xz,yz).
-// The reference returns nothing; errors here fill the outputs with NaN and leave
-// the message in pq3d_last_error(). The half-space branch (jud_halfspace=true)
-// is not provided by this library and is reported the same way.
+// n receivers x 1 triangle, stress and strain n x 6 row-major (xx,yy,zz,xy,xz,yz);
+// jud_halfspace selects the half-space solution (free surface z = 0). The
+// reference returns nothing; errors here fill the outputs with NaN and leave the
+// message in pq3d_last_error().
 void TDstressFS_C(const double* X, const double* Y, const double* Z, size_t n, const double* P1,
                   const double* P2, const double* P3, double Ss, double Ds, double Ts, double mu,
                   double lambda_, bool jud_halfspace, double* stress, double* strain) {
   clear_error();
   if (n == 0) return;
-  int rc;
-  if (jud_halfspace) {
-    set_error("TDstressFS_C: half-space (jud_halfspace=true) is not supported by this library");
-    rc = PQ3D_ENOTSUP;
-  } else {
-    rc = host_stress_points(X, Y, Z, n, P1, P2, P3, Ss, Ds, Ts, mu, lambda_, stress, strain);
-  }
+  const int rc = host_stress_points(X, Y, Z, n, P1, P2, P3, Ss, Ds, Ts, mu, lambda_,
+                                    jud_halfspace, stress, strain);
   if (rc) {
     fprintf(stderr, "pyquake3d_amd: %s\n", pq3d_last_error());
     fill_nan(stress, 6 * n);
@@ -351,16 +466,8 @@ void TDstressEachSourceAtReceiver_C(double x, double y, double z, const double* 
                                     double* strain_out) {
   clear_error();
   if (n_sources == 0) return;
-  int rc;
-  if (jud_halfspace) {
-    set_error(
-        "TDstressEachSourceAtReceiver_C: half-space (jud_halfspace=true) is not supported by "
-        "this library");
-    rc = PQ3D_ENOTSUP;
-  } else {
-    rc = host_stress_sources(x, y, z, P1_flat, P2_flat, P3_flat, n_sources, Ss, Ds, Ts, mu,
-                             lambda_, stress_out, strain_out);
-  }
+  const int rc = host_stress_sources(x, y, z, P1_flat, P2_flat, P3_flat, n_sources, Ss, Ds, Ts,
+                                     mu, lambda_, jud_halfspace, stress_out, strain_out);
   if (rc) {
     fprintf(stderr, "pyquake3d_amd: %s\n", pq3d_last_error());
     fill_nan(stress_out, 6 * n_sources);

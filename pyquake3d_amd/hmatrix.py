@@ -29,7 +29,7 @@ import torch
 from . import _lib
 from .cluster import build_block_partition, build_cluster_tree
 from .mesh import FaultMesh, get_eleVec, receivers
-from .tde import source_records
+from .tde import hs_records, source_records
 
 DENSE_CHUNK = 32   # columns per dense work item
 LR_CHUNK = 16      # rank columns per low-rank work item
@@ -154,7 +154,7 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
 class HMatrix:
     def __init__(self, nodelst, elelst, mu, lambda_, *, leaf=64, eta=2.0, eps=1e-4,
                  dtype=torch.float32, device="cuda", kmax=128, aca_budget_gb=24.0,
-                 jud_ele_order=False, verbose=False):
+                 jud_ele_order=False, verbose=False, halfspace=False):
         t_all = time.perf_counter()
         self.device = torch.device(device)
         self.dtype = dtype
@@ -183,6 +183,10 @@ class HMatrix:
             receivers(self.eleVec, self.xg)[self.perm])).to(dev)
         self.rec = torch.from_numpy(source_records(tri[self.perm])).to(dev)
         self.tri_perm = np.ascontiguousarray(tri[self.perm])
+        # half space (free surface z = 0): second per-triangle record
+        self.halfspace = bool(halfspace)
+        self.hrec = (torch.from_numpy(hs_records(self.tri_perm)).to(dev) if self.halfspace
+                     else None)
 
         t = self.tree
         P = self.part
@@ -231,8 +235,8 @@ class HMatrix:
         _lib.check_width("recv", self.recv)
         _lib.check_width("rec", self.rec)
         _lib.call("pq3d_hmat_dense_assemble", _lib.ptr(self.dtab), len(dense_idx),
-                  _lib.ptr(self.recv), _lib.ptr(self.rec), self.mu, self.lambda_,
-                  _lib.ptr(self.pool), self.esize, stream)
+                  _lib.ptr(self.recv), _lib.ptr(self.rec), _lib.ptr(self.hrec), self.mu,
+                  self.lambda_, _lib.ptr(self.pool), self.esize, stream)
         _sync(dev)
         tm["dense"] = time.perf_counter() - t0
 
@@ -299,7 +303,8 @@ class HMatrix:
         _lib.check_width("aca_ws", wsoff)
         rank = torch.empty(nb, dtype=torch.int64, device=dev)
         _lib.call("pq3d_hmat_aca", _lib.ptr(tab_t), _lib.ptr(wsoff), nb, _lib.ptr(self.recv),
-                  _lib.ptr(self.rec), self.mu, self.lambda_, float(eps), int(kcap.max()),
+                  _lib.ptr(self.rec), _lib.ptr(self.hrec), self.mu, self.lambda_, float(eps),
+                  int(kcap.max()),
                   _lib.ptr(wsU), _lib.ptr(wsV), _lib.ptr(flags), _lib.ptr(rank), stream)
         rk = rank.cpu().numpy()
         return rk, wsU, wsV, wsoff, blocks

@@ -1,20 +1,23 @@
 """Learned step-size proposal for the RK45 stepper.
 
-A 6 -> 64 -> 64 -> 64 -> 1 MLP (FP32, tanh) maps the solver state of each
-simulation to log10 of the next step size. Inputs per instance:
-  [max log10 V, min log10 V, mean log10(V theta/Dc), log10 last dt,
-   log10 last error ratio, fraction of cells with V > 1e-3 m/s]
-(the cell reductions are one HIP kernel, ``pq3d_features``). The network is
-trained on the GPU with the library's own forward/backward/Adam kernels on
-(state, target) pairs logged from runs of the plain controller; the target is
-log10 of the controller's own estimate of the largest acceptable step at the
-accepted attempt, 0.9 h (tol/err)^0.2, without the reference's 1.5 h growth
-cap (src/QDsim.py:941-942).
+A 6 -> 64 -> 64 -> 64 -> 1 MLP (FP32, tanh) corrects the step the reference
+controller would try next. Inputs per instance:
+  [max log10 V, min log10 V, mean log10(V theta/Dc), fraction of cells with
+   V > 1e-3 m/s, log10 of the accepted attempt's error ratio max(c1, c2),
+   log10(dtnext / h) of the reference controller]
+(the cell reductions are one HIP kernel, ``pq3d_features``). The target is
+log10(h_accepted / dtnext): how far the reference controller
+(src/QDsim.py:931-950) ended up shrinking its own proposal dtnext through
+rejections. The network is trained on the GPU with the library's forward,
+backward and Adam kernels on (features, target) pairs logged from runs of the
+reference controller.
 
-In ``LearnedEnsemble`` every step still goes through the reference's embedded
-error test: the proposal is the first attempt's h; a rejected attempt falls
-back to the PI controller (Gustafsson's h (tol/err)^0.14 (err_prev/tol)^0.08
-with 0.9 safety), as does a non-finite proposal.
+In ``FeatureEnsemble`` with a network every step still goes through the
+reference's embedded error test: the first attempt uses
+dtnext * 10^min(prediction - margin, 0); a rejected attempt falls back to a
+PI controller (Gustafsson: h (1/err)^0.14 (err_prev)^0.08 with 0.9 safety,
+clamped to [0.2 h, 0.9 h]), as does a non-finite prediction. The reference's
+dtnext rule (0.9 h (tol/err)^0.2, at most 1.5 h) is unchanged.
 """
 from __future__ import annotations
 
@@ -135,21 +138,21 @@ class StepMLP:
 
 
 class FeatureEnsemble(EnsembleQDsim):
-    """EnsembleQDsim that records per-instance features and, optionally,
-    proposes step sizes with a StepMLP (learned dt)."""
+    """EnsembleQDsim that records per-instance features and targets and,
+    with a trained StepMLP, proposes the first attempt's step size."""
 
-    def __init__(self, problems, op, mlp: StepMLP | None = None, growth_cap=8.0, **kw):
+    def __init__(self, problems, op, mlp: StepMLP | None = None, margin=0.0, **kw):
         super().__init__(problems, op, **kw)
         self.mlp = mlp
-        self.growth_cap = growth_cap
+        self.margin = margin
         self.acc = torch.zeros(4 * self.nb, dtype=torch.int64, device=self.device)
         self.V0 = float(problems[0].V0)
-        self.last_dt = np.full(self.nb, 1e-3)
-        self.last_ratio = np.full(self.nb, 1e-2)
-        self.prev_err = np.full(self.nb, 1.0)
+        self.last_ratio = np.full(self.nb, 0.5)      # max(c1, c2) at the last acceptance
+        self.last_growth = np.ones(self.nb)          # dtnext / h at the last acceptance
         self.log_feats, self.log_target = [], []
         self.record = False
-        self.nproposals_rejected = 0
+        self.nfirst_rejected = 0                      # steps whose first attempt was rejected
+        self.nfallback = 0                            # PI fallback attempts
 
     def features(self) -> np.ndarray:
         _lib.call("pq3d_features", self.ne, self.nb, _lib.ptr(self.slipv), _lib.ptr(self.state),
@@ -157,26 +160,29 @@ class FeatureEnsemble(EnsembleQDsim):
                   _lib.stream_handle())
         a = self.acc.cpu().numpy().reshape(self.nb, 4)
         vmax, vmin, mean, frac = decode_features(a, self.n)
-        return np.stack([vmax, vmin, mean, np.log10(self.last_dt), np.log10(self.last_ratio),
-                         frac], axis=1)
+        return np.stack([vmax, vmin, mean, frac, np.log10(self.last_ratio),
+                         np.log10(self.last_growth)], axis=1)
 
     def step(self):
-        """One accepted step per instance; first attempt uses the proposal."""
+        """One accepted step per instance."""
         f = self.features() if (self.mlp is not None or self.record) else None
         self.compute_K()
-        h = self.dt.copy()
+        base = self.dt.copy()                         # the reference controller's dtnext
+        h = base.copy()
+        learned = np.zeros(self.nb, dtype=bool)
         if self.mlp is not None:
-            prop = 10.0 ** self.mlp.predict_log_dt(f)
-            ok = np.isfinite(prop) & (prop > 0)
-            h = np.where(ok, np.minimum(prop, self.growth_cap * self.last_dt), h)
+            pred = self.mlp.predict_log_dt(f)
+            learned = np.isfinite(pred)
+            corr = np.where(learned, np.minimum(pred - self.margin, 0.0), 0.0)
+            h = np.where(learned, base * 10.0 ** corr, base)
         tol = self.reltol
         done = np.zeros(self.nb, dtype=bool)
         nrj = np.zeros(self.nb, dtype=np.int64)
         dtnext = np.zeros(self.nb)
-        target = np.zeros(self.nb)
-        ratio = np.zeros(self.nb)
+        ratio = np.ones(self.nb)
+        prev = np.maximum(self.last_ratio, 1e-6)
         while not done.all():
-            e = self._attempt(h)
+            e = self._attempt(np.where(done, -1.0, h))
             for b in np.where(~done)[0]:
                 R1 = np.float64(e[b, 0]) + 1e-10
                 R2 = max(np.float64(e[b, 1]), np.float64(e[b, 2]), np.float64(e[b, 3])) + 1e-10
@@ -185,26 +191,26 @@ class FeatureEnsemble(EnsembleQDsim):
                 hn2 = h[b] * 0.9 * (tol / R2) ** 0.2
                 if max(c1, c2) < 1.0 and not (np.isnan(c1) or np.isnan(c2)):
                     dtnext[b] = min(min(hn1, hn2), 1.5 * h[b])
-                    target[b] = min(hn1, hn2)
                     ratio[b] = max(c1, c2)
                     done[b] = True
+                    continue
+                nrj[b] += 1
+                if nrj[b] == 1:
+                    self.nfirst_rejected += 1
+                err = max(c1, c2)
+                if learned[b] and np.isfinite(err):
+                    # PI fallback (Gustafsson), 0.9 safety, within [0.2 h, 0.9 h]
+                    self.nfallback += 1
+                    hpi = h[b] * 0.9 * (1.0 / err) ** 0.14 * prev[b] ** 0.08
+                    h[b] = min(max(0.2 * h[b], hpi), 0.9 * h[b])
                 else:
-                    nrj[b] += 1
-                    if self.mlp is not None and nrj[b] == 1:
-                        self.nproposals_rejected += 1
-                    err = max(c1, c2)
-                    if self.mlp is not None and np.isfinite(err):
-                        # PI fallback (Gustafsson): h (1/err)^0.14 (prev/1)^0.08, 0.9 safety
-                        hpi = h[b] * 0.9 * (1.0 / err) ** 0.14 * (self.prev_err[b]) ** 0.08
-                        h[b] = min(max(0.2 * h[b], hpi), 0.9 * h[b])
-                    else:
-                        h[b] = max(0.5 * h[b], min(hn1, hn2))
-                    if h[b] < 1e-15 or nrj[b] > 20:
-                        raise StepAborted(f"instance {b}: dt is too small (h={h[b]:g})")
+                    h[b] = max(0.5 * h[b], min(hn1, hn2))
+                if h[b] < 1e-15 or nrj[b] > 20:
+                    raise StepAborted(f"instance {b}: dt is too small (h={h[b]:g})")
         self.nrejects += nrj
         if self.record:
             self.log_feats.append(f)
-            self.log_target.append(np.log10(target))
+            self.log_target.append(np.log10(h / base))
         self.hdev.copy_(torch.from_numpy(h))
         _lib.call("pq3d_rk45_commit", self.ne, _lib.ptr(self.inst), _lib.ptr(self.hdev), 0,
                   _lib.ptr(self.a), _lib.ptr(self.cellpar), _lib.ptr(self.y[0]),
@@ -214,10 +220,9 @@ class FeatureEnsemble(EnsembleQDsim):
                   _lib.ptr(self.slipv), _lib.ptr(self.rake), _lib.ptr(self.slip1),
                   _lib.ptr(self.slip2), _lib.ptr(self.vmax), self.nb, _lib.stream_handle())
         self.time += h
+        self.last_growth = dtnext / h
         self.dt = dtnext
-        self.last_dt = h.copy()
         self.last_ratio = np.maximum(ratio, 1e-12)
-        self.prev_err = np.maximum(ratio, 1e-6)
         self.nsteps += 1
         return h
 

@@ -53,14 +53,19 @@ class DenseOperator:
         self._x = torch.empty(2 * n, dtype=dtype, device=self.device)
         self.matvec_bytes = self.M.numel() * self.M.element_size() + 5 * 8 * n
 
-    def matvec(self, v1, v2, out=None, stream=None):
+    def matvec(self, v1, v2, out=None, stream=None, nrhs=1):
+        """nrhs > 1: v1, v2, out rows interleaved by instance (cell*nrhs + b)."""
         n = self.n
         if out is None:
-            out = torch.empty((3, n), dtype=torch.float64, device=self.device)
-        self._x[:n] = v1
-        self._x[n:] = v2
-        y = torch.mv(self.M, self._x.to(self.M.dtype))
-        out.view(-1).copy_(y)
+            out = torch.empty((3, n * nrhs), dtype=torch.float64, device=self.device)
+        if nrhs == 1:
+            self._x[:n] = v1
+            self._x[n:] = v2
+            y = torch.mv(self.M, self._x.to(self.M.dtype))
+            out.view(-1).copy_(y)
+        else:
+            X = torch.cat([v1.view(n, nrhs), v2.view(n, nrhs)]).to(self.M.dtype)
+            out.view(3 * n, nrhs).copy_(self.M @ X)
         return out
 
 
@@ -265,9 +270,9 @@ class EnsembleQDsim:
     instance keeps its own step size and the reference's accept/reject
     controller. Element (cell i, instance b) is stored at i*NB + b.
 
-    A lockstep iteration ends when every instance has accepted its step; an
-    instance that accepted early is recomputed with the same h on the retry
-    rounds of the others (same inputs, same candidate)."""
+    A lockstep iteration ends when every instance has accepted its step; the
+    retry rounds recompute only the instances that were rejected (the others
+    pass h < 0 and keep their accepted candidate)."""
 
     def __init__(self, problems, op, reltol: float = 1e-4, device="cuda", tableau: str = "ref"):
         nb = len(problems)
@@ -375,7 +380,8 @@ class EnsembleQDsim:
         dtnext = np.zeros(self.nb)
         tol = self.reltol
         while not done.all():
-            e = self._attempt(h)
+            # instances that accepted in an earlier round are left out (h < 0)
+            e = self._attempt(np.where(done, -1.0, h))
             for b in np.where(~done)[0]:
                 R1 = np.float64(e[b, 0]) + 1e-10
                 R2 = max(np.float64(e[b, 1]), np.float64(e[b, 2]), np.float64(e[b, 3])) + 1e-10
@@ -422,3 +428,70 @@ class EnsembleQDsim:
         out = np.empty(self.n)
         out[self.perm] = x
         return out
+
+
+class DeviceEnsemble(EnsembleQDsim):
+    """EnsembleQDsim whose accept/reject controller runs on the device.
+
+    One ``step()`` enqueues the operator product, ``rounds`` rounds of
+    (attempt of every instance still attempting, the reference's decision,
+    src/QDsim.py:924-958) and the commit of the accepted instances, with no
+    host synchronisation. An instance that needs more than ``rounds``
+    attempts keeps its state and reduced h and continues in the next
+    ``step()``: the frozen operator product is recomputed from the same state
+    (identical values), so the accepted step is the one the reference loop
+    would take. ``sync()`` copies time, next dt and step counts to the host
+    and raises StepAborted for an instance whose dt fell below 1e-15 s or
+    that was rejected more than 20 times."""
+
+    def __init__(self, problems, op, reltol: float = 1e-4, device="cuda", tableau: str = "ref",
+                 rounds: int = 8, dt0: float = 1e-3):
+        super().__init__(problems, op, reltol=reltol, device=device, tableau=tableau)
+        dev = self.device
+        nb = self.nb
+        self.rounds = int(rounds)
+        self.hcur = torch.full((nb,), float(dt0), dtype=torch.float64, device=dev)
+        self.dtnext = torch.zeros(nb, dtype=torch.float64, device=dev)
+        self.flag = torch.zeros(nb, dtype=torch.uint8, device=dev)
+        self.nrj = torch.zeros(nb, dtype=torch.int32, device=dev)
+        self.status = torch.zeros(nb, dtype=torch.int32, device=dev)
+        self.tdev = torch.zeros(nb, dtype=torch.float64, device=dev)
+        self.nacc = torch.zeros(nb, dtype=torch.int64, device=dev)
+        self.iterations = 0
+        self.naccepted = np.zeros(nb, dtype=np.int64)
+
+    def step(self):
+        s = _lib.stream_handle()
+        self.compute_K()
+        _lib.call("pq3d_rk45_rounds", self.ne, _lib.ptr(self.inst), _lib.ptr(self.hcur),
+                  _lib.ptr(self.flag), _lib.ptr(self.nrj), _lib.ptr(self.dtnext),
+                  _lib.ptr(self.status), self.rounds, self.reltol, _lib.ptr(self.Tno),
+                  _lib.ptr(self.Tt1o), _lib.ptr(self.Tt2o), _lib.ptr(self.state),
+                  _lib.ptr(self.a), _lib.ptr(self.b), _lib.ptr(self.dc), _lib.ptr(self.f0),
+                  _lib.ptr(self.K[0]), _lib.ptr(self.K[1]), _lib.ptr(self.K[2]),
+                  _lib.ptr(self.shear_loading), _lib.ptr(self.slipv), _lib.ptr(self.cellpar),
+                  _lib.ptr(self.y[0]), _lib.ptr(self.y[1]), _lib.ptr(self.y[2]),
+                  _lib.ptr(self.y[3]), _lib.ptr(self.err), self.a63, self.nb, s)
+        _lib.call("pq3d_rk45_advance", self.ne, _lib.ptr(self.inst), _lib.ptr(self.hcur),
+                  _lib.ptr(self.flag), _lib.ptr(self.nrj), _lib.ptr(self.dtnext),
+                  _lib.ptr(self.tdev), _lib.ptr(self.nacc), _lib.ptr(self.a),
+                  _lib.ptr(self.cellpar), _lib.ptr(self.y[0]), _lib.ptr(self.y[1]),
+                  _lib.ptr(self.y[2]), _lib.ptr(self.y[3]), _lib.ptr(self.Tno),
+                  _lib.ptr(self.Tt1o), _lib.ptr(self.Tt2o), _lib.ptr(self.state),
+                  _lib.ptr(self.slipv1), _lib.ptr(self.slipv2), _lib.ptr(self.slipv),
+                  _lib.ptr(self.rake), _lib.ptr(self.slip1), _lib.ptr(self.slip2),
+                  _lib.ptr(self.vmax), self.nb, s)
+        self.iterations += 1
+
+    def sync(self):
+        """Host copies of time, next dt and accepted-step counts; raises
+        StepAborted if an instance aborted."""
+        st = self.status.cpu().numpy()
+        self.time = self.tdev.cpu().numpy().copy()
+        self.dt = self.hcur.cpu().numpy().copy()
+        self.naccepted = self.nacc.cpu().numpy().copy()
+        self.nsteps = int(self.naccepted.min())
+        if st.any():
+            b = int(np.argmax(st != 0))
+            raise StepAborted(f"instance {b}: dt is too small or more than 20 rejections")
+        return self.time

@@ -74,6 +74,16 @@ def source_records(tri: np.ndarray) -> np.ndarray:
     return rec
 
 
+def hs_records(tri: np.ndarray) -> np.ndarray:
+    """(N, 104) per-triangle half-space records (image triangle, side
+    frames), computed on the host by the library."""
+    tri = np.ascontiguousarray(tri, dtype=np.float64).reshape(-1, 9)
+    h = np.zeros((tri.shape[0], 104))
+    _lib.check(_lib.load().pq3d_tde_hs_records(_dp(tri), tri.shape[0], _dp(h)),
+               "pq3d_tde_hs_records")
+    return h
+
+
 class DeviceMesh:
     """Device-resident geometry of one fault mesh.
 
@@ -82,7 +92,8 @@ class DeviceMesh:
     recv (N, 12) FP64 receiver records: centroid xg and frame ev1, ev2, ev3
     """
 
-    def __init__(self, nodelst, elelst, mu, lambda_, device="cuda", jud_ele_order=False):
+    def __init__(self, nodelst, elelst, mu, lambda_, device="cuda", jud_ele_order=False,
+                 halfspace=False):
         self.mu = float(mu)
         self.lambda_ = float(lambda_)
         self.device = torch.device(device)
@@ -96,10 +107,14 @@ class DeviceMesh:
         self.tri = torch.from_numpy(np.ascontiguousarray(tri)).to(self.device)
         self.recv = torch.from_numpy(receivers(self.eleVec, self.xg)).to(self.device)
         self.rec = torch.from_numpy(source_records(tri)).to(self.device)
+        # half space (free surface z = 0): second per-triangle record
+        self.halfspace = bool(halfspace)
+        self.hrec = (torch.from_numpy(hs_records(tri)).to(self.device) if self.halfspace
+                     else None)
 
     @classmethod
-    def from_mesh(cls, mesh: FaultMesh, mu, lambda_, device="cuda"):
-        return cls(mesh.nodelst, mesh.elelst, mu, lambda_, device)
+    def from_mesh(cls, mesh: FaultMesh, mu, lambda_, device="cuda", halfspace=False):
+        return cls(mesh.nodelst, mesh.elelst, mu, lambda_, device, halfspace=halfspace)
 
     def _index(self, idx):
         if idx is None:
@@ -134,12 +149,14 @@ class DeviceMesh:
         if tuple(out.shape) != shape or out.dtype != dtype or not out.is_contiguous():
             raise ValueError("out has the wrong shape, dtype or layout")
         si, sj = (n, 1) if order == "ij" else (1, m)
-        fn = "pq3d_tde_traction_block" if dtype == torch.float64 else "pq3d_tde_traction_block_f32"
         if dtype not in (torch.float64, torch.float32):
             raise ValueError("dtype must be float64 or float32")
+        fn = "pq3d_tde_traction_block" + ("_hs" if self.halfspace else "") + \
+            ("" if dtype == torch.float64 else "_f32")
         _lib.check_width("recv", self.recv)
         _lib.check_width("rec", self.rec)
-        _lib.call(fn, _lib.ptr(self.recv), _lib.ptr(self.rec), _lib.ptr(r), row0, m, _lib.ptr(c),
-                  col0, n, self.mu, self.lambda_, _lib.ptr(out), si, sj, m * n, out.numel(),
-                  _lib.stream_handle(stream))
+        hs = (_lib.ptr(self.hrec),) if self.halfspace else ()
+        _lib.call(fn, _lib.ptr(self.recv), _lib.ptr(self.rec), *hs, _lib.ptr(r), row0, m,
+                  _lib.ptr(c), col0, n, self.mu, self.lambda_, _lib.ptr(out), si, sj, m * n,
+                  out.numel(), _lib.stream_handle(stream))
         return out

@@ -33,7 +33,8 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def golden():
-    return {k: np.load(GOLDEN / f"{k}.npz") for k in ("g1_points", "g2_sources", "g3_blocks")}
+    return {k: np.load(GOLDEN / f"{k}.npz") for k in ("g1_points", "g2_sources", "g3_blocks",
+                                                      "g4_halfspace")}
 
 
 @pytest.fixture(scope="session")

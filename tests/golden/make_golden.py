@@ -12,8 +12,14 @@ binary). The fixtures it writes are data: inputs and the reference outputs.
                   reference C++ per source column (as src/Hmatrix.py:425-511
                   calls it) composed with the projection of
                   src/Hmatrix.py:295-313
+  g4_halfspace.npz TDstressFS_C with the half-space flag
+                  (src/TDstressFS_C.cpp:1911-1999): buried, vertical (a
+                  vertical side in each of the three positions),
+                  surface-breaking and dipping triangles; receivers below and
+                  on the free surface; and the six traction operators of a
+                  small surface-breaking mesh
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [g1 g2 g3 g4]   (default: all)
 """
 from __future__ import annotations
 
@@ -50,14 +56,14 @@ def p(a):
     return a.ctypes.data_as(D)
 
 
-def ref_points(lib, X, Y, Z, P1, P2, P3, Ss, Ds, Ts, mu, lam):
+def ref_points(lib, X, Y, Z, P1, P2, P3, Ss, Ds, Ts, mu, lam, halfspace=False):
     n = len(X)
     s = np.zeros(6 * n)
     e = np.zeros(6 * n)
     X, Y, Z = (np.ascontiguousarray(v, dtype=np.float64) for v in (X, Y, Z))
     P1, P2, P3 = (np.ascontiguousarray(v, dtype=np.float64) for v in (P1, P2, P3))
-    lib.TDstressFS_C(p(X), p(Y), p(Z), n, p(P1), p(P2), p(P3), Ss, Ds, Ts, mu, lam, False, p(s),
-                     p(e))
+    lib.TDstressFS_C(p(X), p(Y), p(Z), n, p(P1), p(P2), p(P3), Ss, Ds, Ts, mu, lam, halfspace,
+                     p(s), p(e))
     return s.reshape(n, 6), e.reshape(n, 6)
 
 
@@ -174,7 +180,7 @@ def traction(stress, f):
                      t1 * f[6] + t2 * f[7] + tn * f[8]], axis=1)
 
 
-def ref_block(lib, mesh, mu, lam):
+def ref_block(lib, mesh, mu, lam, halfspace=False):
     tri = mesh.triangles()
     eleVec, xg = get_eleVec(mesh.nodelst, mesh.elelst)
     n = mesh.n
@@ -182,7 +188,7 @@ def ref_block(lib, mesh, mu, lam):
     for j in range(n):
         for op0, (Ss, Ds) in ((0, (1.0, 0.0)), (3, (0.0, 1.0))):
             s, _ = ref_points(lib, xg[:, 0], xg[:, 1], xg[:, 2], tri[j, 0:3], tri[j, 3:6],
-                              tri[j, 6:9], Ss, Ds, 0.0, mu, lam)
+                              tri[j, 6:9], Ss, Ds, 0.0, mu, lam, halfspace)
             for i in range(n):
                 t = traction(s[i:i + 1], eleVec[i])[0]
                 K[op0:op0 + 3, i, j] = t
@@ -201,12 +207,66 @@ def make_g3(lib):
     return out
 
 
+def make_g4(lib):
+    rng = np.random.default_rng(4404)
+    tris = []
+    for _ in range(4):   # buried, random orientation
+        P = rng.normal(size=(3, 3)) * 800.0
+        P[:, 2] = -np.abs(P[:, 2]) - 1500.0
+        tris.append(P)
+    # vertical triangles of a ruled mesh: the vertical side first, second, third
+    tris.append(np.array([[0.0, 0.0, -1000.0], [0.0, 0.0, -2000.0], [1000.0, 0.0, -2000.0]]))
+    tris.append(np.array([[1000.0, 0.0, -2000.0], [0.0, 0.0, -1000.0], [0.0, 0.0, -2000.0]]))
+    tris.append(np.array([[0.0, 0.0, -2000.0], [1000.0, 0.0, -2000.0], [0.0, 0.0, -1000.0]]))
+    # vertical surface-breaking: a side along z = 0
+    tris.append(np.array([[0.0, 0.0, 0.0], [1000.0, 0.0, -1000.0], [1000.0, 0.0, 0.0]]))
+    # dipping, one vertex at the surface
+    tris.append(np.array([[0.0, 0.0, 0.0], [1500.0, 300.0, -1200.0], [-200.0, 1200.0, -800.0]]))
+    # undulating-strand triangle (vertical, not in a coordinate plane)
+    tris.append(np.array([[200.0, 150.0, -3000.0], [200.0, 150.0, -2000.0],
+                          [1100.0, -120.0, -3000.0]]))
+    recs = []
+    for ti, P in enumerate(tris):
+        c = P.mean(0)
+        L = np.linalg.norm(P[1] - P[0])
+        d = rng.normal(size=(40, 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        pts = c + d * (L * 10 ** rng.uniform(-1.5, 1.0, size=(40, 1)))
+        pts[:, 2] = -np.abs(pts[:, 2])                      # below the surface
+        surf = np.stack([rng.uniform(-3e3, 3e3, 12), rng.uniform(-3e3, 3e3, 12),
+                         np.zeros(12)], 1)                 # on the free surface
+        pts = np.concatenate([pts, surf], 0)
+        for (Ss, Ds, Ts) in ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (0.6, -0.3, 0.2)):
+            for (mu, lam) in ((32e9, 32e9), (30e9, 45e9)):
+                if (mu, lam) != (32e9, 32e9) and (Ss, Ds, Ts) != (0.6, -0.3, 0.2):
+                    continue
+                st, sn = ref_points(lib, pts[:, 0], pts[:, 1], pts[:, 2], P[0], P[1], P[2], Ss,
+                                    Ds, Ts, mu, lam, halfspace=True)
+                recs.append(dict(tri=np.repeat(P.reshape(1, 9), len(pts), 0), pts=pts,
+                                 bv=np.repeat([[Ss, Ds, Ts]], len(pts), 0),
+                                 ml=np.repeat([[mu, lam]], len(pts), 0), stress=st, strain=sn,
+                                 group=np.full(len(pts), ti)))
+    out = {k: np.concatenate([r[k] for r in recs], axis=0) for k in recs[0]}
+    mesh = planar_fault(length=6e3, depth=3e3, nx=6, nz=3, seed=2)
+    K, eleVec, xg, tri = ref_block(lib, mesh, 32e9, 32e9, halfspace=True)
+    out.update(blk_nodes=mesh.nodelst, blk_elelst=mesh.elelst, blk_K=K)
+    return out
+
+
 def main():
     lib = load_ref()
+    which = sys.argv[1:] or ["g1", "g2", "g3", "g4"]
     rng = np.random.default_rng(20240601)
-    np.savez_compressed(HERE / "g1_points.npz", **make_g1(lib, rng))
-    np.savez_compressed(HERE / "g2_sources.npz", **make_g2(lib, rng))
-    np.savez_compressed(HERE / "g3_blocks.npz", **make_g3(lib))
+    g1 = make_g1(lib, rng)        # g1 and g2 draw from one generator, in this order
+    g2 = make_g2(lib, rng)
+    if "g1" in which:
+        np.savez_compressed(HERE / "g1_points.npz", **g1)
+    if "g2" in which:
+        np.savez_compressed(HERE / "g2_sources.npz", **g2)
+    if "g3" in which:
+        np.savez_compressed(HERE / "g3_blocks.npz", **make_g3(lib))
+    if "g4" in which:
+        np.savez_compressed(HERE / "g4_halfspace.npz", **make_g4(lib))
     for f in sorted(HERE.glob("g*.npz")):
         print(f.name, f.stat().st_size)
 

@@ -150,3 +150,71 @@ def test_ensemble_reproduces_single_runs_bitwise(gpu, pq3d_lib):
         assert ens.time[k] == s.time
         assert np.array_equal(ens.instance("slipv", k), s.to_original(s.slipv))
         assert np.array_equal(ens.instance("state", k), s.to_original(s.state))
+
+
+def test_device_controller_matches_host_controller(gpu, pq3d_lib, small):
+    """The accept/reject loop run on the device (no host synchronisation)
+    takes the steps of the reference loop run on the host over the same
+    attempt kernel. The only difference is the rounding of pow (device
+    library vs host libm), which the controller's err^-0.2 amplifies like any
+    rounding difference: step sizes agree to 1e-7 over the first 80 steps,
+    simulated time to 1e-5 after 300."""
+    from pyquake3d_amd.qdsim import DeviceEnsemble
+    p, _ = small
+    op = DenseOperator(p.mesh, p.mu, p.lambda_, device=gpu)
+    sim = QDsim(p, op, device=gpu)
+    ens = DeviceEnsemble([p], op, device=gpu, rounds=8)
+    dt = 1e-3
+    t_prev = 0.0
+    for i in range(300):
+        h, dt = sim.simu_forward(dt)
+        while True:
+            ens.step()
+            ens.sync()
+            if ens.naccepted[0] == i + 1:
+                break
+        hd = ens.time[0] - t_prev
+        t_prev = ens.time[0]
+        if i < 80:
+            assert hd == pytest.approx(h, rel=1e-7), i
+    assert ens.time[0] == pytest.approx(sim.time, rel=1e-5)
+
+
+def test_device_ensemble_is_batch_and_round_independent(gpu, pq3d_lib):
+    """Bitwise: four friction draws batched (one 4-RHS H-matvec per step)
+    equal four single runs, and carrying an unfinished step over to the next
+    iteration (rounds=1) equals finishing it within one (rounds=4)."""
+    from dataclasses import replace
+
+    from pyquake3d_amd.hmatrix import HMatrix
+    from pyquake3d_amd.qdsim import DeviceEnsemble
+
+    base = planar_c1(nx=16, nz=8, seed=0)
+    rng = np.random.default_rng(6)
+    probs = [replace(base, a=base.a * (1 + 0.05 * rng.standard_normal(base.n)),
+                     dc=base.dc * rng.uniform(0.8, 1.2)) for _ in range(4)]
+    H = HMatrix.from_mesh(base.mesh, base.mu, base.lambda_, leaf=16, eta=2.0, eps=1e-6,
+                          dtype=torch.float32, device=gpu)
+    nacc = 200
+    batch = DeviceEnsemble(probs, H, device=gpu, rounds=3)
+    # step the batch until every draw has exactly nacc accepted steps: stop a
+    # draw by comparing per-draw snapshots taken when it reaches nacc
+    snaps = {}
+    while len(snaps) < 4:
+        batch.step()
+        batch.sync()
+        for k in range(4):
+            if k not in snaps and batch.naccepted[k] == nacc:
+                snaps[k] = (batch.time[k], batch.instance("slipv", k), batch.instance("state", k))
+    for k, p in enumerate(probs):
+        for rounds in (1, 4):
+            one = DeviceEnsemble([p], H, device=gpu, rounds=rounds)
+            while True:
+                one.step()
+                one.sync()
+                if one.naccepted[0] == nacc:
+                    break
+            t, v, st = snaps[k]
+            assert one.time[0] == t, (k, rounds)
+            assert np.array_equal(one.instance("slipv", 0), v), (k, rounds)
+            assert np.array_equal(one.instance("state", 0), st), (k, rounds)

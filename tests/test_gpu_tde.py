@@ -58,12 +58,56 @@ def test_each_source_matches_reference_golden(gpu, pq3d_lib, golden):
             assert rel_err(e, g["strain"][r, b]).max() < TOL
 
 
-def test_empty_inputs_and_halfspace_flag(gpu, pq3d_lib):
+def test_empty_inputs(gpu, pq3d_lib):
     s, e = tde.TDstressFS_C([], [], [], [0, 0, 0], [1, 0, 0], [0, 0, 1], 1, 0, 0, MU, LAM)
     assert s.shape == (0, 6)
-    with pytest.raises(RuntimeError, match="half-space"):
-        tde.TDstressFS_C([1.0], [2.0], [-3.0], [0, 0, -1], [1, 0, -1], [0, 0, -2], 1, 0, 0, MU,
-                         LAM, jud_halfspace=True)
+    s, e = tde.TDstressFS_C([], [], [], [0, 0, 0], [1, 0, 0], [0, 0, 1], 1, 0, 0, MU, LAM,
+                            jud_halfspace=True)
+    assert s.shape == (0, 6)
+
+
+def test_halfspace_matches_reference_golden(gpu, pq3d_lib, golden):
+    """Half space through the reference symbol with jud_halfspace = true:
+    1e-10 per point against the reference C++ (forward-mode correction
+    strain: agreement to rounding, not bit for bit)."""
+    g = golden["g4_halfspace"]
+    tri, pts, bv, ml = g["tri"], g["pts"], g["bv"], g["ml"]
+    key = np.concatenate([tri, bv, ml], axis=1)
+    starts = np.r_[0, np.where((np.diff(key, axis=0) != 0).any(1))[0] + 1, len(key)]
+    for a, b in zip(starts[:-1], starts[1:]):
+        s, e = tde.TDstressFS_C(pts[a:b, 0], pts[a:b, 1], pts[a:b, 2], tri[a, 0:3], tri[a, 3:6],
+                                tri[a, 6:9], *bv[a], *ml[a], jud_halfspace=True)
+        assert rel_err(s, g["stress"][a:b]).max() < TOL, int(g["group"][a])
+        assert rel_err(e, g["strain"][a:b]).max() < TOL, int(g["group"][a])
+
+
+def test_halfspace_each_source_equals_points_form(gpu, pq3d_lib, golden):
+    """One receiver x many triangles (symbol 2) equals many receivers x one
+    triangle (symbol 1) in the half space, and matches the golden values."""
+    g = golden["g4_halfspace"]
+    sel = np.where((g["bv"] == [1.0, 0.0, 0.0]).all(1) & (g["ml"][:, 0] == MU))[0]
+    # the first receiver of every triangle group
+    first = sel[np.r_[0, np.where(np.diff(g["group"][sel]) != 0)[0] + 1]]
+    x, y, z = g["pts"][first[0]]
+    tris = g["tri"][first]
+    s, _ = tde.TDstressEachSourceAtReceiver_C(x, y, z, tris[:, 0:3], tris[:, 3:6], tris[:, 6:9],
+                                              1.0, 0.0, 0.0, MU, LAM, jud_halfspace=True)
+    for k, t in enumerate(tris):
+        sp, _ = tde.TDstressFS_C([x], [y], [z], t[0:3], t[3:6], t[6:9], 1.0, 0.0, 0.0, MU, LAM,
+                                 jud_halfspace=True)
+        assert np.array_equal(s[k], sp[0])
+    assert rel_err(s[:1], g["stress"][first[:1]]).max() < TOL
+
+
+def test_halfspace_traction_block_matches_reference_golden(gpu, pq3d_lib, golden):
+    g = golden["g4_halfspace"]
+    dm = tde.DeviceMesh(g["blk_nodes"], g["blk_elelst"], MU, LAM, device=gpu, halfspace=True)
+    K = g["blk_K"]
+    out = dm.traction_block().cpu().numpy()
+    scale = np.abs(K).max(axis=(0, 2))[None, :, None]
+    assert (np.abs(out - K) / scale).max() < TOL
+    out32 = dm.traction_block(dtype=torch.float32).cpu().numpy()
+    assert (np.abs(out32 - K) / scale).max() < 1e-6
 
 
 @pytest.mark.parametrize("tag", ["planar", "undul"])

@@ -55,9 +55,14 @@ class Checker:
         assert self.T(rec).numel() // LC["rec"] == N
         return N
 
-    def pq3d_hmat_dense_assemble(self, tab, nb, recv, rec, mu, lam, pool, esize, stream):
+    def _hrecs(self, hrec, N):
+        if hrec is not None and hrec != 0:
+            assert self.T(hrec).numel() == LC["hrec"] * N
+
+    def pq3d_hmat_dense_assemble(self, tab, nb, recv, rec, hrec, mu, lam, pool, esize, stream):
         tab = rows(self.T(tab), "dense_block")
         N = self._recs(recv, rec)
+        self._hrecs(hrec, N)
         assert len(tab) >= nb
         P = self.T(pool).numel()
         for b in range(nb):
@@ -65,11 +70,12 @@ class Checker:
             assert 0 <= r0 and r0 + m <= N and 0 <= c0 and c0 + n <= N and m > 0 and n > 0
             assert base >= 0 and base + 6 * m * n <= P
 
-    def pq3d_hmat_aca(self, tab, wsoff, nb, recv, rec, mu, lam, eps, kmax, wU, wV, flags, rank,
-                      stream):
+    def pq3d_hmat_aca(self, tab, wsoff, nb, recv, rec, hrec, mu, lam, eps, kmax, wU, wV, flags,
+                      rank, stream):
         tab_a = rows(self.T(tab), "lowrank_block")
         ws = rows(self.T(wsoff), "aca_ws")
         N = self._recs(recv, rec)
+        self._hrecs(hrec, N)
         nU, nV, nF = self.T(wU).numel(), self.T(wV).numel(), self.T(flags).numel()
         assert self.T(rank).numel() >= nb
         r = np.empty(nb, dtype=np.int64)
@@ -133,9 +139,11 @@ class Checker:
                 assert xo + cnt <= (nt if kind == 1 else N)
 
 
-@pytest.mark.parametrize("leaf,eta,dtype", [(32, 2.0, torch.float32), (64, 2.0, torch.float64),
-                                            (32, 0.0, torch.float64)])
-def test_hmatrix_device_accesses_in_bounds(monkeypatch, leaf, eta, dtype):
+@pytest.mark.parametrize("leaf,eta,dtype,hs", [(32, 2.0, torch.float32, False),
+                                               (64, 2.0, torch.float64, False),
+                                               (32, 0.0, torch.float64, False),
+                                               (32, 2.0, torch.float32, True)])
+def test_hmatrix_device_accesses_in_bounds(monkeypatch, leaf, eta, dtype, hs):
     global LC
     LC = _lib.layout_constants()
     chk = Checker(np.random.default_rng(leaf))
@@ -144,7 +152,7 @@ def test_hmatrix_device_accesses_in_bounds(monkeypatch, leaf, eta, dtype):
     monkeypatch.setattr(_lib, "stream_handle", lambda s=None: None)
     mesh = undulating_fault(length=40e3, depth=12e3, nx=44, nz=18, seed=4)
     H = hm.HMatrix.from_mesh(mesh, 32e9, 32e9, leaf=leaf, eta=eta, eps=1e-4, dtype=dtype,
-                             device="cpu", aca_budget_gb=0.002)
+                             device="cpu", aca_budget_gb=0.002, halfspace=hs)
     x = torch.zeros(mesh.n, dtype=torch.float64)
     H.matvec(x, x)
     x8 = torch.zeros(mesh.n * 8, dtype=torch.float64)
@@ -179,7 +187,7 @@ def test_tde_and_blocktree_call_argument_kinds(monkeypatch):
             assert out_len == out.numel()
             assert 5 * plane + (m - 1) * si + (n - 1) * sj < out_len
         if name == "pq3d_hmat_aca":
-            reg[a[12].value][:] = 3
+            reg[a[13].value][:] = 3
 
     monkeypatch.setattr(_lib, "call", call)
     monkeypatch.setattr(_lib, "ptr", ptr)

@@ -97,6 +97,59 @@ class StepChecker:
             self.T(dst)[:n] = self.T(src)[:n]
 
 
+class DeviceStepChecker(StepChecker):
+    """Adds the device-driven controller entry points: bounds of every
+    per-cell and per-instance argument, and the decision rule of
+    k_rk45_control restated in numpy over scripted error maxima."""
+
+    def pq3d_rk45_rounds(self, n, inst, h, flag, nrj, dtn, status, rounds, tol, Tno, Tt1o,
+                         Tt2o, st, a, b, dc, f0, A1, A2, ds, sl, V, cp, y0, y1, y2, y3, err, a63,
+                         ninst, stream):
+        self._cells(n, Tno, Tt1o, Tt2o, st, a, b, dc, f0, A1, A2, ds, sl, V, y0, y1, y2, y3)
+        ii = self.T(inst)
+        assert ii.numel() >= n and int(ii.max()) < ninst and int(ii.min()) >= 0
+        for q in (h, flag, nrj, dtn, status):
+            assert self.T(q).numel() >= ninst
+        assert self.T(cp).numel() >= 3 * ninst and self.T(err).numel() >= 4 * ninst
+        assert rounds >= 1 and tol > 0
+        H, F, R, D, S = (self.T(q) for q in (h, flag, nrj, dtn, status))
+        for _ in range(rounds):
+            for g in range(ninst):
+                if F[g]:
+                    continue
+                e = self._errs[g][min(int(R[g]), len(self._errs[g]) - 1)]
+                R1 = e[0] + 1e-10
+                R2 = max(e[1], e[2], e[3]) + 1e-10
+                c1, c2 = R1 / tol, R2 / tol
+                hh = float(H[g])
+                hn1 = hh * 0.9 * (tol / R1) ** 0.2
+                hn2 = hh * 0.9 * (tol / R2) ** 0.2
+                if max(c1, c2) < 1.0 and not (np.isnan(c1) or np.isnan(c2)):
+                    D[g] = min(min(hn1, hn2), 1.5 * hh)
+                    F[g] = 1
+                    continue
+                R[g] += 1
+                H[g] = max(0.5 * hh, min(hn1, hn2))
+                if H[g] < 1e-15 or R[g] > 20:
+                    F[g] = 2
+                    S[g] = 1
+                self.attempts += 1
+
+    def pq3d_rk45_advance(self, n, inst, h, flag, nrj, dtn, t, nacc, a, cp, y0, y1, y2, y3, Tno,
+                          Tt1o, Tt2o, st, s1, s2, V, rake, sl1, sl2, vmax, ninst, stream):
+        self._cells(n, a, y0, y1, y2, y3, Tno, Tt1o, Tt2o, st, s1, s2, V, rake, sl1, sl2)
+        for q in (h, flag, nrj, dtn, t, nacc, vmax):
+            assert self.T(q).numel() >= ninst
+        H, F, R, D, Tt, N = (self.T(q) for q in (h, flag, nrj, dtn, t, nacc))
+        for g in range(ninst):
+            if F[g] == 1:
+                Tt[g] += H[g]
+                H[g] = D[g]
+                R[g] = 0
+                N[g] += 1
+                F[g] = 0
+
+
 def test_stepper_argument_lists_and_controller(monkeypatch, oracle):
     from oracle.qdsim_oracle import dense_operators
     from pyquake3d_amd.mesh import get_eleVec
@@ -145,3 +198,43 @@ def test_ensemble_argument_lists(monkeypatch, oracle):
     h = ens.step()      # zero errors -> every instance accepts at once
     assert np.all(h == 1e-3) and ens.nattempts == 1
     assert ens.max_slipv().shape == (4,)
+
+
+def test_device_ensemble_argument_lists(monkeypatch, oracle):
+    """DeviceEnsemble drives the device controller with valid arguments; an
+    instance that needs more attempts than one call's rounds carries over
+    to the next iteration and accepts the same step."""
+    from oracle.qdsim_oracle import dense_operators
+    from pyquake3d_amd.mesh import get_eleVec
+    from pyquake3d_amd.qdsim import DeviceEnsemble, StepAborted
+    p = planar_c1(nx=6, nz=3)
+    ev, xg = get_eleVec(p.mesh.nodelst, p.mesh.elelst)
+    K = dense_operators(oracle, xg, ev, p.mesh.triangles(), p.mu, p.lambda_)
+    ora = OracleSim(K, p.a, p.b, p.dc, p.f0, p.V0, p.mu, p.Cs, p.Vpl, p.rake, p.initial_state())
+    chk = DeviceStepChecker(ora)
+    chk.attempts = 0
+    monkeypatch.setattr(_lib, "call", chk.call)
+    monkeypatch.setattr(_lib, "ptr", chk.ptr)
+    monkeypatch.setattr(_lib, "stream_handle", lambda s=None: None)
+    ok = [1e-6, 1e-7, 2e-7, 1e-8]
+    bad = [1e-2, 1e-9, 1e-9, 1e-9]
+    # instance 0 accepts at once; instance 1 is rejected three times first
+    chk._errs = [[ok], [bad, bad, bad, ok]]
+    ens = DeviceEnsemble([p, p], IdentityOp(p.n, K), device="cpu", rounds=2)
+    ens.step()
+    ens.sync()
+    assert list(ens.naccepted) == [1, 0]          # instance 1 carried over
+    ens.step()
+    ens.sync()
+    assert list(ens.naccepted) == [2, 1]
+    h1 = 1e-3
+    for _ in range(3):
+        r1 = 1e-2 + 1e-10
+        h1 = max(0.5 * h1, min(h1 * 0.9 * (1e-4 / r1) ** 0.2, h1 * 0.9 * (1e-4 / 1.1e-9) ** 0.2))
+    assert ens.time[1] == pytest.approx(h1)
+    # more than 20 rejections abort
+    chk._errs = [[ok], [bad]]
+    with pytest.raises(StepAborted):
+        for _ in range(20):
+            ens.step()
+        ens.sync()
