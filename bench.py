@@ -15,6 +15,8 @@ Workloads
                         same mesh stepped together, one multi-RHS H-matvec
                         per step; rate = sum over the draws
             ``value`` is the batched variant; all three are reported.
+  matvec    H-matvec only, --nb right-hand sides, on the hstep operator
+            (--dtype f32|f64); the command the hardware-counter profiles use.
   assembly  Kernel assembly only: all N x N pairs at N = 20,000 (FP64, six
             traction operators, 19.2 GB of output in HBM). One step = one full
             assembly.
@@ -37,6 +39,12 @@ sys.path.insert(0, str(REPO))
 MI355X_HBM_GBS = 8000.0
 MI355X_FP64_TFLOPS = 78.6
 YEAR = 3.15576e7
+
+# HBM bytes per H-matvec (all three phases) at N = 100,352 from two
+# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of the matvec workload,
+# FETCH scaled by 2.0 and WRITE by 1.0: the factors the same counters read
+# for a 2 GiB device-to-device copy (profiles/hmatvec_traffic_N100352.md).
+MEASURED_TRAFFIC = {}
 
 # FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
 # (unit strike and dip slip, both projected), counted operator by operator
@@ -195,6 +203,81 @@ def bench_hstep(args):
     return res
 
 
+# ------------------------------------------------------------- matvec
+def bench_matvec(args):
+    """One H-matvec of --nb right-hand sides per step on the hstep operator,
+    timed with HIP events on the launch stream; reports the effective HBM
+    bandwidth over the algorithmic bytes (stored operator once + five FP64
+    vectors) and the per-phase split used to read hardware counters."""
+    import torch
+
+    from pyquake3d_amd.problem import undulating_c3
+
+    # a plain 2 GiB device copy (read 2 GiB, write 2 GiB): the known-size
+    # reference for the hardware byte counters and the attainable bandwidth
+    src = torch.ones(2 ** 28, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dst.copy_(src)
+    c0.record()
+    for _ in range(3):
+        dst.copy_(src)
+    c1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 8 * 3 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+
+    problem = undulating_c3()
+    dtype = torch.float32 if args.dtype == "f32" else torch.float64
+    H, build_s = build_hmatrix(problem, dtype)
+    n, nb, dev = H.n, args.nb, H.device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    v1 = torch.randn(n * nb, dtype=torch.float64, device=dev, generator=gen)
+    v2 = torch.randn(n * nb, dtype=torch.float64, device=dev, generator=gen)
+    out = torch.empty((3, n * nb), dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        H.matvec(v1, v2, out=out, nrhs=nb)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    w0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record()
+        H.matvec(v1, v2, out=out, nrhs=nb)
+        e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / len(evs)
+    B = H.blocks
+    lr, k = B["lr_idx"], B["ranks"]
+    v_bytes = int((2 * B["n"][lr] * k).sum()) * H.esize
+    u_bytes = int((3 * B["m"][lr] * k).sum()) * H.esize
+    d_bytes = int(H.stats.dense_entries) * H.esize
+    vec_bytes = 5 * 8 * n * nb
+    algo = H.matvec_bytes + 5 * 8 * n * (nb - 1)
+    achieved = algo / (ms * 1e-3) / 1e9
+    return {
+        "metric": "H-matvec effective HBM bandwidth",
+        "value": achieved, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": f"{args.dtype} operator storage, f64 vectors and accumulation",
+        "data": "synthetic undulating fault mesh, seeded random slip-rate vectors",
+        "config": {"workload": f"H-matvec, undulating fault 100x25 km, N={n}, leaf 64, eta 2, "
+                               f"ACA eps 1e-4, {nb} right-hand sides"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((args.dtype, nb)),
+                     "kernel_ms": ms, "algorithmic_bytes": algo},
+        "bytes": {"phase1_lowrank_V": v_bytes, "phase2_lowrank_U": u_bytes,
+                  "phase2_dense": d_bytes, "vectors": vec_bytes, "total": algo},
+        "copy_2GiB_GBs": copy_gbs,
+        "build_s": build_s,
+    }
+
+
 # ------------------------------------------------------------- assembly
 def bench_assembly(args):
     import numpy as np
@@ -271,7 +354,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="hstep", choices=["hstep", "assembly"])
+    ap.add_argument("--workload", default="hstep", choices=["hstep", "assembly", "matvec"])
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"],
+                    help="operator storage of the matvec workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nb", type=int, default=16, choices=[1, 2, 4, 8, 16],
                     help="friction draws in the batched variant")
@@ -284,6 +369,10 @@ def main():
         args.steps = 3000 if args.steps is None else args.steps
         args.warmup = 200 if args.warmup is None else args.warmup
         res = bench_hstep(args)
+    elif args.workload == "matvec":
+        args.steps = 20 if args.steps is None else args.steps
+        args.warmup = 3 if args.warmup is None else args.warmup
+        res = bench_matvec(args)
     else:
         args.steps = 3 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup

@@ -384,3 +384,75 @@ class HMatrix:
     @classmethod
     def from_mesh(cls, mesh: FaultMesh, mu, lambda_, **kw):
         return cls(mesh.nodelst, mesh.elelst, mu, lambda_, **kw)
+
+
+class MultiHMatrix:
+    """Independent H-matrices (one per mesh, same storage type) applied as one
+    block-diagonal operator: every matvec phase is one launch over all of
+    them. Instance b's cells occupy [off[b], off[b] + n_b) of the
+    concatenated vectors, in that mesh's cluster order (``perms[b]``).
+    Each instance's rows see exactly its own work items in its own order,
+    so its product is bitwise that of its own HMatrix."""
+
+    ragged = True
+
+    def __init__(self, hms):
+        if not hms:
+            raise ValueError("need at least one H-matrix")
+        if len({h.dtype for h in hms}) != 1 or len({str(h.device) for h in hms}) != 1:
+            raise ValueError("all H-matrices must share storage type and device")
+        self.parts = list(hms)
+        self.device = hms[0].device
+        self.dtype = hms[0].dtype
+        self.esize = hms[0].esize
+        self.sizes = np.array([h.n for h in hms], dtype=np.int64)
+        self.off = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.n = int(self.off[-1])
+        self.perms = [h.perm for h in hms]
+        self.perm = np.concatenate([h.perm + o for h, o in zip(hms, self.off[:-1])])
+        pool_off = np.cumsum([0] + [int(h.pool.numel()) for h in hms])
+        t_off = np.cumsum([0] + [int(h.nt) for h in hms])
+        p_off = np.cumsum([0] + [int(h.npart) for h in hms])
+        units, reds, tiles, items, ptr = [], [], [], [], [np.zeros(1, dtype=np.int64)]
+        nitems = 0
+        for b, h in enumerate(hms):
+            U = h.units.cpu().numpy().copy()
+            if len(U):
+                U[:, 0] += pool_off[b]                                  # V^T base
+                U[:, 5] += self.off[b]                                  # block c0
+                U[:, 6] += np.where(U[:, 7] != 0, t_off[b], p_off[b])   # output row
+            units.append(U.reshape(-1, 8))
+            R = h.reduce.cpu().numpy().copy()
+            if len(R):
+                R[:, 0] += t_off[b]
+                R[:, 2] += p_off[b]
+            reds.append(R.reshape(-1, 4))
+            T = h.tiles.cpu().numpy().copy()
+            T[:, 0] += self.off[b]
+            tiles.append(T)
+            I = h.items.cpu().numpy().copy()
+            if len(I):
+                I[:, 1] += pool_off[b]
+                I[:, 5] += np.where(I[:, 0] == 1, t_off[b], self.off[b])
+            items.append(I.reshape(-1, 6))
+            tp = h.tile_ptr.cpu().numpy()
+            ptr.append(tp[1:] + nitems)
+            nitems += int(tp[-1])
+        dev = self.device
+        cat = lambda xs, w: torch.from_numpy(
+            np.ascontiguousarray(np.concatenate(xs, 0).reshape(-1, w).astype(np.int64))).to(dev)
+        self.pool = torch.cat([h.pool for h in hms])
+        self.units, self.nunits = cat(units, 8), int(sum(len(u) for u in units))
+        self.reduce, self.nred = cat(reds, 4), int(sum(len(r) for r in reds))
+        self.tiles, self.ntiles = cat(tiles, 2), int(sum(len(t) for t in tiles))
+        self.items = cat(items, 6)
+        self.tile_ptr = torch.from_numpy(np.concatenate(ptr).astype(np.int64)).to(dev)
+        self.nt, self.npart = int(t_off[-1]), int(p_off[-1])
+        self._scratch = {}
+        for name, tab in (("item", self.items), ("unit", self.units), ("reduce", self.reduce),
+                          ("tile", self.tiles)):
+            _lib.check_width(name, tab)
+        self.matvec_bytes = int(self.pool.numel()) * self.esize + 5 * 8 * self.n
+
+    _bufs = HMatrix._bufs
+    matvec = HMatrix.matvec

@@ -153,6 +153,7 @@ class FeatureEnsemble(EnsembleQDsim):
         self.record = False
         self.nfirst_rejected = 0                      # steps whose first attempt was rejected
         self.nfallback = 0                            # PI fallback attempts
+        self.max_accepted_ratio = 0.0                 # largest max(c1, c2) ever accepted
 
     def features(self) -> np.ndarray:
         _lib.call("pq3d_features", self.ne, self.nb, _lib.ptr(self.slipv), _lib.ptr(self.state),
@@ -192,6 +193,7 @@ class FeatureEnsemble(EnsembleQDsim):
                 if max(c1, c2) < 1.0 and not (np.isnan(c1) or np.isnan(c2)):
                     dtnext[b] = min(min(hn1, hn2), 1.5 * h[b])
                     ratio[b] = max(c1, c2)
+                    self.max_accepted_ratio = max(self.max_accepted_ratio, ratio[b])
                     done[b] = True
                     continue
                 nrj[b] += 1

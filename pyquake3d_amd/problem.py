@@ -157,3 +157,37 @@ def undulating_c3(nx=448, nz=112, seed=0, nucleation=True, mu=32e9, nu=0.25, Vs=
 
 def with_friction(p: Problem, **kw) -> Problem:
     return replace(p, **kw)
+
+
+def ensemble_c4(count=64, seed=0, n_min=2000, n_max=20000, mu=32e9, nu=0.25, Vs=3464.0,
+                sigma=25.0, Vpl=1e-9) -> list:
+    """Independent simulations on meshes of uneven size: N log-uniform in
+    [n_min, n_max], a planar or an undulating 40 x 20 km fault by coin flip
+    (2:1 rectangles split in two), and a friction draw each: a uniform in
+    [0.004, 0.02] inside a 20 x 10 km velocity-weakening patch (2 km taper to
+    0.04 outside), Dc uniform in [0.05, 0.3] m, b = 0.03, with a seeded
+    nucleation disc at 100 Vpl."""
+    rng = np.random.default_rng(seed)
+    lam = 2 * mu * nu / (1 - 2 * nu)
+    out = []
+    for k in range(count):
+        n_target = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        nz = max(2, int(round(np.sqrt(n_target / 4.0))))
+        nx = 2 * nz
+        if rng.random() < 0.5:
+            mesh = planar_fault(length=40e3, depth=20e3, nx=nx, nz=nz, seed=seed + k)
+        else:
+            mesh = undulating_fault(length=40e3, depth=20e3, nx=nx, nz=nz, amplitude=300.0,
+                                    wavelength=5e3, seed=seed + k)
+        _, xg = get_eleVec(mesh.nodelst, mesh.elelst)
+        n = mesh.n
+        d = np.maximum(np.abs(xg[:, 0]) - 10e3, np.abs(xg[:, 2] + 10e3) - 5e3)
+        a_vw = rng.uniform(0.004, 0.02)
+        a = a_vw + (0.04 - a_vw) * _taper(d, 2e3)
+        dc = np.full(n, rng.uniform(0.05, 0.3))
+        cx, cz = rng.uniform(-5e3, 5e3), rng.uniform(-12e3, -8e3)
+        V = np.where((xg[:, 0] - cx) ** 2 + (xg[:, 2] - cz) ** 2 < (2.5e3) ** 2, 100 * Vpl, Vpl)
+        out.append(Problem(mesh, mu, lam, mu / Vs ** 2, 1e-6, a, np.full(n, 0.03), dc,
+                           np.full(n, 0.6), np.full(n, sigma), V, np.zeros(n),
+                           np.full(n, Vpl), name=f"c4_{k}_N{n}"))
+    return out

@@ -276,16 +276,27 @@ class EnsembleQDsim:
 
     def __init__(self, problems, op, reltol: float = 1e-4, device="cuda", tableau: str = "ref"):
         nb = len(problems)
-        if nb not in (1, 2, 4, 8, 16):
-            raise ValueError("ensemble size must be 1, 2, 4, 8 or 16")
-        n = problems[0].n
-        for p in problems:
-            if p.n != n:
-                raise ValueError("all instances must share the mesh of the operator")
+        # a MultiHMatrix (meshes of different size): instance-major layout,
+        # one block-diagonal product per step; otherwise one shared mesh,
+        # interleaved layout and an NB-RHS product
+        self.ragged = bool(getattr(op, "ragged", False))
+        if self.ragged:
+            sizes = np.asarray(op.sizes)
+            if len(sizes) != nb or any(p.n != s for p, s in zip(problems, sizes)):
+                raise ValueError("problem b must have the mesh of the operator's part b")
+            n, ne, self.nrhs = None, int(op.n), 1
+        else:
+            if nb not in (1, 2, 4, 8, 16):
+                raise ValueError("ensemble size must be 1, 2, 4, 8 or 16")
+            n = problems[0].n
+            for p in problems:
+                if p.n != n:
+                    raise ValueError("all instances must share the mesh of the operator")
+            ne, self.nrhs = n * nb, nb
         if tableau not in ("ref", "dp5"):
             raise ValueError("tableau must be 'ref' or 'dp5'")
         self.a63 = -46732. / 5247. if tableau == "ref" else 46732. / 5247.
-        self.nb, self.n, self.ne = nb, n, n * nb
+        self.nb, self.n, self.ne = nb, n, ne
         self.op = op
         self.reltol = float(reltol)
         dev = torch.device(device)
@@ -294,6 +305,11 @@ class EnsembleQDsim:
         self.perm = perm
 
         def inter(arrs):
+            if self.ragged:
+                # instance b's cells, in its mesh's cluster order, one after another
+                A = np.concatenate([np.asarray(x, dtype=np.float64)[pb]
+                                    for x, pb in zip(arrs, op.perms)])
+                return torch.from_numpy(np.ascontiguousarray(A)).to(dev)
             # (nb, n) original order -> (n*nb,) cluster order, cell-major
             A = np.stack([np.asarray(x, dtype=np.float64)[perm] for x in arrs], axis=1)
             return torch.from_numpy(np.ascontiguousarray(A.reshape(-1))).to(dev)
@@ -315,7 +331,11 @@ class EnsembleQDsim:
         if len({bool(p.fix_Tn) for p in problems}) != 1:
             raise ValueError("fix_Tn must be the same for all instances")
         self.fix_Tn = bool(problems[0].fix_Tn)
-        self.inst = torch.from_numpy(np.tile(np.arange(nb, dtype=np.int32), n)).to(dev)
+        if self.ragged:
+            inst = np.repeat(np.arange(nb, dtype=np.int32), op.sizes)
+        else:
+            inst = np.tile(np.arange(nb, dtype=np.int32), n)
+        self.inst = torch.from_numpy(inst).to(dev)
         self.cellpar = torch.tensor([[p.V0, p.mu, p.Cs] for p in problems],
                                     dtype=torch.float64, device=dev).reshape(-1)
         self.y = torch.empty((4, self.ne), dtype=torch.float64, device=dev)
@@ -346,11 +366,11 @@ class EnsembleQDsim:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nb)
+            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs)
             e1.record()
             self.matvec_events.append((e0, e1))
         else:
-            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nb)
+            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs)
         _lib.call("pq3d_add_loading", self.ne, _lib.ptr(self.K[2]),
                   _lib.ptr(self.normal_loading), int(self.fix_Tn), s)
 
@@ -424,6 +444,12 @@ class EnsembleQDsim:
 
     def instance(self, name: str, b: int) -> np.ndarray:
         """Field ``name`` of instance b in original element order."""
+        if self.ragged:
+            o, n = int(self.op.off[b]), int(self.op.sizes[b])
+            x = getattr(self, name)[o:o + n].cpu().numpy()
+            out = np.empty(n)
+            out[self.op.perms[b]] = x
+            return out
         x = getattr(self, name).reshape(self.n, self.nb)[:, b].cpu().numpy()
         out = np.empty(self.n)
         out[self.perm] = x

@@ -146,3 +146,28 @@ def test_halfspace_hmatrix_matches_dense_halfspace_operators(gpu, pq3d_lib):
     # the free surface matters: the full-space operators differ
     K_fs = tde.DeviceMesh.from_mesh(mesh, MU, LAM, device=gpu).traction_block().cpu().numpy()
     assert np.abs(K_fs - K).max() > 1e-3 * np.abs(K).max()
+
+
+def test_full_size_fp32_hmatrix_sampled_rows(gpu, pq3d_lib):
+    """N = 100,352 undulating fault, leaf 64, eta 2, ACA eps 1e-4, FP32
+    storage: 256 sampled rows of the product against the FP64 dense rows
+    (evaluated on the device, all 100,352 columns) for a random and a smooth
+    deficit-like vector: max error <= 1e-3 of the largest value."""
+    from pyquake3d_amd import tde
+    from pyquake3d_amd.problem import undulating_c3
+    p = undulating_c3()
+    H = HMatrix.from_mesh(p.mesh, MU, LAM, leaf=64, eta=2.0, eps=1e-4, dtype=torch.float32,
+                          device=gpu)
+    dm = tde.DeviceMesh.from_mesh(p.mesh, MU, LAM, device=gpu)
+    n = H.n
+    rng = np.random.default_rng(11)
+    rows = np.sort(rng.choice(n, 256, replace=False))          # original element order
+    Kr = dm.traction_block(rows, None)                          # (6, 256, n) FP64
+    xg = get_eleVec(p.mesh.nodelst, p.mesh.elelst)[1]
+    smooth = np.exp(-((xg[:, 0] / 20e3) ** 2 + ((xg[:, 2] + 10e3) / 6e3) ** 2)) * 1e-6
+    for v1, v2 in ((rng.standard_normal(n), rng.standard_normal(n)), (smooth, 0.2 * smooth)):
+        x1 = torch.from_numpy(v1).to(gpu)
+        x2 = torch.from_numpy(v2).to(gpu)
+        ref = torch.stack([Kr[c] @ x1 + Kr[3 + c] @ x2 for c in range(3)]).cpu().numpy()
+        y = _apply(H, v1, v2)[:, rows]
+        assert _relerr(y, ref) < 1e-3

@@ -103,3 +103,36 @@ def test_features_match_numpy(gpu, pq3d_lib):
     lvt = np.log10(V / V0) + (psi - f0) / (b * np.log(10.0))
     assert np.abs(mean - lvt.mean(0)).max() <= 1e-8
     assert np.array_equal(frac, (V > 1e-3).sum(0) / n)
+
+
+def test_learned_step_size_loop(gpu, pq3d_lib):
+    """Learned first-attempt step size on the 144-cell problem: trained with
+    the kernels on (features, log10 h_accepted/dtnext) pairs logged from the
+    reference controller, it cuts the rejected attempts per accepted step;
+    the embedded error test still decides every step (no accepted step has
+    max(c1, c2) >= 1) and the PI fallback handles the rejections. The
+    operator product is frozen per step, so a rejected attempt costs one
+    cell-local RK45 attempt, not an operator product: this is a gain in
+    attempts, not in operator products per simulated year."""
+    from pyquake3d_amd.learned_dt import FeatureEnsemble, StepMLP
+    from pyquake3d_amd.problem import planar_c1
+    from pyquake3d_amd.qdsim import DenseOperator
+    p = planar_c1(nx=12, nz=6, seed=0)
+    op = DenseOperator(p.mesh, p.mu, p.lambda_, device=gpu)
+    rec = FeatureEnsemble([p], op, device=gpu)
+    rec.record = True
+    for _ in range(1500):
+        rec.step()
+    plain_aps = rec.nattempts / rec.nsteps
+    X, T = rec.dataset()
+    assert (T <= 0).all() and (T < 0).any()
+    mlp = StepMLP(device=gpu, seed=0)
+    hist = mlp.fit(X, T, epochs=60, batch=1024, lr=2e-3)
+    assert hist[-1] < hist[0]
+    le = FeatureEnsemble([p], op, mlp=mlp, device=gpu)
+    for _ in range(1500):
+        le.step()
+    learned_aps = le.nattempts / le.nsteps
+    assert learned_aps < plain_aps, (learned_aps, plain_aps)
+    assert le.max_accepted_ratio < 1.0
+    assert le.nfallback > 0

@@ -218,3 +218,70 @@ def test_device_ensemble_is_batch_and_round_independent(gpu, pq3d_lib):
             assert one.time[0] == t, (k, rounds)
             assert np.array_equal(one.instance("slipv", 0), v), (k, rounds)
             assert np.array_equal(one.instance("state", 0), st), (k, rounds)
+
+
+def test_config1_dense_fp64_rtol_1e8(gpu, pq3d_lib, oracle):
+    """The plain method at its reference size: planar 40 x 20 km fault, 1,980
+    triangles, dense FP64 operators, rtol 1e-8, 1,000 RK45 steps, against the
+    CPU oracle of the reference step. Step sizes agree to 1e-7 over the first
+    100 steps (rounding only: device exp, GEMV order), the simulated time
+    after 1,000 steps to 1e-4."""
+    p = planar_c1()
+    assert p.n == 1980
+    ev, xg = get_eleVec(p.mesh.nodelst, p.mesh.elelst)
+    K = dense_operators(oracle, xg, ev, p.mesh.triangles(), p.mu, p.lambda_)
+    op = DenseOperator(p.mesh, p.mu, p.lambda_, device=gpu)
+    sim = QDsim(p, op, device=gpu, reltol=1e-8)
+    ora = OracleSim(K, p.a, p.b, p.dc, p.f0, p.V0, p.mu, p.Cs, p.Vpl, p.rake, p.initial_state(),
+                    reltol=1e-8)
+    dt_g = dt_o = 1e-3
+    for i in range(1000):
+        hg, dt_g = sim.simu_forward(dt_g)
+        ho, dt_o = ora.simu_forward(dt_o)
+        if i < 100:
+            assert hg == pytest.approx(ho, rel=1e-7), i
+    assert sim.time == pytest.approx(ora.time, rel=1e-4)
+
+
+def test_ragged_ensemble_matches_solo_runs(gpu, pq3d_lib):
+    """Eight simulations on meshes of uneven size (N from 100 to 600, planar
+    or undulating, own friction draws) stepped as one block-diagonal
+    operator (every matvec phase one launch for all) with the device
+    controller. Each instance equals its own FP32 solo run bit for bit, and
+    stays within 1e-3 in simulated time of its FP64 solo run after 150
+    accepted steps."""
+    from pyquake3d_amd.hmatrix import HMatrix, MultiHMatrix
+    from pyquake3d_amd.problem import ensemble_c4
+    from pyquake3d_amd.qdsim import DeviceEnsemble
+
+    probs = ensemble_c4(8, seed=1, n_min=100, n_max=600)
+    sizes = [p.n for p in probs]
+    assert len(set(sizes)) > 3
+    H32 = [HMatrix.from_mesh(p.mesh, p.mu, p.lambda_, leaf=16, eta=2.0, eps=1e-6,
+                             dtype=torch.float32, device=gpu) for p in probs]
+    M = MultiHMatrix(H32)
+    nacc = 150
+    ens = DeviceEnsemble(probs, M, device=gpu, rounds=4)
+    snaps = {}
+    while len(snaps) < len(probs):
+        ens.step()
+        ens.sync()
+        for k in range(len(probs)):
+            if k not in snaps and ens.naccepted[k] == nacc:
+                snaps[k] = (ens.time[k], ens.instance("slipv", k))
+
+    def solo(p, H):
+        one = DeviceEnsemble([p], H, device=gpu, rounds=4)
+        while True:
+            one.step()
+            one.sync()
+            if one.naccepted[0] == nacc:
+                return one.time[0], one.instance("slipv", 0)
+
+    for k, (p, H) in enumerate(zip(probs, H32)):
+        t, v = solo(p, H)
+        assert t == snaps[k][0] and np.array_equal(v, snaps[k][1]), k
+        H64 = HMatrix.from_mesh(p.mesh, p.mu, p.lambda_, leaf=16, eta=2.0, eps=1e-6,
+                                dtype=torch.float64, device=gpu)
+        t64, _ = solo(p, H64)
+        assert abs(snaps[k][0] - t64) <= 1e-3 * t64, (k, snaps[k][0], t64)

@@ -5,6 +5,7 @@ a pool filled from the dense FP64 oracle (near field) and truncated SVDs of the
 stacked 3m x 2n blocks (far field), and compared with the dense product."""
 import numpy as np
 import pytest
+import torch
 
 from pyquake3d_amd.cluster import build_block_partition, build_cluster_tree
 from pyquake3d_amd.hmatrix import build_worklists, plan_layout
@@ -125,3 +126,63 @@ def test_layout_and_worklists_reproduce_dense_product(oracle, leaf):
         rb = np.stack([K[0] @ V1[:, b] + K[3] @ V2[:, b], K[1] @ V1[:, b] + K[4] @ V2[:, b],
                        K[2] @ V1[:, b] + K[5] @ V2[:, b]])
         assert np.abs(y4[:, :, b] - rb).max() <= 1e-8 * scale
+
+
+def test_block_diagonal_union_reproduces_each_product(oracle):
+    """MultiHMatrix-style merge of the work lists of two meshes (pool, t and
+    partial offsets, column and row offsets) reproduces each mesh's dense
+    product on its own slice of the concatenated vectors."""
+    import types
+    from pyquake3d_amd import hmatrix as hm
+    parts = []
+    for nx, nz, seed in ((20, 8, 1), (14, 6, 2)):
+        mesh = undulating_fault(length=20e3, depth=8e3, nx=nx, nz=nz, seed=seed)
+        ev, xg = get_eleVec(mesh.nodelst, mesh.elelst)
+        n = mesh.n
+        t = build_cluster_tree(xg, leaf=16)
+        P = build_block_partition(t, eta=2.0)
+        K = oracle.traction_block(xg, ev, mesh.triangles(), t.perm, t.perm, MU, LAM)
+        r0, m = t.start[P.rows], t.size[P.rows]
+        c0, nn = t.start[P.cols], t.size[P.cols]
+        lr_idx = np.where(P.adm)[0]
+        ranks, fac = [], {}
+        for bi in lr_idx:
+            a, b = slice(r0[bi], r0[bi] + m[bi]), slice(c0[bi], c0[bi] + nn[bi])
+            S = np.block([[K[0][a, b], K[3][a, b]], [K[1][a, b], K[4][a, b]],
+                          [K[2][a, b], K[5][a, b]]])
+            U, s, Vt = np.linalg.svd(S, full_matrices=False)
+            k = int((s > 1e-9 * s[0]).sum())
+            ranks.append(k)
+            fac[bi] = (U[:, :k] * s[:k], Vt[:k])
+        L = plan_layout(r0, m, c0, nn, P.adm, lr_idx, np.array(ranks))
+        pool = np.zeros(L["total"])
+        for bi, base in zip(L["dense_idx"], L["d_base"]):
+            bm, bn = m[bi], nn[bi]
+            for op in range(6):
+                blk = K[op][r0[bi]:r0[bi] + bm, c0[bi]:c0[bi] + bn]
+                pool[base + op * bm * bn: base + (op + 1) * bm * bn] = blk.T.ravel()
+        for bi, ub, vb, k in zip(L["lr_idx"], L["l_ub"], L["l_vb"], L["k"]):
+            U, V = fac[bi]
+            pool[ub:ub + U.size] = U.T.ravel()
+            pool[vb:vb + V.size] = V.T.ravel()
+        W = build_worklists(t, n, r0, m, c0, nn, L["dense_idx"], L["d_base"], L["lr_idx"],
+                            L["l_ub"], L["l_vb"], L["k"])
+        h = types.SimpleNamespace(
+            n=n, perm=t.perm, dtype=torch.float64, device=torch.device("cpu"), esize=8,
+            pool=torch.from_numpy(pool), units=torch.from_numpy(W["units"]),
+            reduce=torch.from_numpy(W["reduce"]), tiles=torch.from_numpy(W["tiles"]),
+            items=torch.from_numpy(W["items"]), tile_ptr=torch.from_numpy(W["tile_ptr"]),
+            nt=W["nt"], npart=W["npart"])
+        parts.append((h, K))
+    Mu = hm.MultiHMatrix([h for h, _ in parts])
+    Wm = dict(units=Mu.units.numpy(), reduce=Mu.reduce.numpy(), tiles=Mu.tiles.numpy(),
+              items=Mu.items.numpy(), tile_ptr=Mu.tile_ptr.numpy(), nt=Mu.nt, npart=Mu.npart)
+    rng = np.random.default_rng(3)
+    v1, v2 = rng.standard_normal(Mu.n), rng.standard_normal(Mu.n)
+    y = emulate_matvec(Wm, Mu.pool.numpy(), v1, v2, Mu.n)
+    for b, (h, K) in enumerate(parts):
+        o, n = int(Mu.off[b]), h.n
+        x1, x2 = v1[o:o + n], v2[o:o + n]
+        ref = np.stack([K[0] @ x1 + K[3] @ x2, K[1] @ x1 + K[4] @ x2, K[2] @ x1 + K[5] @ x2])
+        scale = np.abs(K).max() * np.abs(np.concatenate([x1, x2])).sum()
+        assert np.abs(y[:, o:o + n] - ref).max() <= 1e-8 * scale
