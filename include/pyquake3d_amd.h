@@ -160,26 +160,34 @@ int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
                   double lambda_, double eps, int kmax, double* d_wU, double* d_wV,
                   uint8_t* d_flags, int64_t* d_rank, void* stream);
 
-/* Copy converged factors into the pool: U (3m x k, column-major) at
- * d_tab[b][4], V transposed (2n x k, row-major) at d_tab[b][5],
- * k = d_tab[b][6]. */
+/* Copy converged factors into the pool (the storage half of ACA_worker,
+ * src/Hmatrix.py:1014-1066): U (3m x k) at d_tab[b][4] in one panel per leaf
+ * cluster of the block's rows -- the leaf with row offset ro and ls rows in
+ * the block holds element (o*m + i, l) at 3*k*ro + l*3*ls + o*ls + (i - ro);
+ * V transposed (2n x k, row-major) at d_tab[b][5]; k = d_tab[b][6].
+ * d_tiles: leaf clusters as (first row, rows) int64 pairs; d_tile_of_row:
+ * leaf index of every cluster-order row. */
 int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
-                       const double* d_wU, const double* d_wV, void* d_pool, int dtype_bytes,
-                       void* stream);
+                       const int64_t* d_tiles, const int64_t* d_tile_of_row, const double* d_wU,
+                       const double* d_wV, void* d_pool, int dtype_bytes, void* stream);
 
 /* Six-operator H-matvec for nrhs right-hand sides (replaces the six
  * blocks_process_MVM calls of one step, src/Hmatrix.py:1445-1538 via
  * src/QDsim.py:906-917): y1 = A1s v1 + A1d v2, y2 = A2s v1 + A2d v2,
- * y3 = Bs v1 + Bd v2, FP64 vectors interleaved by instance (element (row, b)
- * at row*nrhs + b), FP64 accumulation; nrhs in {1, 2, 4, 8, 16}. Low-rank V
- * factors are stored transposed (2n x k). Work lists (phase-1 units, phase-1b
- * reductions, tiles, items) as built by pyquake3d_amd.hmatrix.build_worklists;
- * d_t: nt*nrhs and d_part: npart*nrhs doubles of scratch. */
+ * y3 = Bs v1 + Bd v2, FP64 inputs interleaved by instance (element (row, b)
+ * at row*nrhs + b), FP64 accumulation; nrhs in {1, 2, 4, 8, 16}. Outputs in
+ * the same layout (y_bstride = 0) or instance-major, element (row, b) at
+ * b*y_bstride + row (y_bstride > 0, normally the row count). Low-rank V
+ * factors are stored transposed (2n x k), U in leaf panels. Work lists
+ * (phase-1 units, phase-1b reductions, tiles, items) as built by
+ * pyquake3d_amd.hmatrix.build_worklists; d_t: nt*nrhs and d_part:
+ * npart*nrhs doubles of scratch. */
 int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_red, int64_t nred,
                      const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
                      const int64_t* d_items, const void* d_pool, int dtype_bytes, int nrhs,
                      const double* d_v1, const double* d_v2, double* d_t, double* d_part,
-                     double* d_y1, double* d_y2, double* d_y3, void* stream);
+                     double* d_y1, double* d_y2, double* d_y3, int64_t y_bstride,
+                     void* stream);
 
 /* ---------------------------------------------------------- time step
  * Cells may belong to n_inst independent instances (d_inst[i], or NULL for
@@ -191,6 +199,13 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_re
 int pq3d_deficit(int64_t n, const double* d_slipv1, const double* d_slipv2,
                  const double* d_pl1, const double* d_pl2, double* d_x1, double* d_x2,
                  void* stream);
+
+/* The same for nb instances of n cells stored instance-major (cell c of
+ * instance b at b*n + c), written interleaved for the multi-RHS H-matvec:
+ * x[c*nb + b] = V[b*n + c] - Vpl[b*n + c]. */
+int pq3d_deficit_interleave(int64_t n, int64_t nb, const double* d_slipv1,
+                            const double* d_slipv2, const double* d_pl1, const double* d_pl2,
+                            double* d_x1, double* d_x2, void* stream);
 
 /* dsigma/dt = K part + normal loading (src/QDsim.py:911-913); fix_tn: loading only. */
 int pq3d_add_loading(int64_t n, double* d_dsig, const double* d_load, int fix_tn,

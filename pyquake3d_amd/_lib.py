@@ -64,10 +64,11 @@ SIGNATURES = {
                               c_double, c_double, c_double, c_int, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p]),
     "pq3d_hmat_aca_pack": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
-                                   c_int, c_void_p]),
+                                   c_void_p, c_void_p, c_int, c_void_p]),
     "pq3d_hmat_matvec": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p,
                                  c_int64, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
-                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                                 c_void_p]),
     # time step
     "pq3d_rk45_attempt": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -85,6 +86,8 @@ SIGNATURES = {
                                   c_void_p, c_void_p] + [c_void_p] * 17 + [c_int64, c_void_p]),
     "pq3d_deficit": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p]),
+    "pq3d_deficit_interleave": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p, c_void_p]),
     "pq3d_qd_rhs": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_double, c_double, c_double, c_void_p, c_void_p]),

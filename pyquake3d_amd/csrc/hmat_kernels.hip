@@ -274,24 +274,35 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
   if (tid == 0) out_rank[bid] = converged ? rank : -1;
 }
 
-// Copy the converged factors into the storage pool (type T).
+// Copy the converged factors into the storage pool (type T). U goes into one
+// panel per leaf cluster of the block's rows: the panel of leaf (ro, ls) (row
+// offset in the block, rows) starts at 3*k*ro and holds element (o*m + i, l)
+// at l*3*ls + o*ls + (i - ro), so each row tile of the product reads one
+// contiguous range. V is stored transposed (2n x k, row-major).
 template <typename T>
 __global__ void __launch_bounds__(256) k_aca_pack(const int64_t* __restrict__ tab,
                                                   const int64_t* __restrict__ wsoff,
+                                                  const int64_t* __restrict__ tiles,
+                                                  const int64_t* __restrict__ tile_of_row,
                                                   const double* __restrict__ wU,
                                                   const double* __restrict__ wV,
                                                   T* __restrict__ pool) {
   const int64_t bid = blockIdx.x;
   const int64_t* b = tab + bid * LB_W;
-  const int64_t m = b[LB_M], n = b[LB_N], k = b[LB_K];
+  const int64_t r0 = b[LB_R0], m = b[LB_M], n = b[LB_N], k = b[LB_K];
   if (k <= 0) return;
   const double* U = wU + wsoff[WS_W * bid + WS_U];
   const double* V = wV + wsoff[WS_W * bid + WS_V];
   T* pu = pool + b[LB_UB];
   T* pv = pool + b[LB_VB];
-  const int64_t nu = 3 * m * k, nvv = 2 * n * k;
-  for (int64_t p = threadIdx.x; p < nu; p += blockDim.x) pu[p] = (T)U[p];
-  // V is stored transposed (2n x k, row-major): element (c, l) at c*k + l
+  const int64_t m3 = 3 * m, nu = m3 * k, nvv = 2 * n * k;
+  // workspace U: column l contiguous, element (r, l) at l*3m + r, r = o*m + i
+  for (int64_t p = threadIdx.x; p < nu; p += blockDim.x) {
+    const int64_t l = p / m3, r = p - l * m3, o = r / m, i = r - o * m;
+    const int64_t* tl = tiles + TL_W * tile_of_row[r0 + i];
+    const int64_t ro = tl[0] - r0, ls = tl[1];
+    pu[3 * k * ro + l * 3 * ls + o * ls + (i - ro)] = (T)U[p];
+  }
   const int64_t C = 2 * n;
   for (int64_t p = threadIdx.x; p < nvv; p += blockDim.x) {
     const int64_t c = p / k, l = p - c * k;
@@ -314,6 +325,17 @@ __device__ __forceinline__ int wave_id() {
   return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
+// MI355X hands consecutive workgroups of a launch to its 8 XCDs in turn, and
+// each XCD has its own L2. Work index w = xcd_order(blockIdx.x) gives every
+// XCD a contiguous range of work indices instead, so neighbouring work items
+// (units reading the same source columns, row tiles of the same blocks) share
+// one L2. A bijection of [0, g) for any grid size g.
+__device__ __forceinline__ int64_t xcd_order(int64_t b, int64_t g) {
+  constexpr int64_t X = 8;
+  const int64_t q = g / X, r = g - q * X, x = b % X, i = b / X;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // Column loops are unrolled so that a group's loads are all issued before its
 // multiply-adds (several loads in flight per wave). A larger NB means more
 // arithmetic and more live registers per loaded value, hence less unrolling.
@@ -331,6 +353,27 @@ __device__ __forceinline__ int wave_id() {
 #ifndef PQ3D_P2_U16
 #define PQ3D_P2_U16 2
 #endif
+#ifndef PQ3D_NT_LOADS
+#define PQ3D_NT_LOADS 1
+#endif
+// phase 2 software-pipelines its column loop for NB up to this value (with
+// more right-hand sides the extra registers cost more occupancy than the
+// pipelining gains)
+#ifndef PQ3D_P2_PF_NB
+#define PQ3D_P2_PF_NB 1
+#endif
+
+// Operator loads: every stored element is used once per product, so these
+// may bypass cache retention (leaving L2 to the vectors, which are re-read).
+template <typename T>
+__device__ __forceinline__ T ld_op(const T* p) {
+#if PQ3D_NT_LOADS
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
 template <int NB>
 constexpr int p1_unroll() {
   return NB <= 2 ? 8 : NB <= 4 ? 4 : NB <= 8 ? PQ3D_P1_U8 : PQ3D_P1_U16;
@@ -338,6 +381,10 @@ constexpr int p1_unroll() {
 template <int NB>
 constexpr int p2_unroll() {
   return NB <= 2 ? 8 : NB <= 4 ? 4 : NB <= 8 ? PQ3D_P2_U8 : PQ3D_P2_U16;
+}
+template <int NB>
+constexpr bool p2_prefetch() {
+  return NB <= PQ3D_P2_PF_NB;
 }
 
 // Phase 1: t[l][b] = sum over the unit's stacked columns c of V^T[c][l] x_b[c]
@@ -352,7 +399,7 @@ __device__ __forceinline__ void p1_columns(const T* __restrict__ vc, int lc, int
   for (; q + U <= len; q += U) {
     T v[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = (vc + (q + u) * k)[lc];
+    for (int u = 0; u < U; ++u) v[u] = ld_op(vc + (q + u) * k + lc);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const double vu = (double)v[u];
@@ -361,7 +408,7 @@ __device__ __forceinline__ void p1_columns(const T* __restrict__ vc, int lc, int
     }
   }
   for (; q < len; ++q) {
-    const double vu = (double)(vc + q * k)[lc];
+    const double vu = (double)ld_op(vc + q * k + lc);
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = fma(vu, xs[q * NB + b], acc[b]);
   }
@@ -374,7 +421,7 @@ __global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ 
                                                     const double* __restrict__ v2,
                                                     double* __restrict__ t,
                                                     double* __restrict__ part) {
-  const int64_t u = (int64_t)blockIdx.x * (blockDim.x / 64) + wave_id();
+  const int64_t u = xcd_order(blockIdx.x, gridDim.x) * (blockDim.x / 64) + wave_id();
   const int lane = threadIdx.x & 63;
   if (u >= nunits) return;
   const int64_t* it = units + u * UN_W;
@@ -433,21 +480,49 @@ __global__ void __launch_bounds__(256) k_hmv_phase1b(const int64_t* __restrict__
 
 constexpr int kTileWaves = 4;
 
+// Phase-2 helpers: load U columns (three planes) of an item starting at
+// column c, and multiply-add them into the 3*NB accumulators.
+template <typename T, int U>
+__device__ __forceinline__ void p2_load(const T* p, int64_t ld, int64_t plane, int row,
+                                        int64_t c, T (&a)[U][3]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int o = 0; o < 3; ++o) a[u][o] = ld_op(p + (c + u) * ld + o * plane + row);
+}
+
+template <typename T, int NB, int U>
+__device__ __forceinline__ void p2_madd(const T (&a)[U][3], const double* __restrict__ x0,
+                                        double (&acc)[3 * NB]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const double a0 = (double)a[u][0], a1 = (double)a[u][1], a2 = (double)a[u][2];
+    const double* x = x0 + u * NB;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      acc[3 * b + 0] = fma(a0, x[b], acc[3 * b + 0]);
+      acc[3 * b + 1] = fma(a1, x[b], acc[3 * b + 1]);
+      acc[3 * b + 2] = fma(a2, x[b], acc[3 * b + 2]);
+    }
+  }
+}
+
 // Phase 2: one workgroup per row tile (a leaf cluster of <= 64 rows); lane <->
 // row. Every item is three row planes (outputs y1, y2, y3) of cnt columns
 // times weights w[xoff + j] (kind 0: v1, dense strike operators; kind 2: v2,
 // dense dip operators; kind 1: t, low-rank U). Waves split the tile's items;
 // partial sums meet in LDS in wave order. y1 = AdotV1, y2 = AdotV2,
-// y3 = dsigma (K part only), interleaved.
+// y3 = dsigma (K part only): element (row r, rhs b) at r*NB + b, or at
+// b*ybs + r when ybs > 0 (instance-major).
 template <typename T, int NB>
 __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
     const int64_t* __restrict__ tiles, const int64_t* __restrict__ tile_ptr,
     const int64_t* __restrict__ items, const T* __restrict__ pool,
     const double* __restrict__ v1, const double* __restrict__ v2, const double* __restrict__ t,
-    double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3) {
+    double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3, int64_t ybs) {
   constexpr int U = p2_unroll<NB>();
   __shared__ double red[3 * NB][64];
-  const int64_t tile = blockIdx.x;
+  const int64_t tile = xcd_order(blockIdx.x, gridDim.x);
   const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
   const int w = wave_id();
   const int lane = threadIdx.x & 63;
@@ -465,28 +540,36 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
     const T* p = pool + base;  // wave-uniform; lanes add their row
     const double* xs = (kind == 0 ? v1 : kind == 1 ? t : v2) + xo * NB;
     int64_t c = 0;
-    for (; c + U <= cnt; c += U) {
-      T a[U][3];
+    if constexpr (p2_prefetch<NB>()) {
+      // software pipeline: the next group's loads are in flight while the
+      // current group is multiplied
+      if (U <= cnt) {
+        T a[U][3];
+        p2_load<T, U>(p, ld, plane, row, c, a);
+        for (;;) {
+          const bool more = c + 2 * U <= cnt;
+          T an[U][3];
+          if (more) p2_load<T, U>(p, ld, plane, row, c + U, an);
+          p2_madd<T, NB, U>(a, xs + c * NB, acc);
+          c += U;
+          if (!more) break;
 #pragma unroll
-      for (int u = 0; u < U; ++u)
+          for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int o = 0; o < 3; ++o) a[u][o] = (p + (c + u) * ld + o * plane)[row];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const double a0 = (double)a[u][0], a1 = (double)a[u][1], a2 = (double)a[u][2];
-        const double* x = xs + (c + u) * NB;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          acc[3 * b + 0] = fma(a0, x[b], acc[3 * b + 0]);
-          acc[3 * b + 1] = fma(a1, x[b], acc[3 * b + 1]);
-          acc[3 * b + 2] = fma(a2, x[b], acc[3 * b + 2]);
+            for (int o = 0; o < 3; ++o) a[u][o] = an[u][o];
         }
+      }
+    } else {
+      for (; c + U <= cnt; c += U) {
+        T a[U][3];
+        p2_load<T, U>(p, ld, plane, row, c, a);
+        p2_madd<T, NB, U>(a, xs + c * NB, acc);
       }
     }
     for (; c < cnt; ++c) {
-      const double a0 = (double)(p + c * ld)[row];
-      const double a1 = (double)(p + c * ld + plane)[row];
-      const double a2 = (double)(p + c * ld + 2 * plane)[row];
+      const double a0 = (double)ld_op(p + c * ld + row);
+      const double a1 = (double)ld_op(p + c * ld + plane + row);
+      const double a2 = (double)ld_op(p + c * ld + 2 * plane + row);
       const double* x = xs + c * NB;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -506,11 +589,12 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
   }
   if (w == 0 && live) {
     const int64_t r = R0 + lane;
+    const int64_t sr = ybs > 0 ? 1 : NB, sb = ybs > 0 ? ybs : 1;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      y1[r * NB + b] = red[3 * b + 0][lane];
-      y2[r * NB + b] = red[3 * b + 1][lane];
-      y3[r * NB + b] = red[3 * b + 2][lane];
+      y1[r * sr + b * sb] = red[3 * b + 0][lane];
+      y2[r * sr + b * sb] = red[3 * b + 1][lane];
+      y3[r * sr + b * sb] = red[3 * b + 2][lane];
     }
   }
 }
@@ -520,7 +604,7 @@ static void launch_matvec(const int64_t* units, int64_t nunits, const int64_t* r
                           int64_t nred, const int64_t* tiles, const int64_t* tile_ptr,
                           int64_t ntiles, const int64_t* items, const T* pool, const double* v1,
                           const double* v2, double* t, double* part, double* y1, double* y2,
-                          double* y3, hipStream_t s) {
+                          double* y3, int64_t ybs, hipStream_t s) {
   if (nunits > 0)
     k_hmv_phase1<T, NB><<<(unsigned)((nunits + 3) / 4), 256, 0, s>>>(units, nunits, pool, v1, v2,
                                                                       t, part);
@@ -528,7 +612,8 @@ static void launch_matvec(const int64_t* units, int64_t nunits, const int64_t* r
     k_hmv_phase1b<NB><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(red, nred, part, t);
   if (ntiles > 0)
     k_hmv_phase2<T, NB><<<(unsigned)ntiles, 64 * kTileWaves, 0, s>>>(tiles, tile_ptr, items, pool,
-                                                                     v1, v2, t, y1, y2, y3);
+                                                                     v1, v2, t, y1, y2, y3,
+                                                                     ybs);
 }
 
 }  // namespace pq3d
@@ -606,32 +691,40 @@ int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
 }
 
 int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
-                       const double* d_wU, const double* d_wV, void* d_pool, int dtype_bytes,
-                       void* stream) {
+                       const int64_t* d_tiles, const int64_t* d_tile_of_row, const double* d_wU,
+                       const double* d_wV, void* d_pool, int dtype_bytes, void* stream) {
   clear_error();
+  PQ3D_REQUIRE(nblocks >= 0, "nblocks must be >= 0");
   if (nblocks == 0) return PQ3D_OK;
-  PQ3D_REQUIRE(d_tab && d_wsoff && d_wU && d_wV && d_pool, "null pointer");
+  PQ3D_REQUIRE(d_tab && d_wsoff && d_tiles && d_tile_of_row && d_wU && d_wV && d_pool,
+               "null pointer");
   PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
+  PQ3D_REQUIRE(nblocks < (1LL << 31), "too many blocks");
+  hipStream_t s = as_stream(stream);
   if (dtype_bytes == 4)
-    k_aca_pack<float><<<(unsigned)nblocks, 256, 0, as_stream(stream)>>>(d_tab, d_wsoff, d_wU,
-                                                                         d_wV, (float*)d_pool);
+    k_aca_pack<float><<<(unsigned)nblocks, 256, 0, s>>>(d_tab, d_wsoff, d_tiles, d_tile_of_row,
+                                                         d_wU, d_wV, (float*)d_pool);
   else
-    k_aca_pack<double><<<(unsigned)nblocks, 256, 0, as_stream(stream)>>>(d_tab, d_wsoff, d_wU,
-                                                                          d_wV, (double*)d_pool);
+    k_aca_pack<double><<<(unsigned)nblocks, 256, 0, s>>>(d_tab, d_wsoff, d_tiles, d_tile_of_row,
+                                                          d_wU, d_wV, (double*)d_pool);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }
 
 // y1 = AdotV1, y2 = AdotV2, y3 = dsigma (K part) for nrhs interleaved
-// right-hand sides (all FP64, cluster order, element (row, b) at row*nrhs + b).
+// right-hand sides (all FP64, cluster order, element (row, b) of v1, v2 at
+// row*nrhs + b). Outputs: the same layout with y_bstride = 0, or element
+// (row, b) at b*y_bstride + row (instance-major) with y_bstride > 0.
 // d_t: nt*nrhs doubles, d_part: npart*nrhs doubles of scratch.
 int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_red, int64_t nred,
                      const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
                      const int64_t* d_items, const void* d_pool, int dtype_bytes, int nrhs,
                      const double* d_v1, const double* d_v2, double* d_t, double* d_part,
-                     double* d_y1, double* d_y2, double* d_y3, void* stream) {
+                     double* d_y1, double* d_y2, double* d_y3, int64_t y_bstride,
+                     void* stream) {
   clear_error();
   PQ3D_REQUIRE(ntiles >= 0 && nunits >= 0 && nred >= 0, "sizes must be >= 0");
+  PQ3D_REQUIRE(y_bstride >= 0, "y_bstride must be >= 0");
   PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
   PQ3D_REQUIRE(d_tiles && d_tile_ptr && d_items && d_pool && d_v1 && d_v2 && d_y1 && d_y2 &&
                    d_y3,
@@ -646,11 +739,11 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_re
     if (dtype_bytes == 4)                                                                   \
       launch_matvec<float, NBV>(d_units, nunits, d_red, nred, d_tiles, d_tile_ptr, ntiles,  \
                                 d_items, (const float*)d_pool, d_v1, d_v2, d_t, d_part,     \
-                                d_y1, d_y2, d_y3, s);                                       \
+                                d_y1, d_y2, d_y3, y_bstride, s);                            \
     else                                                                                    \
       launch_matvec<double, NBV>(d_units, nunits, d_red, nred, d_tiles, d_tile_ptr, ntiles, \
                                  d_items, (const double*)d_pool, d_v1, d_v2, d_t, d_part,   \
-                                 d_y1, d_y2, d_y3, s);                                      \
+                                 d_y1, d_y2, d_y3, y_bstride, s);                           \
   } while (0)
   switch (nrhs) {
     case 1: PQ3D_MV(1); break;

@@ -269,6 +269,19 @@ __global__ void __launch_bounds__(256) k_deficit(int64_t n, const double* __rest
   x2[i] = slipv2[i] - pl2[i];
 }
 
+// Instance-major slip rates -> interleaved deficit (thread o = c*nb + b; the
+// writes are contiguous, the nb reads of a wave fall in nb cache lines).
+__global__ void __launch_bounds__(256) k_deficit_interleave(
+    int64_t n, int64_t nb, const double* __restrict__ slipv1, const double* __restrict__ slipv2,
+    const double* __restrict__ pl1, const double* __restrict__ pl2, double* __restrict__ x1,
+    double* __restrict__ x2) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n * nb) return;
+  const int64_t c = o / nb, b = o - c * nb, i = b * n + c;
+  x1[o] = slipv1[i] - pl1[i];
+  x2[o] = slipv2[i] - pl2[i];
+}
+
 // dsigma/dt = K part + normal loading (src/QDsim.py:911-913), or the loading
 // alone when the normal traction is fixed.
 __global__ void __launch_bounds__(256) k_add_loading(int64_t n, double* __restrict__ dsig,
@@ -315,7 +328,7 @@ __device__ inline double py_min(double a, double b) { return b < a ? b : a; }
 // h = max(0.5 h, min(hn1, hn2)), and flag 2 with status 1 when h < 1e-15 or
 // after more than 20 rejections. hn_i = 0.9 h (tol / R_i)^0.2,
 // R1 = err_psi + 1e-10, R2 = max(err_tau1, err_tau2, err_sigma) + 1e-10.
-__global__ void k_rk45_control(int64_t n_inst, const unsigned long long* __restrict__ err,
+__global__ void k_rk45_control(int64_t n_inst, unsigned long long* __restrict__ err,
                                double tol, double* __restrict__ hcur, uint8_t* __restrict__ flag,
                                int32_t* __restrict__ nrj, double* __restrict__ dtnext,
                                int32_t* __restrict__ status) {
@@ -325,6 +338,9 @@ __global__ void k_rk45_control(int64_t n_inst, const unsigned long long* __restr
   const double e1 = __longlong_as_double((long long)err[4 * b + 1]);
   const double e2 = __longlong_as_double((long long)err[4 * b + 2]);
   const double e3 = __longlong_as_double((long long)err[4 * b + 3]);
+  // cleared for the next round's maxima (only attempting instances add to them)
+#pragma unroll
+  for (int q = 0; q < 4; ++q) err[4 * b + q] = 0ull;
   const double R1 = e0 + 1e-10;
   const double R2 = py_max(py_max(e1, e2), e3) + 1e-10;
   const double c1 = R1 / tol, c2 = R2 / tol;
@@ -427,6 +443,19 @@ int pq3d_deficit(int64_t n, const double* d_slipv1, const double* d_slipv2,
   return PQ3D_OK;
 }
 
+int pq3d_deficit_interleave(int64_t n, int64_t nb, const double* d_slipv1,
+                            const double* d_slipv2, const double* d_pl1, const double* d_pl2,
+                            double* d_x1, double* d_x2, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0 && nb >= 1, "bad sizes");
+  if (n == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_slipv1 && d_slipv2 && d_pl1 && d_pl2 && d_x1 && d_x2, "null pointer");
+  k_deficit_interleave<<<blocks_for(n * nb, 256), 256, 0, as_stream(stream)>>>(
+      n, nb, d_slipv1, d_slipv2, d_pl1, d_pl2, d_x1, d_x2);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
 int pq3d_qd_rhs(int64_t n, const double* d_Tno, const double* d_Tt1o, const double* d_Tt2o,
                 const double* d_state, const double* d_a, const double* d_b, const double* d_dc,
                 const double* d_f0, const double* d_AdotV1, const double* d_AdotV2,
@@ -473,8 +502,9 @@ int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_f
   PQ3D_REQUIRE(tol > 0.0, "tol must be > 0");
   if (n == 0) return PQ3D_OK;
   hipStream_t s = as_stream(stream);
+  // the control kernel clears the slots it reads, so one clear serves all rounds
+  k_zero_u64<<<blocks_for(4 * n_inst, 256), 256, 0, s>>>(d_err, 4 * n_inst);
   for (int r = 0; r < rounds; ++r) {
-    k_zero_u64<<<blocks_for(4 * n_inst, 256), 256, 0, s>>>(d_err, 4 * n_inst);
     k_rk45_attempt<<<blocks_for(n, 256), 256, 0, s>>>(
         n, d_inst, d_h, d_Tno, d_Tt1o, d_Tt2o, d_state, d_a, d_b, d_dc, d_f0, d_AdotV1,
         d_AdotV2, d_dsigmadt, d_shear_loading, d_slipv, d_cellpar, d_yTno, d_yTt1o, d_yTt2o,

@@ -83,6 +83,51 @@ def plan_layout(r0, m, c0, nn, adm, lr_idx, ranks):
                 total=total, u_size=u_size, v_size=v_size)
 
 
+def leaf_tables(tree):
+    """(tiles, tile_of_row): the leaf clusters as (first row, rows) in cluster
+    order, and the leaf index of every cluster-order row. Leaves tile the rows
+    without gaps, so every block's rows are a run of whole leaves."""
+    leaves = tree.leaves()
+    lstart = tree.start[leaves].astype(np.int64)
+    lsize = tree.size[leaves].astype(np.int64)
+    if lstart[0] != 0 or np.any(lstart[1:] != lstart[:-1] + lsize[:-1]):
+        raise ValueError("leaf clusters must tile the rows in order")
+    tile_of_row = np.repeat(np.arange(len(leaves), dtype=np.int64), lsize)
+    return np.ascontiguousarray(np.stack([lstart, lsize], axis=1)), tile_of_row
+
+
+def leaf_segments(tiles, tile_of_row, r0, m):
+    """(row offset in the block, rows) of each leaf of block rows [r0, r0+m)."""
+    q0, q1 = int(tile_of_row[r0]), int(tile_of_row[r0 + m - 1])
+    seg = tiles[q0:q1 + 1].copy()
+    seg[:, 0] -= r0
+    if seg[0, 0] != 0 or int(seg[:, 1].sum()) != m:
+        raise ValueError("block rows are not a run of whole leaves")
+    return seg
+
+
+def pack_u(U, seg):
+    """U (3m x k, rows o*m + i) in pool order: one panel per leaf segment
+    (ro, ls) at offset 3*k*ro, element (o*m + i, l) at l*3*ls + o*ls + i - ro.
+    A row tile then reads one contiguous panel, sharing no cache line with the
+    tiles of the block's other leaves."""
+    m3, k = U.shape
+    m = m3 // 3
+    U3 = U.reshape(3, m, k)
+    out = np.empty(m3 * k, dtype=U.dtype)
+    for ro, ls in seg:
+        out[3 * k * ro:3 * k * (ro + ls)] = U3[:, ro:ro + ls, :].transpose(2, 0, 1).ravel()
+    return out
+
+
+def unpack_u(flat, m, k, seg):
+    """Inverse of ``pack_u``: (3m x k)."""
+    U = np.empty((3, m, k), dtype=flat.dtype)
+    for ro, ls in seg:
+        U[:, ro:ro + ls, :] = flat[3 * k * ro:3 * k * (ro + ls)].reshape(k, 3, ls).transpose(1, 2, 0)
+    return U.reshape(3 * m, k)
+
+
 def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_vb, k_keep):
     """Matvec work lists.
 
@@ -96,19 +141,17 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
     tile_ptr (T+1,): CSR offsets of each tile's items
     units   (Q, 8): phase-1 units (V^T base, k, first stacked column, columns,
             block n, block c0, output row, direct); V^T of a block is 2n x k
-            row-major at its V base
+            row-major at its V base, U is in leaf panels (``pack_u``). Units
+            are ordered by the source columns they read, so that units sharing
+            x run next to each other
     reduce  (R, 4): phase-1b sums (t row, k, first partial row, units)
     nt, npart: rows of t and of the partial buffer
     """
-    leaves = tree.leaves()
-    lstart = tree.start[leaves]
-    lsize = tree.size[leaves]
+    tiles, tile_of_row = leaf_tables(tree)
+    lstart, lsize = tiles[:, 0], tiles[:, 1]
     if np.any(lsize > 64):
         raise ValueError("leaf clusters larger than 64 rows are not supported by the tile kernel")
-    ntiles = len(leaves)
-    tile_of_row = np.empty(n, dtype=np.int64)
-    for q in range(ntiles):
-        tile_of_row[lstart[q]:lstart[q] + lsize[q]] = q
+    ntiles = len(tiles)
     items = []
     for bi, base in zip(np.asarray(dense_idx).tolist(), np.asarray(d_base).tolist()):
         br0, bm, bc0, bn = int(r0[bi]), int(m[bi]), int(c0[bi]), int(nn[bi])
@@ -135,18 +178,21 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
             reds.append((toff, k, poff, nu))
             poff += nu * k
         for q in range(tile_of_row[br0], tile_of_row[br0 + bm - 1] + 1):
-            ro = int(lstart[q]) - br0
+            ro, ls = int(lstart[q]) - br0, int(lsize[q])
+            pb = ub + 3 * k * ro   # this leaf's panel of U
             for l0 in range(0, k, LR_CHUNK):
                 cnt = min(LR_CHUNK, k - l0)
-                items.append((q, 1, ub + l0 * 3 * bm + ro, 3 * bm, bm, cnt, toff + l0))
+                items.append((q, 1, pb + l0 * 3 * ls, 3 * ls, ls, cnt, toff + l0))
         toff += k
+    un = np.array(units, dtype=np.int64).reshape(-1, 8)
+    un = un[np.lexsort((un[:, 2], un[:, 5]))]   # by block column start, then chunk
     it = np.array(items, dtype=np.int64).reshape(-1, 7)
     it = it[np.argsort(it[:, 0], kind="stable")]
     counts = np.bincount(it[:, 0], minlength=ntiles)
     tile_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     return dict(items=np.ascontiguousarray(it[:, 1:]), tile_ptr=tile_ptr,
-                tiles=np.ascontiguousarray(np.stack([lstart, lsize], axis=1).astype(np.int64)),
-                units=np.ascontiguousarray(np.array(units, dtype=np.int64).reshape(-1, 8)),
+                tiles=tiles,
+                units=np.ascontiguousarray(un),
                 reduce=np.ascontiguousarray(np.array(reds, dtype=np.int64).reshape(-1, 4)),
                 nt=toff, npart=poff)
 
@@ -172,6 +218,7 @@ class HMatrix:
         t0 = time.perf_counter()
         self.tree = build_cluster_tree(self.xg, leaf)
         self.part = build_block_partition(self.tree, eta)
+        self.leaf_tiles, self.tile_of_row = leaf_tables(self.tree)
         tm["tree"] = time.perf_counter() - t0
         self.perm = self.tree.perm
         self.iperm = np.empty_like(self.perm)
@@ -243,6 +290,9 @@ class HMatrix:
         # pack low-rank factors (per ACA batch)
         t0 = time.perf_counter()
         lr_global = lr_idx[keep]
+        tiles_d = torch.from_numpy(self.leaf_tiles).to(dev)
+        tor_d = torch.from_numpy(self.tile_of_row).to(dev)
+        _lib.check_width("tile", tiles_d)
         ub_of = dict(zip(lr_global.tolist(), l_ub.tolist()))
         vb_of = dict(zip(lr_global.tolist(), l_vb.tolist()))
         for sel, res in lr_results:
@@ -257,7 +307,8 @@ class HMatrix:
             _lib.check_width("lowrank_block", lt)
             _lib.check_width("aca_ws", wsoff)
             _lib.call("pq3d_hmat_aca_pack", _lib.ptr(lt), _lib.ptr(wsoff), len(sel),
-                      _lib.ptr(wsU), _lib.ptr(wsV), _lib.ptr(self.pool), self.esize, stream)
+                      _lib.ptr(tiles_d), _lib.ptr(tor_d), _lib.ptr(wsU), _lib.ptr(wsV),
+                      _lib.ptr(self.pool), self.esize, stream)
         _sync(dev)
         del lr_results
         if dev.type == "cuda":
@@ -339,10 +390,12 @@ class HMatrix:
         return self._scratch[nrhs]
 
     # ----------------------------------------------------------------- apply
-    def matvec(self, v1, v2, out=None, stream=None, nrhs=1):
+    def matvec(self, v1, v2, out=None, stream=None, nrhs=1, instance_major=False):
         """(AdotV1, AdotV2, dsigma_K) = six-operator product, cluster order,
-        FP64. With nrhs > 1, v1/v2/out rows are interleaved by instance
-        (element (cell, b) at cell*nrhs + b) and out has shape (3, n*nrhs)."""
+        FP64. With nrhs > 1, v1/v2 rows are interleaved by instance (element
+        (cell, b) at cell*nrhs + b) and out has shape (3, n*nrhs), in the
+        same layout or, with ``instance_major``, element (cell, b) at
+        b*n + cell."""
         n = self.n
         if nrhs not in NRHS:
             raise ValueError(f"nrhs must be one of {NRHS}")
@@ -361,7 +414,8 @@ class HMatrix:
                   self.nred, _lib.ptr(self.tiles), _lib.ptr(self.tile_ptr), self.ntiles,
                   _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
                   _lib.ptr(v2), _lib.ptr(t), _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]),
-                  _lib.ptr(out[2]), _lib.stream_handle(stream))
+                  _lib.ptr(out[2]), n if (instance_major and nrhs > 1) else 0,
+                  _lib.stream_handle(stream))
         return out
 
     def to_cluster(self, x: torch.Tensor) -> torch.Tensor:
@@ -377,7 +431,8 @@ class HMatrix:
         bi = B["lr_idx"][q]
         m, n, k = int(B["m"][bi]), int(B["n"][bi]), int(B["ranks"][q])
         ub, vb = int(B["l_ub"][q]), int(B["l_vb"][q])
-        U = self.pool[ub:ub + 3 * m * k].double().cpu().numpy().reshape(k, 3 * m).T
+        seg = leaf_segments(self.leaf_tiles, self.tile_of_row, int(B["r0"][bi]), m)
+        U = unpack_u(self.pool[ub:ub + 3 * m * k].double().cpu().numpy(), m, k, seg)
         V = self.pool[vb:vb + 2 * n * k].double().cpu().numpy().reshape(2 * n, k).T
         return U, V
 

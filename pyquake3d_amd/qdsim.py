@@ -53,8 +53,9 @@ class DenseOperator:
         self._x = torch.empty(2 * n, dtype=dtype, device=self.device)
         self.matvec_bytes = self.M.numel() * self.M.element_size() + 5 * 8 * n
 
-    def matvec(self, v1, v2, out=None, stream=None, nrhs=1):
-        """nrhs > 1: v1, v2, out rows interleaved by instance (cell*nrhs + b)."""
+    def matvec(self, v1, v2, out=None, stream=None, nrhs=1, instance_major=False):
+        """nrhs > 1: v1, v2 rows interleaved by instance (cell*nrhs + b); out
+        in the same layout, or instance-major (b*n + cell)."""
         n = self.n
         if out is None:
             out = torch.empty((3, n * nrhs), dtype=torch.float64, device=self.device)
@@ -65,7 +66,11 @@ class DenseOperator:
             out.view(-1).copy_(y)
         else:
             X = torch.cat([v1.view(n, nrhs), v2.view(n, nrhs)]).to(self.M.dtype)
-            out.view(3 * n, nrhs).copy_(self.M @ X)
+            Y = (self.M @ X).view(3, n, nrhs)
+            if instance_major:
+                out.view(3, nrhs, n).copy_(Y.transpose(1, 2))
+            else:
+                out.view(3, n, nrhs).copy_(Y)
         return out
 
 
@@ -268,7 +273,11 @@ class EnsembleQDsim:
     fields / initial states), stepped together: one multi-RHS operator product
     per step serves all of them (the operator is read once), and every
     instance keeps its own step size and the reference's accept/reject
-    controller. Element (cell i, instance b) is stored at i*NB + b.
+    controller. Cell fields are instance-major (cell i of instance b at
+    b*n + i, cluster order), so a wavefront of the cell kernels belongs to
+    one instance and a retry round costs only the instances still
+    attempting; the deficit is written interleaved (i*NB + b) for the
+    product, which returns instance-major.
 
     A lockstep iteration ends when every instance has accepted its step; the
     retry rounds recompute only the instances that were rejected (the others
@@ -276,9 +285,8 @@ class EnsembleQDsim:
 
     def __init__(self, problems, op, reltol: float = 1e-4, device="cuda", tableau: str = "ref"):
         nb = len(problems)
-        # a MultiHMatrix (meshes of different size): instance-major layout,
-        # one block-diagonal product per step; otherwise one shared mesh,
-        # interleaved layout and an NB-RHS product
+        # a MultiHMatrix (meshes of different size): one block-diagonal
+        # product per step; otherwise one shared mesh and an NB-RHS product
         self.ragged = bool(getattr(op, "ragged", False))
         if self.ragged:
             sizes = np.asarray(op.sizes)
@@ -304,15 +312,14 @@ class EnsembleQDsim:
         perm = np.asarray(op.perm)
         self.perm = perm
 
+        perms = op.perms if self.ragged else [perm] * nb
+        self.off = np.concatenate([[0], np.cumsum([len(pb) for pb in perms])]).astype(np.int64)
+
         def inter(arrs):
-            if self.ragged:
-                # instance b's cells, in its mesh's cluster order, one after another
-                A = np.concatenate([np.asarray(x, dtype=np.float64)[pb]
-                                    for x, pb in zip(arrs, op.perms)])
-                return torch.from_numpy(np.ascontiguousarray(A)).to(dev)
-            # (nb, n) original order -> (n*nb,) cluster order, cell-major
-            A = np.stack([np.asarray(x, dtype=np.float64)[perm] for x in arrs], axis=1)
-            return torch.from_numpy(np.ascontiguousarray(A.reshape(-1))).to(dev)
+            # instance b's cells, in its mesh's cluster order, one after another
+            A = np.concatenate([np.asarray(x, dtype=np.float64)[pb]
+                                for x, pb in zip(arrs, perms)])
+            return torch.from_numpy(np.ascontiguousarray(A)).to(dev)
 
         inits = [p.initial_state() for p in problems]
         (self.Tno, self.Tt1o, self.Tt2o, self.state, self.slipv1, self.slipv2,
@@ -331,10 +338,7 @@ class EnsembleQDsim:
         if len({bool(p.fix_Tn) for p in problems}) != 1:
             raise ValueError("fix_Tn must be the same for all instances")
         self.fix_Tn = bool(problems[0].fix_Tn)
-        if self.ragged:
-            inst = np.repeat(np.arange(nb, dtype=np.int32), op.sizes)
-        else:
-            inst = np.tile(np.arange(nb, dtype=np.int32), n)
+        inst = np.repeat(np.arange(nb, dtype=np.int32), np.diff(self.off))
         self.inst = torch.from_numpy(inst).to(dev)
         self.cellpar = torch.tensor([[p.V0, p.mu, p.Cs] for p in problems],
                                     dtype=torch.float64, device=dev).reshape(-1)
@@ -359,18 +363,23 @@ class EnsembleQDsim:
 
     def compute_K(self):
         s = _lib.stream_handle()
-        _lib.call("pq3d_deficit", self.ne, _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
-                  _lib.ptr(self.pl1), _lib.ptr(self.pl2), _lib.ptr(self.x[0]),
-                  _lib.ptr(self.x[1]), s)
+        if self.nrhs > 1:
+            _lib.call("pq3d_deficit_interleave", self.n, self.nrhs, _lib.ptr(self.slipv1),
+                      _lib.ptr(self.slipv2), _lib.ptr(self.pl1), _lib.ptr(self.pl2),
+                      _lib.ptr(self.x[0]), _lib.ptr(self.x[1]), s)
+        else:
+            _lib.call("pq3d_deficit", self.ne, _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
+                      _lib.ptr(self.pl1), _lib.ptr(self.pl2), _lib.ptr(self.x[0]),
+                      _lib.ptr(self.x[1]), s)
         if self.timing:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs)
+            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs, instance_major=True)
             e1.record()
             self.matvec_events.append((e0, e1))
         else:
-            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs)
+            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs, instance_major=True)
         _lib.call("pq3d_add_loading", self.ne, _lib.ptr(self.K[2]),
                   _lib.ptr(self.normal_loading), int(self.fix_Tn), s)
 
@@ -444,15 +453,10 @@ class EnsembleQDsim:
 
     def instance(self, name: str, b: int) -> np.ndarray:
         """Field ``name`` of instance b in original element order."""
-        if self.ragged:
-            o, n = int(self.op.off[b]), int(self.op.sizes[b])
-            x = getattr(self, name)[o:o + n].cpu().numpy()
-            out = np.empty(n)
-            out[self.op.perms[b]] = x
-            return out
-        x = getattr(self, name).reshape(self.n, self.nb)[:, b].cpu().numpy()
-        out = np.empty(self.n)
-        out[self.perm] = x
+        o, e = int(self.off[b]), int(self.off[b + 1])
+        x = getattr(self, name)[o:e].cpu().numpy()
+        out = np.empty(e - o)
+        out[self.op.perms[b] if self.ragged else self.perm] = x
         return out
 
 

@@ -107,8 +107,8 @@ def test_fp32_matvec_within_stated_bound(gpu, pq3d_lib, case):
 @pytest.mark.parametrize("dtype,leaf", [(torch.float64, 32), (torch.float32, 64)])
 def test_multi_rhs_products_match_single(gpu, pq3d_lib, case, dtype, leaf):
     """Batched products (NB = 2, 4, 8, 16) equal the single-vector product bit
-    for bit in every column. Ranks above 64 exercise the second pass over
-    rank chunks."""
+    for bit in every column, with interleaved and with instance-major
+    outputs. Ranks above 64 exercise the second pass over rank chunks."""
     mesh, _ = case
     H = HMatrix.from_mesh(mesh, MU, LAM, leaf=leaf, eta=2.0, eps=1e-9, dtype=dtype, device=gpu,
                           kmax=128)
@@ -122,6 +122,10 @@ def test_multi_rhs_products_match_single(gpu, pq3d_lib, case, dtype, leaf):
                      nrhs=nb).view(3, n, nb)
         for b in range(nb):
             assert torch.equal(y[:, :, b], single[b]), (nb, b)
+        yi = H.matvec(X1[:, :nb].contiguous().view(-1), X2[:, :nb].contiguous().view(-1),
+                      nrhs=nb, instance_major=True).view(3, nb, n)
+        for b in range(nb):
+            assert torch.equal(yi[:, b, :], single[b]), ("instance-major", nb, b)
 
 
 def test_halfspace_hmatrix_matches_dense_halfspace_operators(gpu, pq3d_lib):

@@ -91,9 +91,11 @@ class Checker:
             self.aca_caps[(int(r0), int(c0))] = kcap
         self.T(rank)[:nb] = torch.from_numpy(r)
 
-    def pq3d_hmat_aca_pack(self, tab, wsoff, nb, wU, wV, pool, esize, stream):
+    def pq3d_hmat_aca_pack(self, tab, wsoff, nb, tiles, tor, wU, wV, pool, esize, stream):
         tab_a = rows(self.T(tab), "lowrank_block")
         ws = rows(self.T(wsoff), "aca_ws")
+        tl = rows(self.T(tiles), "tile")
+        tor_a = self.T(tor).numpy()
         P = self.T(pool).numel()
         for b in range(nb):
             r0, m, c0, n, ub, vb, k = tab_a[b, :7]
@@ -103,14 +105,28 @@ class Checker:
             assert ws[b, 0] + 3 * m * k <= self.T(wU).numel()
             assert ws[b, 1] + 2 * n * k <= self.T(wV).numel()
             assert 0 <= ub and ub + 3 * m * k <= P and 0 <= vb and vb + 2 * n * k <= P
+            # the kernel's U destination, every row of the block: inside the
+            # block's 3mk range, and a bijection onto it
+            assert 0 <= r0 and r0 + m <= len(tor_a)
+            q = tor_a[r0:r0 + m]
+            assert (q >= 0).all() and (q < len(tl)).all()
+            ro, ls = tl[q, 0] - r0, tl[q, 1]
+            i = np.arange(m)
+            assert (ro >= 0).all() and (i >= ro).all() and (i < ro + ls).all()
+            assert (ro + ls <= m).all()
+            dst = (3 * k * ro[None, None, :] + np.arange(k)[:, None, None] * 3 * ls[None, None, :]
+                   + np.arange(3)[None, :, None] * ls[None, None, :] + (i - ro)[None, None, :])
+            assert np.array_equal(np.sort(dst.ravel()), np.arange(3 * m * k))
 
     def pq3d_hmat_matvec(self, units, nun, red, nred, tiles, tile_ptr, ntiles, items, pool,
-                         esize, nrhs, v1, v2, t, part, y1, y2, y3, stream):
+                         esize, nrhs, v1, v2, t, part, y1, y2, y3, ybs, stream):
         P = self.T(pool).numel()
         assert nrhs in (1, 2, 4, 8, 16)
         N = self.T(v1).numel() // nrhs
         assert self.T(v2).numel() == N * nrhs
         assert min(self.T(y).numel() for y in (y1, y2, y3)) >= N * nrhs
+        # instance-major outputs: (row, b) at b*ybs + row
+        assert ybs == 0 or (ybs >= N and (nrhs - 1) * ybs + N <= self.T(y1).numel())
         nt = self.T(t).numel() // nrhs
         npart = self.T(part).numel() // nrhs
         if nun:
@@ -157,6 +173,7 @@ def test_hmatrix_device_accesses_in_bounds(monkeypatch, leaf, eta, dtype, hs):
     H.matvec(x, x)
     x8 = torch.zeros(mesh.n * 8, dtype=torch.float64)
     H.matvec(x8, x8, nrhs=8)
+    H.matvec(x8, x8, nrhs=8, instance_major=True)
     assert "pq3d_hmat_matvec" in chk.calls
     if eta > 0:
         assert chk.calls.count("pq3d_hmat_aca") > 1   # the workspace budget forced batches

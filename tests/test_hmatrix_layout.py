@@ -8,7 +8,8 @@ import pytest
 import torch
 
 from pyquake3d_amd.cluster import build_block_partition, build_cluster_tree
-from pyquake3d_amd.hmatrix import build_worklists, plan_layout
+from pyquake3d_amd.hmatrix import (build_worklists, leaf_segments, leaf_tables, pack_u,
+                                   plan_layout, unpack_u)
 from pyquake3d_amd.mesh import get_eleVec, undulating_fault
 
 MU = LAM = 32e9
@@ -105,9 +106,12 @@ def test_layout_and_worklists_reproduce_dense_product(oracle, leaf):
         for op in range(6):
             blk = K[op][r0[bi]:r0[bi] + bm, c0[bi]:c0[bi] + bn]
             pool[base + op * bm * bn: base + (op + 1) * bm * bn] = blk.T.ravel()  # column-major
+    tiles, tor = leaf_tables(t)
     for bi, ub, vb, k in zip(L["lr_idx"], L["l_ub"], L["l_vb"], L["k"]):
         U, V = fac[bi]
-        pool[ub:ub + U.size] = U.T.ravel()   # column l of U contiguous
+        seg = leaf_segments(tiles, tor, r0[bi], m[bi])
+        pool[ub:ub + U.size] = pack_u(U, seg)   # one panel per leaf of the block rows
+        assert np.array_equal(unpack_u(pool[ub:ub + U.size], m[bi], k, seg), U)
         pool[vb:vb + V.size] = V.T.ravel()   # V stored transposed: 2n x k row-major
     W = build_worklists(t, n, r0, m, c0, nn, L["dense_idx"], L["d_base"], L["lr_idx"],
                         L["l_ub"], L["l_vb"], L["k"])
@@ -161,9 +165,10 @@ def test_block_diagonal_union_reproduces_each_product(oracle):
             for op in range(6):
                 blk = K[op][r0[bi]:r0[bi] + bm, c0[bi]:c0[bi] + bn]
                 pool[base + op * bm * bn: base + (op + 1) * bm * bn] = blk.T.ravel()
+        tiles, tor = leaf_tables(t)
         for bi, ub, vb, k in zip(L["lr_idx"], L["l_ub"], L["l_vb"], L["k"]):
             U, V = fac[bi]
-            pool[ub:ub + U.size] = U.T.ravel()
+            pool[ub:ub + U.size] = pack_u(U, leaf_segments(tiles, tor, r0[bi], m[bi]))
             pool[vb:vb + V.size] = V.T.ravel()
         W = build_worklists(t, n, r0, m, c0, nn, L["dense_idx"], L["d_base"], L["lr_idx"],
                             L["l_ub"], L["l_vb"], L["k"])
@@ -186,3 +191,34 @@ def test_block_diagonal_union_reproduces_each_product(oracle):
         ref = np.stack([K[0] @ x1 + K[3] @ x2, K[1] @ x1 + K[4] @ x2, K[2] @ x1 + K[5] @ x2])
         scale = np.abs(K).max() * np.abs(np.concatenate([x1, x2])).sum()
         assert np.abs(y[:, o:o + n] - ref).max() <= 1e-8 * scale
+
+
+def test_leaf_panels_of_u():
+    """U of a block spanning several leaves: every leaf's rows of all three
+    planes and all ranks form one contiguous panel, in leaf order, and the
+    layout of a single-leaf block is plain column-major 3m x k."""
+    U = np.arange(3 * 10 * 4, dtype=np.float64).reshape(30, 4)   # m = 10, k = 4
+    seg = np.array([[0, 3], [3, 5], [8, 2]])
+    flat = pack_u(U, seg)
+    for ro, ls in seg:
+        panel = flat[3 * 4 * ro:3 * 4 * (ro + ls)].reshape(4, 3, ls)
+        for o in range(3):
+            assert np.array_equal(panel[:, o, :].T, U[o * 10 + ro:o * 10 + ro + ls, :])
+    assert np.array_equal(unpack_u(flat, 10, 4, seg), U)
+    assert np.array_equal(pack_u(U, np.array([[0, 10]])), U.T.ravel())
+
+
+def test_xcd_order_is_a_bijection():
+    """Host restatement of the kernels' workgroup -> work-index map: each of
+    the 8 XCDs (workgroup b runs on XCD b % 8) gets one contiguous range of
+    work indices, and every index in [0, g) is produced once."""
+    def xcd_order(b, g):
+        q, r = divmod(g, 8)
+        x, i = b % 8, b // 8
+        return x * (q + 1) + i if x < r else r * (q + 1) + (x - r) * q + i
+    for g in (1, 7, 8, 9, 100, 2048, 2053):
+        w = np.array([xcd_order(b, g) for b in range(g)])
+        assert np.array_equal(np.sort(w), np.arange(g))
+        for x in range(min(g, 8)):
+            mine = np.sort(w[x::8])
+            assert np.array_equal(mine, np.arange(mine[0], mine[0] + len(mine)))

@@ -18,13 +18,15 @@ class IdentityOp:
         self.perm = np.arange(n)
         self.K = K
 
-    def matvec(self, v1, v2, out=None, nrhs=1):
+    def matvec(self, v1, v2, out=None, nrhs=1, instance_major=False):
         K = self.K
         n = K["A1s"].shape[0]
         v1, v2 = v1.numpy().reshape(n, nrhs), v2.numpy().reshape(n, nrhs)
         y = np.stack([K["A1s"] @ v1 + K["A1d"] @ v2, K["A2s"] @ v1 + K["A2d"] @ v2,
                       K["Bs"] @ v1 + K["Bd"] @ v2])
-        out.copy_(torch.from_numpy(y.reshape(3, n * nrhs)))
+        if instance_major:
+            y = y.transpose(0, 2, 1)
+        out.copy_(torch.from_numpy(np.ascontiguousarray(y).reshape(3, n * nrhs)))
         return out
 
 
@@ -55,6 +57,12 @@ class StepChecker:
         self._cells(n, s1, s2, p1, p2, x1, x2)
         self.T(x1)[:n] = self.T(s1)[:n] - self.T(p1)[:n]
         self.T(x2)[:n] = self.T(s2)[:n] - self.T(p2)[:n]
+
+    def pq3d_deficit_interleave(self, n, nb, s1, s2, p1, p2, x1, x2, stream):
+        self._cells(n * nb, s1, s2, p1, p2, x1, x2)
+        for src, pl, dst in ((s1, p1, x1), (s2, p2, x2)):
+            d = (self.T(src)[:n * nb] - self.T(pl)[:n * nb]).view(nb, n)
+            self.T(dst)[:n * nb] = d.t().reshape(-1)
 
     def pq3d_add_loading(self, n, dsig, load, fix, stream):
         self._cells(n, dsig, load)
