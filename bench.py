@@ -15,6 +15,12 @@ Workloads
                         same mesh stepped together, one multi-RHS H-matvec
                         per step; rate = sum over the draws
             ``value`` is the batched variant; all three are reported.
+  ragged    --count simulations (default 64) on meshes of uneven size as
+            one block-diagonal FP32 H-matrix with the device controller;
+            per-instance accuracy of every 8th against its FP64 solo run.
+  learned   Learned step size against the reference controller at equal
+            rtol (--nb draws, default 8): rejected-step fraction and wall
+            time of each over --steps steps from the same state.
   matvec    H-matvec only, --nb right-hand sides, on the hstep operator
             (--dtype f32|f64); the command the hardware-counter profiles use.
   assembly  Kernel assembly only: all N x N pairs at N = 20,000 (FP64, six
@@ -44,7 +50,7 @@ YEAR = 3.15576e7
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of the matvec workload,
 # FETCH scaled by 2.0 and WRITE by 1.0: the factors the same counters read
 # for a 2 GiB device-to-device copy (profiles/hmatvec_traffic_N100352.md).
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {("f32", 1): 6_242_980_245, ("f32", 16): 7_433_304_896}
 
 # FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
 # (unit strike and dip slip, both projected), counted operator by operator
@@ -193,7 +199,8 @@ def bench_hstep(args):
         "fast_fp32": fast,
         "batched_fp32": batched,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / MI355X_HBM_GBS, "traffic": None,
+                     "frac": achieved / MI355X_HBM_GBS,
+                     "traffic": MEASURED_TRAFFIC.get(("f32", args.nb)),
                      "kernel": f"H-matvec, {args.nb} right-hand sides (k_hmv_phase1 + "
                                f"k_hmv_phase1b + k_hmv_phase2)",
                      "kernel_ms": head["matvec_ms"], "algorithmic_bytes": head["matvec_bytes"]},
@@ -278,6 +285,178 @@ def bench_matvec(args):
     }
 
 
+# ------------------------------------------------------------- learned
+def _controller_run(ens, steps):
+    import torch
+    a0, s0, r0, t0 = ens.nattempts, ens.nsteps, int(ens.nrejects.sum()), ens.time.copy()
+    w0 = time.perf_counter()
+    for _ in range(steps):
+        ens.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    attempts = ens.nattempts - a0
+    rejected = int(ens.nrejects.sum()) - r0
+    accepted = (ens.nsteps - s0) * ens.nb
+    yrs = float(((ens.time - t0) / YEAR).sum())
+    return {"steps": ens.nsteps - s0, "attempt_rounds": attempts,
+            "accepted_steps": accepted, "rejected_steps": rejected,
+            "rejected_fraction": rejected / (accepted + rejected),
+            "attempt_rounds_per_step": attempts / max(ens.nsteps - s0, 1),
+            "wall_s": wall, "sim_years": yrs, "years_per_hour": yrs / (wall / 3600.0)}
+
+
+def bench_learned(args):
+    """Learned first-attempt step size against the reference controller at
+    equal rtol (1e-4) on the hstep problem: --nb friction draws, FP32
+    H-matrix at N = 100,352. The reference controller runs --record steps
+    and logs (features, log10 h_accepted/dtnext) pairs; the MLP (hand-written
+    forward/backward/Adam kernels) trains on them; then both controllers run
+    --steps steps from the same saved state. Both loops take their accept or
+    reject decision on the host after every attempt round, so the wall times
+    compare the controllers, not the drivers."""
+    import torch
+
+    from pyquake3d_amd.learned_dt import FeatureEnsemble, StepMLP
+    from pyquake3d_amd.problem import undulating_c3
+
+    problem = undulating_c3()
+    draws = [problem] + [undulating_c3(friction_seed=k, mesh=problem.mesh)
+                         for k in range(1, args.nb)]
+    H, _ = build_hmatrix(problem, torch.float32)
+    rec = FeatureEnsemble(draws, H, device=H.device)
+    rec.record = True
+    for _ in range(args.record):
+        rec.step()
+    X, T = rec.dataset()
+    mlp = StepMLP(device=H.device, seed=0)
+    t0 = time.perf_counter()
+    hist = mlp.fit(X, T, epochs=40, batch=1024, lr=2e-3)
+    train_s = time.perf_counter() - t0
+    saved = rec.save_state()
+    rec.record = False
+    plain = _controller_run(rec, args.steps)
+    le = FeatureEnsemble(draws, H, mlp=mlp, device=H.device)
+    le.load_state(saved)
+    learned = _controller_run(le, args.steps)
+    learned.update(first_attempt_rejected=le.nfirst_rejected, pi_fallback_attempts=le.nfallback,
+                   max_accepted_error_ratio=le.max_accepted_ratio)
+    log(f"learned: plain {json.dumps(plain)} learned {json.dumps(learned)}")
+    return {
+        "metric": "wall-time ratio, plain / learned step-size controller at equal rtol "
+                  "(N=1e5, rejected-step fractions alongside)",
+        "value": plain["wall_s"] / learned["wall_s"], "unit": "x",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.record,
+        "ms_per_step": 1e3 * learned["wall_s"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 operator storage, f64 state; f32 MLP",
+        "data": "synthetic undulating fault mesh and seeded friction fields",
+        "config": {"workload": f"learned step size, undulating fault 100x25 km, N={problem.n}, "
+                               f"{args.nb} friction draws, rtol 1e-4, {args.record} recorded "
+                               f"steps ({len(T)} samples), then {args.steps} steps per "
+                               f"controller from the same state"},
+        "plain": plain, "learned": learned,
+        "training": {"samples": int(len(T)), "epochs": 40, "loss_first": float(hist[0]),
+                     "loss_last": float(hist[-1]), "train_s": train_s},
+    }
+
+
+# ------------------------------------------------------------- ragged
+def bench_ragged(args):
+    """Ragged ensemble: --count simulations (default 64) on meshes of uneven
+    size (N log-uniform in [2000, 20000], planar or undulating 40 x 20 km
+    faults, own friction draws), one block-diagonal FP32 H-matrix
+    (MultiHMatrix: every matvec phase is one launch for all instances), the
+    device controller, --steps iterations after --warmup. Accuracy: every
+    8th instance is re-run alone in FP64 (and in FP32) to the same accepted
+    step; the report gives its event times (max V through 1e-3 m/s) and
+    final simulated time against the FP64 run."""
+    import numpy as np
+    import torch
+
+    from pyquake3d_amd.hmatrix import HMatrix, MultiHMatrix
+    from pyquake3d_amd.problem import ensemble_c4
+    from pyquake3d_amd.qdsim import DeviceEnsemble
+
+    dev = torch.device("cuda:0")
+    probs = ensemble_c4(args.count, seed=0)
+    t0 = time.perf_counter()
+    hs = [HMatrix.from_mesh(p.mesh, p.mu, p.lambda_, leaf=64, eta=2.0, eps=1e-4,
+                            dtype=torch.float32, device=dev) for p in probs]
+    M = MultiHMatrix(hs)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    log(f"ragged: {len(probs)} instances, N total {M.n}, built in {build_s:.1f} s")
+    ens = DeviceEnsemble(probs, M, device=dev, rounds=8)
+    for _ in range(args.warmup):
+        ens.step()
+    ens.sync()
+    t_sim0 = ens.time.copy()
+    ens.timing = True
+    ens.matvec_events = []
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        ens.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ens.sync()
+    mv = ens.matvec_ms()
+    yrs = (ens.time - t_sim0) / YEAR
+
+    def solo(k, H):
+        """Instance k alone to the ensemble's accepted-step count; event
+        times = upward crossings of max V through 1e-3 m/s (re-armed below
+        1e-4 m/s)."""
+        one = DeviceEnsemble([probs[k]], H, device=dev, rounds=8)
+        events, armed = [], True
+        while one.naccepted[0] < ens.naccepted[k]:
+            one.step()
+            one.sync()
+            vmax = float(one.max_slipv()[0])
+            if armed and vmax > 1e-3:
+                events.append(float(one.time[0]))
+                armed = False
+            elif vmax < 1e-4:
+                armed = True
+        return one.time[0], one.instance("slipv", 0), events
+
+    acc = []
+    for k in range(0, len(probs), 8):
+        v = ens.instance("slipv", k)
+        # the FP32 solo run equals the ensemble's instance bit for bit (checked
+        # on the final state), so its event times are the ensemble's
+        t32, v32, ev32 = solo(k, hs[k])
+        H64 = HMatrix.from_mesh(probs[k].mesh, probs[k].mu, probs[k].lambda_, leaf=64, eta=2.0,
+                                eps=1e-4, dtype=torch.float64, device=dev)
+        t64, v64, ev64 = solo(k, H64)
+        del H64
+        ne = min(len(ev32), len(ev64))
+        acc.append({"instance": k, "n": probs[k].n, "accepted_steps": int(ens.naccepted[k]),
+                    "equals_fp32_solo": bool(t32 == ens.time[k] and np.array_equal(v32, v)),
+                    "events_fp32": len(ev32), "events_fp64": len(ev64),
+                    "event_time_rel_err_vs_fp64": [abs(a - b) / b for a, b in
+                                                   zip(ev32[:ne], ev64[:ne])],
+                    "final_time_rel_err_vs_fp64": abs(ens.time[k] - t64) / t64})
+    algo = M.matvec_bytes
+    kern_ms = sum(mv) / len(mv)
+    achieved = algo / (kern_ms * 1e-3) / 1e9
+    return {
+        "metric": "sim-years/wall-hour, ragged ensemble (FP32 H-matrix, device controller)",
+        "value": float(yrs.sum()) / (wall / 3600.0), "unit": "sim-years/wall-hour",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 operator storage, f64 state and accumulation",
+        "data": "synthetic planar/undulating meshes and seeded friction draws",
+        "config": {"workload": f"ragged ensemble of {len(probs)} simulations, N log-uniform in "
+                               f"[2000, 20000] (total {M.n}), leaf 64, eta 2, ACA eps 1e-4"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / MI355X_HBM_GBS, "traffic": None, "kernel_ms": kern_ms,
+                     "algorithmic_bytes": algo},
+        "sizes": [int(p.n) for p in probs], "build_s": build_s,
+        "accepted_steps_min_max": [int(ens.naccepted.min()), int(ens.naccepted.max())],
+        "accuracy_sampled": acc,
+    }
+
+
 # ------------------------------------------------------------- assembly
 def bench_assembly(args):
     import numpy as np
@@ -354,21 +533,34 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="hstep", choices=["hstep", "assembly", "matvec"])
+    ap.add_argument("--workload", default="hstep", choices=["hstep", "assembly", "matvec", "learned", "ragged"])
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"],
                     help="operator storage of the matvec workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--nb", type=int, default=16, choices=[1, 2, 4, 8, 16],
-                    help="friction draws in the batched variant")
+    ap.add_argument("--nb", type=int, default=None, choices=[1, 2, 4, 8, 16],
+                    help="friction draws in the batched variant (default 16; learned: 8)")
+    ap.add_argument("--count", type=int, default=64, help="ragged: number of simulations")
+    ap.add_argument("--record", type=int, default=800,
+                    help="learned: steps of the reference controller logged for training")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the "
                                    "multi-GPU design is documented, not built)"}))
         sys.exit(2)
+    if args.workload != "learned":
+        args.nb = 16 if args.nb is None else args.nb
     if args.workload == "hstep":
         args.steps = 3000 if args.steps is None else args.steps
         args.warmup = 200 if args.warmup is None else args.warmup
         res = bench_hstep(args)
+    elif args.workload == "ragged":
+        args.steps = 1000 if args.steps is None else args.steps
+        args.warmup = 100 if args.warmup is None else args.warmup
+        res = bench_ragged(args)
+    elif args.workload == "learned":
+        args.steps = 1500 if args.steps is None else args.steps
+        args.nb = 8 if args.nb is None else args.nb
+        res = bench_learned(args)
     elif args.workload == "matvec":
         args.steps = 20 if args.steps is None else args.steps
         args.warmup = 3 if args.warmup is None else args.warmup

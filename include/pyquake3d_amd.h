@@ -299,16 +299,16 @@ int pq3d_mlp_adam(float* d_prm, const float* d_grad, int64_t nblk, int64_t nsamp
                   float* d_v, float lr, float beta1, float beta2, float eps, int t, float* d_gout,
                   void* stream);
 
-/* Per-instance solver features over ne = n*nb cells (cell-major ensemble
- * layout, 1 <= nb <= 1024), as 64-bit integers independent of summation
+/* Per-instance solver features over ne cells, cell e of instance d_inst[e]
+ * (NULL: one instance; 1 <= nb <= 1024), as 64-bit integers independent of summation
  * order: d_acc[4b] and d_acc[4b+1] = largest bit patterns of the doubles
  * 1000 + log10 V and 1000 - log10 V (log10 V clamped to [-999, 999]),
  * d_acc[4b+2] = sum of log10(V theta / Dc) in units of 2^-32 (two's
  * complement), d_acc[4b+3] = count(V > 1e-3); theta = Dc/V0 exp((psi-f0)/b).
  * No counterpart in the reference (it has no learned step size). */
-int pq3d_features(int64_t ne, int nb, const double* d_slipv, const double* d_psi,
-                  const double* d_b, const double* d_f0, double V0, uint64_t* d_acc,
-                  void* stream);
+int pq3d_features(int64_t ne, int nb, const int32_t* d_inst, const double* d_slipv,
+                  const double* d_psi, const double* d_b, const double* d_f0, double V0,
+                  uint64_t* d_acc, void* stream);
 
 #ifdef __cplusplus
 }

@@ -98,8 +98,8 @@ SIGNATURES = {
                                           c_void_p, c_void_p, c_void_p, c_void_p]),
     "pq3d_mlp_adam": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_float,
                               c_float, c_float, c_float, c_int, c_void_p, c_void_p]),
-    "pq3d_features": (c_int, [c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
-                              c_void_p, c_void_p]),
+    "pq3d_features": (c_int, [c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_double, c_void_p, c_void_p]),
 }
 
 _lib = None

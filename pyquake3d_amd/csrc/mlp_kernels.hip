@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float* __restrict__ prm,
 }
 
 // Per-instance solver-state features over the cells of each instance
-// (element e belongs to instance e % nb, cell-major layout):
+// (element e belongs to instance inst[e]):
 //   {max log10 V, min log10 V, sum log10(V theta/Dc), count(V > 1e-3)}
 // with theta = Dc/V0 exp((psi - f0)/b) (src/QDsim.py:642 steady-state law).
 // Integer accumulation keeps the result independent of the order of the
@@ -156,6 +156,7 @@ constexpr int kFeatMaxInst = 1024;
 constexpr double kFeatScale = 4294967296.0;  // 2^32
 
 __global__ void __launch_bounds__(256) k_features(int64_t ne, int nb,
+                                                  const int32_t* __restrict__ inst,
                                                   const double* __restrict__ slipv,
                                                   const double* __restrict__ psi,
                                                   const double* __restrict__ pb,
@@ -166,7 +167,7 @@ __global__ void __launch_bounds__(256) k_features(int64_t ne, int nb,
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
-    const int b = (int)(e % nb);
+    const int b = inst ? inst[e] : 0;
     const double V = slipv[e];
     double lv = log10(V);
     lv = fmax(fmin(lv, 999.0), -999.0);
@@ -233,22 +234,24 @@ int pq3d_mlp_adam(float* d_prm, const float* d_grad, int64_t nblk, int64_t nsamp
   return PQ3D_OK;
 }
 
-// Per-instance features of an ensemble state (see k_features); d_acc: 4*nb
-// 64-bit integers, overwritten. Host finishing: max = bits(acc0) - 1000,
+// Per-instance features of an ensemble state (see k_features); cell e
+// belongs to instance d_inst[e] (NULL: one instance); d_acc: 4*nb 64-bit
+// integers, overwritten. Host finishing: max = bits(acc0) - 1000,
 // min = 1000 - bits(acc1), mean = (int64)acc2 / 2^32 / n_cells,
 // frac = acc3 / n_cells.
-int pq3d_features(int64_t ne, int nb, const double* d_slipv, const double* d_psi,
-                  const double* d_b, const double* d_f0, double V0, uint64_t* d_acc,
-                  void* stream) {
+int pq3d_features(int64_t ne, int nb, const int32_t* d_inst, const double* d_slipv,
+                  const double* d_psi, const double* d_b, const double* d_f0, double V0,
+                  uint64_t* d_acc, void* stream) {
   clear_error();
   PQ3D_REQUIRE(ne >= 0 && nb >= 1 && nb <= kFeatMaxInst, "nb must be in [1, 1024]");
   PQ3D_REQUIRE(d_slipv && d_psi && d_b && d_f0 && d_acc, "null pointer");
+  PQ3D_REQUIRE(nb == 1 || d_inst, "d_inst is required for nb > 1");
   hipStream_t s = as_stream(stream);
   k_feat_clear<<<blocks_for(4 * nb, 256), 256, 0, s>>>((unsigned long long*)d_acc, 4 * nb);
   if (ne > 0) {
     const int64_t want = (ne + 255) / 256;
     const unsigned grid = (unsigned)(want < 512 ? want : 512);
-    k_features<<<grid, 256, 0, s>>>(ne, nb, d_slipv, d_psi, d_b, d_f0, V0,
+    k_features<<<grid, 256, 0, s>>>(ne, nb, d_inst, d_slipv, d_psi, d_b, d_f0, V0,
                                     (unsigned long long*)d_acc);
   }
   PQ3D_LAUNCH_CHECK();

@@ -31,9 +31,10 @@ NFEAT = 6
 FEAT_SCALE = 2.0 ** 32
 
 
-def decode_features(acc: np.ndarray, ncell: int):
+def decode_features(acc: np.ndarray, ncell):
     """Finish the integer slots written by pq3d_features (nb x 4 int64):
-    max log10 V, min log10 V, mean log10(V theta/Dc), fraction V > 1e-3."""
+    max log10 V, min log10 V, mean log10(V theta/Dc), fraction V > 1e-3.
+    ``ncell``: cells per instance (a number, or one per instance)."""
     acc = np.ascontiguousarray(acc, dtype=np.int64)
     vmax = acc[:, 0].view(np.float64) - 1000.0
     vmin = 1000.0 - acc[:, 1].view(np.float64)
@@ -156,11 +157,11 @@ class FeatureEnsemble(EnsembleQDsim):
         self.max_accepted_ratio = 0.0                 # largest max(c1, c2) ever accepted
 
     def features(self) -> np.ndarray:
-        _lib.call("pq3d_features", self.ne, self.nb, _lib.ptr(self.slipv), _lib.ptr(self.state),
-                  _lib.ptr(self.b), _lib.ptr(self.f0), self.V0, _lib.ptr(self.acc),
-                  _lib.stream_handle())
+        _lib.call("pq3d_features", self.ne, self.nb, _lib.ptr(self.inst), _lib.ptr(self.slipv),
+                  _lib.ptr(self.state), _lib.ptr(self.b), _lib.ptr(self.f0), self.V0,
+                  _lib.ptr(self.acc), _lib.stream_handle())
         a = self.acc.cpu().numpy().reshape(self.nb, 4)
-        vmax, vmin, mean, frac = decode_features(a, self.n)
+        vmax, vmin, mean, frac = decode_features(a, np.diff(self.off))
         return np.stack([vmax, vmin, mean, frac, np.log10(self.last_ratio),
                          np.log10(self.last_growth)], axis=1)
 
@@ -230,3 +231,12 @@ class FeatureEnsemble(EnsembleQDsim):
 
     def dataset(self):
         return np.concatenate(self.log_feats), np.concatenate(self.log_target)
+
+    def save_state(self) -> dict:
+        d = super().save_state()
+        d["last_ratio"], d["last_growth"] = self.last_ratio.copy(), self.last_growth.copy()
+        return d
+
+    def load_state(self, d: dict) -> None:
+        super().load_state(d)
+        self.last_ratio, self.last_growth = d["last_ratio"].copy(), d["last_growth"].copy()

@@ -444,6 +444,22 @@ class EnsembleQDsim:
         torch.cuda.synchronize()
         return [a.elapsed_time(b) for a, b in self.matvec_events]
 
+    STATE_FIELDS = ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "rake", "slip1",
+                    "slip2")
+
+    def save_state(self) -> dict:
+        """Copies of the evolving fields, the simulated times and the next
+        step sizes: ``load_state`` on an ensemble of the same problems
+        continues from exactly this point."""
+        d = {k: getattr(self, k).clone() for k in self.STATE_FIELDS}
+        d["time"], d["dt"] = self.time.copy(), self.dt.copy()
+        return d
+
+    def load_state(self, d: dict) -> None:
+        for k in self.STATE_FIELDS:
+            getattr(self, k).copy_(d[k])
+        self.time, self.dt = d["time"].copy(), d["dt"].copy()
+
     def max_slipv(self) -> np.ndarray:
         """Max slip rate of every instance after the last commit."""
         self.vmax_host.copy_(self.vmax)
@@ -512,6 +528,20 @@ class DeviceEnsemble(EnsembleQDsim):
                   _lib.ptr(self.rake), _lib.ptr(self.slip1), _lib.ptr(self.slip2),
                   _lib.ptr(self.vmax), self.nb, s)
         self.iterations += 1
+
+    DEVICE_FIELDS = ("hcur", "dtnext", "flag", "nrj", "status", "tdev", "nacc")
+
+    def save_state(self) -> dict:
+        self.sync()
+        d = super().save_state()
+        d.update({k: getattr(self, k).clone() for k in self.DEVICE_FIELDS})
+        return d
+
+    def load_state(self, d: dict) -> None:
+        super().load_state(d)
+        for k in self.DEVICE_FIELDS:
+            getattr(self, k).copy_(d[k])
+        self.sync()
 
     def sync(self):
         """Host copies of time, next dt and accepted-step counts; raises

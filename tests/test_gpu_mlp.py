@@ -81,28 +81,36 @@ def test_training_reduces_loss(gpu, pq3d_lib):
     assert np.sqrt(np.mean((pred - T) ** 2)) < 0.3 * T.std()
 
 
-def test_features_match_numpy(gpu, pq3d_lib):
+@pytest.mark.parametrize("ragged", [False, True])
+def test_features_match_numpy(gpu, pq3d_lib, ragged):
+    """Per-instance reductions over instance-major cells (equal or uneven
+    instance sizes) against numpy."""
     from pyquake3d_amd.learned_dt import decode_features
     rng = np.random.default_rng(11)
-    n, nb = 5003, 7
-    V = 10.0 ** rng.uniform(-14, 0.5, (n, nb))
-    psi = rng.uniform(0.5, 0.8, (n, nb))
-    b = rng.uniform(0.01, 0.02, (n, nb))
-    f0 = np.full((n, nb), 0.6)
+    nb = 7
+    sizes = rng.integers(1, 6000, nb) if ragged else np.full(nb, 5003)
+    inst = np.repeat(np.arange(nb, dtype=np.int32), sizes)
+    ne = int(sizes.sum())
+    V = 10.0 ** rng.uniform(-14, 0.5, ne)
+    psi = rng.uniform(0.5, 0.8, ne)
+    b = rng.uniform(0.01, 0.02, ne)
+    f0 = np.full(ne, 0.6)
     V0 = 1e-6
-    d = lambda x: torch.from_numpy(np.ascontiguousarray(x).ravel()).to(gpu)
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(gpu)
     acc = torch.zeros(4 * nb, dtype=torch.int64, device=gpu)
-    dV, dpsi, db, df0 = d(V), d(psi), d(b), d(f0)  # held until the kernel has run
-    _lib.call("pq3d_features", n * nb, nb, _lib.ptr(dV), _lib.ptr(dpsi), _lib.ptr(db),
+    dI, dV, dpsi, db, df0 = d(inst), d(V), d(psi), d(b), d(f0)  # held until the kernel has run
+    _lib.call("pq3d_features", ne, nb, _lib.ptr(dI), _lib.ptr(dV), _lib.ptr(dpsi), _lib.ptr(db),
               _lib.ptr(df0), V0, _lib.ptr(acc), _lib.stream_handle())
-    vmax, vmin, mean, frac = decode_features(acc.cpu().numpy().reshape(nb, 4), n)
+    vmax, vmin, mean, frac = decode_features(acc.cpu().numpy().reshape(nb, 4), sizes)
     lv = np.log10(V)
-    # 1000 + log10 V is stored as a double: resolution 2^-43 * 1000
-    assert np.abs(vmax - lv.max(0)).max() <= 1e-12
-    assert np.abs(vmin - lv.min(0)).max() <= 1e-12
     lvt = np.log10(V / V0) + (psi - f0) / (b * np.log(10.0))
-    assert np.abs(mean - lvt.mean(0)).max() <= 1e-8
-    assert np.array_equal(frac, (V > 1e-3).sum(0) / n)
+    for k in range(nb):
+        s = inst == k
+        # 1000 + log10 V is stored as a double: resolution 2^-43 * 1000
+        assert abs(vmax[k] - lv[s].max()) <= 1e-12
+        assert abs(vmin[k] - lv[s].min()) <= 1e-12
+        assert abs(mean[k] - lvt[s].mean()) <= 1e-8
+        assert frac[k] == (V[s] > 1e-3).sum() / sizes[k]
 
 
 def test_learned_step_size_loop(gpu, pq3d_lib):
