@@ -135,8 +135,8 @@ int pq3d_tde_traction_block_hs_f32(const double* d_recv, const double* d_rec,
 /* Row widths (int64 elements) of the tables the kernels below read, in the
  * order: dense block, low-rank block, ACA workspace offsets, matvec item,
  * phase-1 unit, phase-1b reduction, tile, receiver record, source record,
- * half-space source record. Host-only; returns the number of values written
- * (at most n). */
+ * half-space source record, phase-1 unit member. Host-only; returns the
+ * number of values written (at most n). */
 int pq3d_layout_constants(int64_t* out, int64_t n);
 
 /* Near-field blocks (replaces ACA_worker's dense branch with
@@ -179,10 +179,12 @@ int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nbl
  * the same layout (y_bstride = 0) or instance-major, element (row, b) at
  * b*y_bstride + row (y_bstride > 0, normally the row count). Low-rank V
  * factors are stored transposed (2n x k), U in leaf panels. Work lists
- * (phase-1 units, phase-1b reductions, tiles, items) as built by
+ * (phase-1 units and their member blocks, phase-1b reductions, tiles, items)
+ * as built by
  * pyquake3d_amd.hmatrix.build_worklists; d_t: nt*nrhs and d_part:
  * npart*nrhs doubles of scratch. */
-int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_red, int64_t nred,
+int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
+                     const int64_t* d_red, int64_t nred,
                      const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
                      const int64_t* d_items, const void* d_pool, int dtype_bytes, int nrhs,
                      const double* d_v1, const double* d_v2, double* d_t, double* d_part,

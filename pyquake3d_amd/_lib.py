@@ -65,10 +65,10 @@ SIGNATURES = {
                               c_void_p, c_void_p]),
     "pq3d_hmat_aca_pack": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_int, c_void_p]),
-    "pq3d_hmat_matvec": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p,
-                                 c_int64, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
-                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
-                                 c_void_p]),
+    "pq3d_hmat_matvec": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p,
+                                 c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_int64, c_void_p]),
     # time step
     "pq3d_rk45_attempt": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -128,7 +128,7 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
 
 
 LAYOUT_NAMES = ("dense_block", "lowrank_block", "aca_ws", "item", "unit", "reduce", "tile",
-                "recv", "rec", "hrec")
+                "recv", "rec", "hrec", "member")
 
 
 def layout_constants() -> dict:

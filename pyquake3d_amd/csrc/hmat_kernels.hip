@@ -36,11 +36,14 @@ enum { LB_R0 = 0, LB_M = 1, LB_C0 = 2, LB_N = 3, LB_UB = 4, LB_VB = 5, LB_K = 6,
 enum { WS_U = 0, WS_V = 1, WS_F = 2, WS_W = 4 };
 // matvec work item: kind (0 dense, 1 low-rank), base, ld, plane, count, xoff
 enum { IT_KIND = 0, IT_BASE = 1, IT_LD = 2, IT_PLANE = 3, IT_COUNT = 4, IT_XOFF = 5, IT_W = 6 };
-// phase-1 unit (a column chunk of one low-rank block): V^T base, k, first
-// stacked column, columns, block n, block c0, output row offset, direct flag
-// (1: output rows are the block's t rows; 0: partial rows reduced in phase 1b)
-enum { UN_VT = 0, UN_K = 1, UN_CB = 2, UN_CC = 3, UN_N = 4, UN_C0 = 5, UN_OUT = 6, UN_DIRECT = 7,
-       UN_W = 8 };
+// phase-1 unit (a column chunk of one or more low-rank blocks with the same
+// source columns): first member row, members, first stacked column, columns,
+// block n, block c0, direct flag (1: output rows are the blocks' t rows; 0:
+// partial rows reduced in phase 1b), total rank (<= 64 with several members)
+enum { UN_MEM = 0, UN_NMEM = 1, UN_CB = 2, UN_CC = 3, UN_N = 4, UN_C0 = 5, UN_DIRECT = 6,
+       UN_KTOT = 7, UN_W = 8 };
+// phase-1 unit member: V^T base, k, output row offset, first lane
+enum { MB_VT = 0, MB_K = 1, MB_OUT = 2, MB_LANE = 3, MB_W = 4 };
 // phase-1b reduction (blocks split over several units): t row offset, k,
 // first partial row, number of units (partial rows of unit u start at
 // first + u*k)
@@ -388,18 +391,18 @@ constexpr bool p2_prefetch() {
 }
 
 // Phase 1: t[l][b] = sum over the unit's stacked columns c of V^T[c][l] x_b[c]
-// (x = v1 for c < n, v2 for c >= n); one wavefront per unit, lane <-> l; the
-// terms of every (l, b) are added in order of c.
+// (x = v1 for c < n, v2 for c >= n); one wavefront per unit, lane <-> (member
+// block, rank l); the terms of every (l, b) are added in order of c. vp is
+// the lane's V^T element of the first column, k its row stride.
 template <typename T, int NB>
-__device__ __forceinline__ void p1_columns(const T* __restrict__ vc, int lc, int64_t k,
-                                           int64_t len, const double* __restrict__ xs,
-                                           double (&acc)[NB]) {
+__device__ __forceinline__ void p1_columns(const T* __restrict__ vp, int64_t k, int64_t len,
+                                           const double* __restrict__ xs, double (&acc)[NB]) {
   constexpr int U = p1_unroll<NB>();
   int64_t q = 0;
   for (; q + U <= len; q += U) {
     T v[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = ld_op(vc + (q + u) * k + lc);
+    for (int u = 0; u < U; ++u) v[u] = ld_op(vp + (q + u) * k);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const double vu = (double)v[u];
@@ -408,15 +411,30 @@ __device__ __forceinline__ void p1_columns(const T* __restrict__ vc, int lc, int
     }
   }
   for (; q < len; ++q) {
-    const double vu = (double)ld_op(vc + q * k + lc);
+    const double vu = (double)ld_op(vp + q * k);
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = fma(vu, xs[q * NB + b], acc[b]);
   }
 }
 
+// The unit's strike columns [cb, e1) read v1, its dip columns [s2, ce) v2.
+template <typename T, int NB>
+__device__ __forceinline__ void p1_unit_columns(const T* __restrict__ vrow, int64_t k, int64_t cb,
+                                                int64_t ce, int64_t n, int64_t c0,
+                                                const double* __restrict__ v1,
+                                                const double* __restrict__ v2,
+                                                double (&acc)[NB]) {
+  const int64_t e1 = ce < n ? ce : n;
+  const int64_t s2 = cb > n ? cb : n;
+  if (cb < e1) p1_columns<T, NB>(vrow + cb * k, k, e1 - cb, v1 + (c0 + cb) * NB, acc);
+  if (s2 < ce) p1_columns<T, NB>(vrow + s2 * k, k, ce - s2, v2 + (c0 + s2 - n) * NB, acc);
+}
+
 template <typename T, int NB>
 __global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ units,
-                                                    int64_t nunits, const T* __restrict__ pool,
+                                                    int64_t nunits,
+                                                    const int64_t* __restrict__ members,
+                                                    const T* __restrict__ pool,
                                                     const double* __restrict__ v1,
                                                     const double* __restrict__ v2,
                                                     double* __restrict__ t,
@@ -425,27 +443,48 @@ __global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ 
   const int lane = threadIdx.x & 63;
   if (u >= nunits) return;
   const int64_t* it = units + u * UN_W;
-  const int64_t vt = it[UN_VT], k = it[UN_K], cb = it[UN_CB], cc = it[UN_CC], n = it[UN_N];
-  const int64_t c0 = it[UN_C0], out = it[UN_OUT];
+  const int64_t mem0 = it[UN_MEM], nmem = it[UN_NMEM], cb = it[UN_CB], cc = it[UN_CC];
+  const int64_t n = it[UN_N], c0 = it[UN_C0], ktot = it[UN_KTOT];
   double* dst = it[UN_DIRECT] ? t : part;
   const int64_t ce = cb + cc;
-  const int64_t e1 = ce < n ? ce : n;   // strike columns [cb, e1) read v1
-  const int64_t s2 = cb > n ? cb : n;   // dip columns [s2, ce) read v2
-  for (int64_t l0 = 0; l0 < k; l0 += 64) {
-    const int64_t l = l0 + lane;
-    const int lc = (int)((l < k ? l : k - 1) - l0);  // idle lanes re-read row k-1, never stored
-    const T* vb = pool + vt + l0;
-    double acc[NB];
+  if (nmem == 1) {
+    // one block, lanes <-> ranks in chunks of 64
+    const int64_t* mb = members + mem0 * MB_W;
+    const int64_t vt = mb[MB_VT], k = mb[MB_K], out = mb[MB_OUT];
+    for (int64_t l0 = 0; l0 < k; l0 += 64) {
+      const int64_t l = l0 + lane;
+      const int64_t lc = l < k ? l : k - 1;  // idle lanes re-read row k-1, never stored
+      double acc[NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = 0.0;
-    if (cb < e1)
-      p1_columns<T, NB>(vb + cb * k, lc, k, e1 - cb, v1 + (c0 + cb) * NB, acc);
-    if (s2 < ce)
-      p1_columns<T, NB>(vb + s2 * k, lc, k, ce - s2, v2 + (c0 + s2 - n) * NB, acc);
-    if (l < k) {
+      for (int b = 0; b < NB; ++b) acc[b] = 0.0;
+      p1_unit_columns<T, NB>(pool + vt + lc, k, cb, ce, n, c0, v1, v2, acc);
+      if (l < k) {
 #pragma unroll
-      for (int b = 0; b < NB; ++b) dst[(out + l) * NB + b] = acc[b];
+        for (int b = 0; b < NB; ++b) dst[(out + l) * NB + b] = acc[b];
+      }
     }
+    return;
+  }
+  // several blocks (total rank <= 64): lane -> the member whose lanes hold it
+  int64_t vt = 0, k = 1, out = 0, lane0 = 0;
+  for (int64_t q = 0; q < nmem; ++q) {
+    const int64_t* mb = members + (mem0 + q) * MB_W;
+    if (q == 0 || lane >= mb[MB_LANE]) {
+      vt = mb[MB_VT];
+      k = mb[MB_K];
+      out = mb[MB_OUT];
+      lane0 = mb[MB_LANE];
+    }
+  }
+  const bool live = lane < ktot;
+  const int64_t l = live ? lane - lane0 : k - 1;  // idle lanes: last member's last rank
+  double acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = 0.0;
+  p1_unit_columns<T, NB>(pool + vt + l, k, cb, ce, n, c0, v1, v2, acc);
+  if (live) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) dst[(out + l) * NB + b] = acc[b];
   }
 }
 
@@ -600,14 +639,15 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
 }
 
 template <typename T, int NB>
-static void launch_matvec(const int64_t* units, int64_t nunits, const int64_t* red,
+static void launch_matvec(const int64_t* units, int64_t nunits, const int64_t* members,
+                          const int64_t* red,
                           int64_t nred, const int64_t* tiles, const int64_t* tile_ptr,
                           int64_t ntiles, const int64_t* items, const T* pool, const double* v1,
                           const double* v2, double* t, double* part, double* y1, double* y2,
                           double* y3, int64_t ybs, hipStream_t s) {
   if (nunits > 0)
-    k_hmv_phase1<T, NB><<<(unsigned)((nunits + 3) / 4), 256, 0, s>>>(units, nunits, pool, v1, v2,
-                                                                      t, part);
+    k_hmv_phase1<T, NB><<<(unsigned)((nunits + 3) / 4), 256, 0, s>>>(units, nunits, members,
+                                                                      pool, v1, v2, t, part);
   if (nred > 0)
     k_hmv_phase1b<NB><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(red, nred, part, t);
   if (ntiles > 0)
@@ -625,11 +665,11 @@ extern "C" {
 // Row widths of every table the H-matrix kernels read, in this order:
 // dense block, low-rank block, ACA workspace offsets, matvec item, phase-1
 // unit, phase-1b reduction, tile, receiver record, source record, half-space
-// source record. Returns the number written.
+// source record, phase-1 unit member. Returns the number written.
 int pq3d_layout_constants(int64_t* out, int64_t n) {
-  const int64_t v[10] = {DB_W, LB_W, WS_W, IT_W, UN_W, RD_W, TL_W, kRecvW, kSrcRec, kHsRec};
+  const int64_t v[11] = {DB_W, LB_W, WS_W, IT_W, UN_W, RD_W, TL_W, kRecvW, kSrcRec, kHsRec, MB_W};
   int64_t k = 0;
-  for (; k < n && k < 10; ++k) out[k] = v[k];
+  for (; k < n && k < 11; ++k) out[k] = v[k];
   return (int)k;
 }
 
@@ -716,7 +756,8 @@ int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nbl
 // row*nrhs + b). Outputs: the same layout with y_bstride = 0, or element
 // (row, b) at b*y_bstride + row (instance-major) with y_bstride > 0.
 // d_t: nt*nrhs doubles, d_part: npart*nrhs doubles of scratch.
-int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_red, int64_t nred,
+int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
+                     const int64_t* d_red, int64_t nred,
                      const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
                      const int64_t* d_items, const void* d_pool, int dtype_bytes, int nrhs,
                      const double* d_v1, const double* d_v2, double* d_t, double* d_part,
@@ -729,7 +770,7 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_re
   PQ3D_REQUIRE(d_tiles && d_tile_ptr && d_items && d_pool && d_v1 && d_v2 && d_y1 && d_y2 &&
                    d_y3,
                "null pointer");
-  PQ3D_REQUIRE(nunits == 0 || (d_units && d_t), "null pointer");
+  PQ3D_REQUIRE(nunits == 0 || (d_units && d_members && d_t), "null pointer");
   PQ3D_REQUIRE(nred == 0 || (d_red && d_part), "null pointer");
   PQ3D_REQUIRE(nunits < (1LL << 33) && nred < (1LL << 33) && ntiles < (1LL << 31),
                "too many work items");
@@ -737,11 +778,13 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_re
 #define PQ3D_MV(NBV)                                                                        \
   do {                                                                                      \
     if (dtype_bytes == 4)                                                                   \
-      launch_matvec<float, NBV>(d_units, nunits, d_red, nred, d_tiles, d_tile_ptr, ntiles,  \
+      launch_matvec<float, NBV>(d_units, nunits, d_members, d_red, nred, d_tiles,           \
+                                d_tile_ptr, ntiles,                                         \
                                 d_items, (const float*)d_pool, d_v1, d_v2, d_t, d_part,     \
                                 d_y1, d_y2, d_y3, y_bstride, s);                            \
     else                                                                                    \
-      launch_matvec<double, NBV>(d_units, nunits, d_red, nred, d_tiles, d_tile_ptr, ntiles, \
+      launch_matvec<double, NBV>(d_units, nunits, d_members, d_red, nred, d_tiles,          \
+                                 d_tile_ptr, ntiles,                                        \
                                  d_items, (const double*)d_pool, d_v1, d_v2, d_t, d_part,   \
                                  d_y1, d_y2, d_y3, y_bstride, s);                           \
   } while (0)

@@ -128,6 +128,42 @@ def unpack_u(flat, m, k, seg):
     return U.reshape(3 * m, k)
 
 
+def pack_units(groups):
+    """Phase-1 units from low-rank blocks grouped by source columns:
+    ``groups[(c0, n)]`` lists (V base, k, t row, first partial row) per
+    block. Blocks of a group go into wavefronts by first fit in order of
+    decreasing rank (capacity 64 lanes; a block of rank > 64 is alone and
+    its kernel loops over 64-rank chunks); each wavefront then covers the
+    group's 2n stacked columns in chunks of UNIT_COLS. Returns (units (Q, 8),
+    members (M, 4)) as described in ``build_worklists``."""
+    units, members = [], []
+    for (bc0, bn) in sorted(groups):
+        blocks = sorted(groups[(bc0, bn)], key=lambda b: -b[1])
+        bins = []
+        for blk in blocks:
+            k = blk[1]
+            for bin_ in bins:
+                if k <= 64 and bin_[0] + k <= 64:
+                    bin_[0] += k
+                    bin_[1].append(blk)
+                    break
+            else:
+                bins.append([k, [blk]])
+        C = 2 * bn
+        nu = -(-C // UNIT_COLS)
+        for u in range(nu):                  # units reading the same columns are adjacent
+            cb = u * UNIT_COLS
+            for ktot, blks in bins:
+                mem0, lane0 = len(members), 0
+                for vb, k, to, po in blks:
+                    members.append((vb, k, to if nu == 1 else po + u * k, lane0))
+                    lane0 += k
+                units.append((mem0, len(blks), cb, min(UNIT_COLS, C - cb), bn, bc0,
+                              1 if nu == 1 else 0, ktot))
+    return (np.ascontiguousarray(np.array(units, dtype=np.int64).reshape(-1, 8)),
+            np.ascontiguousarray(np.array(members, dtype=np.int64).reshape(-1, 4)))
+
+
 def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_vb, k_keep):
     """Matvec work lists.
 
@@ -139,11 +175,17 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
             (w = v1), kind 2 = its dip operators (w = v2), kind 1 = low-rank
             chunk of U (w = t)
     tile_ptr (T+1,): CSR offsets of each tile's items
-    units   (Q, 8): phase-1 units (V^T base, k, first stacked column, columns,
-            block n, block c0, output row, direct); V^T of a block is 2n x k
-            row-major at its V base, U is in leaf panels (``pack_u``). Units
-            are ordered by the source columns they read, so that units sharing
-            x run next to each other
+    units   (Q, 8): phase-1 units, one wavefront each: (first member row,
+            members, first stacked column, columns, block n, block c0,
+            direct, total rank). The members of a unit are low-rank blocks
+            with the same source columns (c0, n), so x is the same for all
+            of them; a unit of several members has total rank <= 64 and its
+            lanes run over (member, rank). Units reading the same columns
+            are adjacent.
+    members (M, 4): (V^T base, k, output row, first lane) per unit member;
+            V^T of a block is 2n x k row-major at its V base, U is in leaf
+            panels (``pack_u``). Output rows are the block's t rows (direct)
+            or its partial rows for this column chunk.
     reduce  (R, 4): phase-1b sums (t row, k, first partial row, units)
     nt, npart: rows of t and of the partial buffer
     """
@@ -161,20 +203,16 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
                 cnt = min(DENSE_CHUNK, bn - j0)
                 items.append((q, 0, base + j0 * bm + ro, bm, bm * bn, cnt, bc0 + j0))
                 items.append((q, 2, base + 3 * bm * bn + j0 * bm + ro, bm, bm * bn, cnt, bc0 + j0))
-    units, reds = [], []
+    reds = []
     toff = 0
     poff = 0
+    groups = {}   # (c0, n) -> [(V base, k, t row, first partial row)]
     for bi, ub, vb, k in zip(np.asarray(lr_idx).tolist(), np.asarray(l_ub).tolist(),
                              np.asarray(l_vb).tolist(), np.asarray(k_keep).tolist()):
         br0, bm, bc0, bn = int(r0[bi]), int(m[bi]), int(c0[bi]), int(nn[bi])
-        C = 2 * bn
-        nu = -(-C // UNIT_COLS)
-        if nu == 1:
-            units.append((vb, k, 0, C, bn, bc0, toff, 1))
-        else:
-            for u in range(nu):
-                cb = u * UNIT_COLS
-                units.append((vb, k, cb, min(UNIT_COLS, C - cb), bn, bc0, poff + u * k, 0))
+        nu = -(-2 * bn // UNIT_COLS)
+        groups.setdefault((bc0, bn), []).append((vb, k, toff, poff))
+        if nu > 1:
             reds.append((toff, k, poff, nu))
             poff += nu * k
         for q in range(tile_of_row[br0], tile_of_row[br0 + bm - 1] + 1):
@@ -184,15 +222,14 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
                 cnt = min(LR_CHUNK, k - l0)
                 items.append((q, 1, pb + l0 * 3 * ls, 3 * ls, ls, cnt, toff + l0))
         toff += k
-    un = np.array(units, dtype=np.int64).reshape(-1, 8)
-    un = un[np.lexsort((un[:, 2], un[:, 5]))]   # by block column start, then chunk
+    units, members = pack_units(groups)
     it = np.array(items, dtype=np.int64).reshape(-1, 7)
     it = it[np.argsort(it[:, 0], kind="stable")]
     counts = np.bincount(it[:, 0], minlength=ntiles)
     tile_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     return dict(items=np.ascontiguousarray(it[:, 1:]), tile_ptr=tile_ptr,
                 tiles=tiles,
-                units=np.ascontiguousarray(un),
+                units=units, members=members,
                 reduce=np.ascontiguousarray(np.array(reds, dtype=np.int64).reshape(-1, 4)),
                 nt=toff, npart=poff)
 
@@ -371,12 +408,14 @@ class HMatrix:
         self.ntiles = W["tiles"].shape[0]
         self.units = torch.from_numpy(W["units"]).to(dev)
         self.nunits = W["units"].shape[0]
+        self.members = torch.from_numpy(W["members"]).to(dev)
         self.reduce = torch.from_numpy(W["reduce"]).to(dev)
         self.nred = W["reduce"].shape[0]
         self.nt, self.npart = W["nt"], W["npart"]
         self._scratch = {}
         _lib.check_width("item", self.items)
         _lib.check_width("unit", self.units)
+        _lib.check_width("member", self.members)
         _lib.check_width("reduce", self.reduce)
         _lib.check_width("tile", self.tiles)
         # algorithmic bytes of one matvec: stored operators + 2 input + 3 output vectors
@@ -410,7 +449,8 @@ class HMatrix:
         if tuple(out.shape) != (3, n * nrhs) or not out.is_contiguous() or out.dtype != torch.float64:
             raise ValueError("out must be a contiguous (3, n*nrhs) float64 tensor")
         t, part = self._bufs(nrhs)
-        _lib.call("pq3d_hmat_matvec", _lib.ptr(self.units), self.nunits, _lib.ptr(self.reduce),
+        _lib.call("pq3d_hmat_matvec", _lib.ptr(self.units), self.nunits, _lib.ptr(self.members),
+                  _lib.ptr(self.reduce),
                   self.nred, _lib.ptr(self.tiles), _lib.ptr(self.tile_ptr), self.ntiles,
                   _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
                   _lib.ptr(v2), _lib.ptr(t), _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]),
@@ -468,15 +508,24 @@ class MultiHMatrix:
         pool_off = np.cumsum([0] + [int(h.pool.numel()) for h in hms])
         t_off = np.cumsum([0] + [int(h.nt) for h in hms])
         p_off = np.cumsum([0] + [int(h.npart) for h in hms])
-        units, reds, tiles, items, ptr = [], [], [], [], [np.zeros(1, dtype=np.int64)]
-        nitems = 0
+        units, mems, reds, tiles, items = [], [], [], [], []
+        ptr = [np.zeros(1, dtype=np.int64)]
+        nitems = nmem = 0
         for b, h in enumerate(hms):
             U = h.units.cpu().numpy().copy()
+            Mb = h.members.cpu().numpy().copy()
             if len(U):
-                U[:, 0] += pool_off[b]                                  # V^T base
+                # direct flag of every member's unit
+                direct = np.zeros(len(Mb), dtype=bool)
+                for mem0, cnt, dr in zip(U[:, 0], U[:, 1], U[:, 6]):
+                    direct[mem0:mem0 + cnt] = dr != 0
+                Mb[:, 0] += pool_off[b]                                 # V^T base
+                Mb[:, 2] += np.where(direct, t_off[b], p_off[b])        # output row
+                U[:, 0] += nmem                                         # first member row
                 U[:, 5] += self.off[b]                                  # block c0
-                U[:, 6] += np.where(U[:, 7] != 0, t_off[b], p_off[b])   # output row
+            nmem += len(Mb)
             units.append(U.reshape(-1, 8))
+            mems.append(Mb.reshape(-1, 4))
             R = h.reduce.cpu().numpy().copy()
             if len(R):
                 R[:, 0] += t_off[b]
@@ -498,14 +547,15 @@ class MultiHMatrix:
             np.ascontiguousarray(np.concatenate(xs, 0).reshape(-1, w).astype(np.int64))).to(dev)
         self.pool = torch.cat([h.pool for h in hms])
         self.units, self.nunits = cat(units, 8), int(sum(len(u) for u in units))
+        self.members = cat(mems, 4)
         self.reduce, self.nred = cat(reds, 4), int(sum(len(r) for r in reds))
         self.tiles, self.ntiles = cat(tiles, 2), int(sum(len(t) for t in tiles))
         self.items = cat(items, 6)
         self.tile_ptr = torch.from_numpy(np.concatenate(ptr).astype(np.int64)).to(dev)
         self.nt, self.npart = int(t_off[-1]), int(p_off[-1])
         self._scratch = {}
-        for name, tab in (("item", self.items), ("unit", self.units), ("reduce", self.reduce),
-                          ("tile", self.tiles)):
+        for name, tab in (("item", self.items), ("unit", self.units), ("member", self.members),
+                          ("reduce", self.reduce), ("tile", self.tiles)):
             _lib.check_width(name, tab)
         self.matvec_bytes = int(self.pool.numel()) * self.esize + 5 * 8 * self.n
 

@@ -118,7 +118,8 @@ class Checker:
                    + np.arange(3)[None, :, None] * ls[None, None, :] + (i - ro)[None, None, :])
             assert np.array_equal(np.sort(dst.ravel()), np.arange(3 * m * k))
 
-    def pq3d_hmat_matvec(self, units, nun, red, nred, tiles, tile_ptr, ntiles, items, pool,
+    def pq3d_hmat_matvec(self, units, nun, members, red, nred, tiles, tile_ptr, ntiles, items,
+                         pool,
                          esize, nrhs, v1, v2, t, part, y1, y2, y3, ybs, stream):
         P = self.T(pool).numel()
         assert nrhs in (1, 2, 4, 8, 16)
@@ -132,11 +133,19 @@ class Checker:
         if nun:
             un = rows(self.T(units), "unit")
             assert len(un) >= nun
-            for vt, k, cb, cc, bn, c0, out, direct in un[:nun]:
-                assert k > 0 and cc > 0 and 0 <= cb and cb + cc <= 2 * bn
-                assert vt >= 0 and vt + (cb + cc) * k <= P          # V^T rows cb..cb+cc
+            mb = rows(self.T(members), "member")
+            for mem0, nmem, cb, cc, bn, c0, direct, ktot in un[:nun]:
+                assert nmem >= 1 and 0 <= mem0 and mem0 + nmem <= len(mb)
+                assert cc > 0 and 0 <= cb and cb + cc <= 2 * bn
                 assert c0 >= 0 and c0 + bn <= N
-                assert out >= 0 and out + k <= (nt if direct else npart)
+                ms = mb[mem0:mem0 + nmem]
+                assert ktot == ms[:, 1].sum() and (nmem == 1 or ktot <= 64)
+                lanes = np.concatenate([[0], np.cumsum(ms[:-1, 1])])
+                assert np.array_equal(ms[:, 3], lanes)   # the kernel's lane -> member map
+                for vt, k, out, lane0 in ms:
+                    assert k > 0
+                    assert vt >= 0 and vt + (cb + cc) * k <= P      # V^T rows cb..cb+cc
+                    assert out >= 0 and out + k <= (nt if direct else npart)
         if nred:
             rd = rows(self.T(red), "reduce")
             for to, k, p0, nu in rd[:nred]:

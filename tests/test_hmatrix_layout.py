@@ -56,12 +56,13 @@ def emulate_matvec(W, pool, v1, v2, n, nb=1):
     t = np.zeros((max(W["nt"], 1), nb))
     part = np.zeros((max(W["npart"], 1), nb))
     X1, X2 = v1.reshape(n, nb), v2.reshape(n, nb)
-    for vt, k, cb, cc, bn, c0, out, direct in W["units"]:
-        acc = np.zeros((k, nb))
-        for c in range(cb, cb + cc):
-            x = X2[c0 + c - bn] if c >= bn else X1[c0 + c]
-            acc += np.outer(pool[vt + c * k: vt + c * k + k], x)
-        (t if direct else part)[out:out + k] = acc
+    for mem0, nmem, cb, cc, bn, c0, direct, ktot in W["units"]:
+        for vt, k, out, lane0 in W["members"][mem0:mem0 + nmem]:
+            acc = np.zeros((k, nb))
+            for c in range(cb, cb + cc):
+                x = X2[c0 + c - bn] if c >= bn else X1[c0 + c]
+                acc += np.outer(pool[vt + c * k: vt + c * k + k], x)
+            (t if direct else part)[out:out + k] = acc
     for to, k, p0, nu in W["reduce"]:
         t[to:to + k] = sum(part[p0 + q * k: p0 + q * k + k] for q in range(nu))
     y = np.zeros((3, n, nb))
@@ -175,12 +176,14 @@ def test_block_diagonal_union_reproduces_each_product(oracle):
         h = types.SimpleNamespace(
             n=n, perm=t.perm, dtype=torch.float64, device=torch.device("cpu"), esize=8,
             pool=torch.from_numpy(pool), units=torch.from_numpy(W["units"]),
+            members=torch.from_numpy(W["members"]),
             reduce=torch.from_numpy(W["reduce"]), tiles=torch.from_numpy(W["tiles"]),
             items=torch.from_numpy(W["items"]), tile_ptr=torch.from_numpy(W["tile_ptr"]),
             nt=W["nt"], npart=W["npart"])
         parts.append((h, K))
     Mu = hm.MultiHMatrix([h for h, _ in parts])
-    Wm = dict(units=Mu.units.numpy(), reduce=Mu.reduce.numpy(), tiles=Mu.tiles.numpy(),
+    Wm = dict(units=Mu.units.numpy(), members=Mu.members.numpy(), reduce=Mu.reduce.numpy(),
+              tiles=Mu.tiles.numpy(),
               items=Mu.items.numpy(), tile_ptr=Mu.tile_ptr.numpy(), nt=Mu.nt, npart=Mu.npart)
     rng = np.random.default_rng(3)
     v1, v2 = rng.standard_normal(Mu.n), rng.standard_normal(Mu.n)
@@ -222,3 +225,41 @@ def test_xcd_order_is_a_bijection():
         for x in range(min(g, 8)):
             mine = np.sort(w[x::8])
             assert np.array_equal(mine, np.arange(mine[0], mine[0] + len(mine)))
+
+
+def test_unit_packing_covers_every_block_chunk_once():
+    """Blocks sharing source columns share wavefronts: total rank <= 64 for a
+    unit of several members, lanes consecutive from 0, every (block, column
+    chunk) in exactly one unit, output rows direct for one chunk and partial
+    rows u*k apart otherwise; units reading the same columns are adjacent."""
+    from pyquake3d_amd.hmatrix import UNIT_COLS, pack_units
+    rng = np.random.default_rng(3)
+    groups, expect = {}, {}
+    toff = poff = 0
+    for g in range(40):
+        bc0, bn = 1000 * g, int(rng.choice([30, 200, 300, 700]))
+        for j in range(int(rng.integers(1, 9))):
+            k = int(rng.choice([3, 10, 17, 31, 40, 64, 90]))
+            vb = 10 ** 6 * g + 10 ** 4 * j
+            nu = -(-2 * bn // UNIT_COLS)
+            groups.setdefault((bc0, bn), []).append((vb, k, toff, poff))
+            expect[vb] = (k, toff, poff, nu)
+            toff += k
+            poff += nu * k if nu > 1 else 0
+    units, members = pack_units(groups)
+    seen = {}
+    for mem0, nmem, cb, cc, bn, c0, direct, ktot in units:
+        ms = members[mem0:mem0 + nmem]
+        assert ktot == ms[:, 1].sum() and (nmem == 1 or ktot <= 64)
+        assert np.array_equal(ms[:, 3], np.concatenate([[0], np.cumsum(ms[:-1, 1])]))
+        assert cb % UNIT_COLS == 0 and 0 < cc <= UNIT_COLS and cb + cc <= 2 * bn
+        for vb, k, out, lane0 in ms:
+            kk, to, po, nu = expect[vb]
+            u = cb // UNIT_COLS
+            assert k == kk and direct == (nu == 1)
+            assert out == (to if nu == 1 else po + u * k)
+            seen[(vb, u)] = seen.get((vb, u), 0) + 1
+    assert seen == {(vb, u): 1 for vb, (k, to, po, nu) in expect.items() for u in range(nu)}
+    key = units[:, 5] * 10 ** 6 + units[:, 2]
+    assert np.all(np.diff(key) >= 0)
+    assert len(units) < sum(nu for (k, to, po, nu) in expect.values())   # packing happened
