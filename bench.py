@@ -15,6 +15,9 @@ Workloads
                         same mesh stepped together, one multi-RHS H-matvec
                         per step; rate = sum over the draws
             ``value`` is the batched variant; all three are reported.
+  accuracy  Event times of the hstep problem with FP32 and with FP64
+            operator storage (one draw, device controller): the relative
+            event-time error behind "matched accuracy".
   ragged    --count simulations (default 64) on meshes of uneven size as
             one block-diagonal FP32 H-matrix with the device controller;
             per-instance accuracy of every 8th against its FP64 solo run.
@@ -412,7 +415,7 @@ def bench_ragged(args):
             one.step()
             one.sync()
             vmax = float(one.max_slipv()[0])
-            if armed and vmax > 1e-3:
+            if armed and vmax > args.vup:
                 events.append(float(one.time[0]))
                 armed = False
             elif vmax < 1e-4:
@@ -454,6 +457,73 @@ def bench_ragged(args):
         "sizes": [int(p.n) for p in probs], "build_s": build_s,
         "accepted_steps_min_max": [int(ens.naccepted.min()), int(ens.naccepted.max())],
         "accuracy_sampled": acc,
+    }
+
+
+# ------------------------------------------------------------- accuracy
+def bench_accuracy(args):
+    """Event times of the hstep problem (draw 0, N = 100,352) with FP32 and
+    with FP64 operator storage: the device controller, one accepted step per
+    iteration, max slip rate read after every step; an event starts when it
+    rises through --vup (re-armed below --vdown). Runs until --events events
+    or --tmax years or --steps steps, whichever comes first. With the
+    defaults (1e-8 / 3e-9 m/s) the events are the slow-slip transients this
+    fault produces (max V ~2e-8 m/s every ~140 years; no seismic rates within
+    600 years); --vup 1e-3 --vdown 1e-4 counts seismic events."""
+    import torch
+
+    from pyquake3d_amd.problem import undulating_c3
+    from pyquake3d_amd.qdsim import DeviceEnsemble
+
+    problem = undulating_c3()
+    runs = {}
+    for name, dtype in (("fp64", torch.float64), ("fp32", torch.float32)):
+        H, _ = build_hmatrix(problem, dtype)
+        ens = DeviceEnsemble([problem], H, device=H.device, rounds=8)
+        events, armed, peak = [], True, 0.0
+        w0 = time.perf_counter()
+        for i in range(args.steps):
+            ens.step()
+            ens.sync()
+            vmax = float(ens.max_slipv()[0])
+            peak = max(peak, vmax)
+            if armed and vmax > 1e-3:
+                events.append(float(ens.time[0]) / YEAR)
+                armed = False
+                log(f"accuracy {name}: event {len(events)} at {events[-1]:.4f} yr, step {i}")
+                if len(events) >= args.events:
+                    break
+            elif vmax < args.vdown:
+                armed = True
+            if ens.time[0] / YEAR > args.tmax:
+                break
+            if i % 5000 == 4999:
+                log(f"accuracy {name}: step {i + 1}, t = {ens.time[0] / YEAR:.2f} yr, "
+                    f"max V {vmax:.3e}")
+        wall = time.perf_counter() - w0
+        runs[name] = {"event_years": events, "steps": int(ens.naccepted[0]),
+                      "sim_years": float(ens.time[0]) / YEAR, "wall_s": wall,
+                      "peak_slip_rate": peak}
+        del H, ens
+        torch.cuda.empty_cache()
+    e64, e32 = runs["fp64"]["event_years"], runs["fp32"]["event_years"]
+    ne = min(len(e64), len(e32))
+    errs = [abs(a - b) / b for a, b in zip(e32[:ne], e64[:ne])]
+    return {
+        "metric": "event-time relative error, FP32 vs FP64 operator storage (N=1e5)",
+        "value": max(errs) if errs else None, "unit": "relative",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": 1e3 * runs["fp32"]["wall_s"] / max(runs["fp32"]["steps"], 1),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 vs f64 operator storage, f64 state",
+        "data": "synthetic undulating fault mesh and seeded friction fields",
+        "config": {"workload": f"event times, undulating fault 100x25 km, N={problem.n}, leaf 64, "
+                               f"eta 2, ACA eps 1e-4, rtol 1e-4, event = max V up through "
+                               f"{args.vup:g} m/s (re-armed below {args.vdown:g}), up to "
+                               f"{args.events} events / "
+                               f"{args.tmax} yr / {args.steps} steps (host reads max V every "
+                               f"step)"},
+        "event_time_rel_err": errs, "fp64": runs["fp64"], "fp32": runs["fp32"],
     }
 
 
@@ -533,13 +603,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="hstep", choices=["hstep", "assembly", "matvec", "learned", "ragged"])
+    ap.add_argument("--workload", default="hstep", choices=["hstep", "assembly", "matvec", "learned", "ragged",
+                                                   "accuracy"])
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"],
                     help="operator storage of the matvec workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nb", type=int, default=None, choices=[1, 2, 4, 8, 16],
                     help="friction draws in the batched variant (default 16; learned: 8)")
     ap.add_argument("--count", type=int, default=64, help="ragged: number of simulations")
+    ap.add_argument("--events", type=int, default=3, help="accuracy: events to reach")
+    ap.add_argument("--vup", type=float, default=1e-8, help="accuracy: event onset max V (m/s)")
+    ap.add_argument("--vdown", type=float, default=3e-9, help="accuracy: re-arm max V (m/s)")
+    ap.add_argument("--tmax", type=float, default=600.0, help="accuracy: simulated years cap")
     ap.add_argument("--record", type=int, default=800,
                     help="learned: steps of the reference controller logged for training")
     args = ap.parse_args()
@@ -553,6 +628,9 @@ def main():
         args.steps = 3000 if args.steps is None else args.steps
         args.warmup = 200 if args.warmup is None else args.warmup
         res = bench_hstep(args)
+    elif args.workload == "accuracy":
+        args.steps = 60000 if args.steps is None else args.steps
+        res = bench_accuracy(args)
     elif args.workload == "ragged":
         args.steps = 1000 if args.steps is None else args.steps
         args.warmup = 100 if args.warmup is None else args.warmup

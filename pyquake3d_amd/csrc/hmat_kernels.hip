@@ -389,6 +389,20 @@ template <int NB>
 constexpr bool p2_prefetch() {
   return NB <= PQ3D_P2_PF_NB;
 }
+// Right-hand sides per phase-2 wavefront for 8 and 16 right-hand sides: with
+// fewer than NB, each item group gets NB / NW wavefronts that split the
+// right-hand sides (fewer accumulator registers per wave, more waves per
+// tile, the operator loads repeated per wave).
+#ifndef PQ3D_P2_NW8
+#define PQ3D_P2_NW8 8
+#endif
+#ifndef PQ3D_P2_NW16
+#define PQ3D_P2_NW16 8
+#endif
+template <int NB>
+constexpr int p2_halves() {
+  return NB == 16 ? 16 / PQ3D_P2_NW16 : NB == 8 ? 8 / PQ3D_P2_NW8 : 1;
+}
 
 // Phase 1: t[l][b] = sum over the unit's stacked columns c of V^T[c][l] x_b[c]
 // (x = v1 for c < n, v2 for c >= n); one wavefront per unit, lane <-> (member
@@ -530,15 +544,17 @@ __device__ __forceinline__ void p2_load(const T* p, int64_t ld, int64_t plane, i
     for (int o = 0; o < 3; ++o) a[u][o] = ld_op(p + (c + u) * ld + o * plane + row);
 }
 
-template <typename T, int NB, int U>
+// x0: the weights of the group's first column; NB per column, of which this
+// wavefront uses NW (x0 already points at its first).
+template <typename T, int NB, int NW, int U>
 __device__ __forceinline__ void p2_madd(const T (&a)[U][3], const double* __restrict__ x0,
-                                        double (&acc)[3 * NB]) {
+                                        double (&acc)[3 * NW]) {
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const double a0 = (double)a[u][0], a1 = (double)a[u][1], a2 = (double)a[u][2];
     const double* x = x0 + u * NB;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
+    for (int b = 0; b < NW; ++b) {
       acc[3 * b + 0] = fma(a0, x[b], acc[3 * b + 0]);
       acc[3 * b + 1] = fma(a1, x[b], acc[3 * b + 1]);
       acc[3 * b + 2] = fma(a2, x[b], acc[3 * b + 2]);
@@ -549,35 +565,48 @@ __device__ __forceinline__ void p2_madd(const T (&a)[U][3], const double* __rest
 // Phase 2: one workgroup per row tile (a leaf cluster of <= 64 rows); lane <->
 // row. Every item is three row planes (outputs y1, y2, y3) of cnt columns
 // times weights w[xoff + j] (kind 0: v1, dense strike operators; kind 2: v2,
-// dense dip operators; kind 1: t, low-rank U). Waves split the tile's items;
-// partial sums meet in LDS in wave order. y1 = AdotV1, y2 = AdotV2,
+// dense dip operators; kind 1: t, low-rank U). The tile's items are dealt to
+// kTileWaves item groups (item q to group q mod kTileWaves); with H > 1 each
+// group has H wavefronts, each taking NB/H of the right-hand sides. Partial
+// sums meet in LDS in group order, so the sum for every right-hand side is
+// formed in the same order whatever NB is. y1 = AdotV1, y2 = AdotV2,
 // y3 = dsigma (K part only): element (row r, rhs b) at r*NB + b, or at
 // b*ybs + r when ybs > 0 (instance-major).
+#ifdef PQ3D_P2_WPE
+#define PQ3D_P2_ATTR __attribute__((amdgpu_waves_per_eu(PQ3D_P2_WPE)))
+#else
+#define PQ3D_P2_ATTR
+#endif
 template <typename T, int NB>
-__global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
+__global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATTR k_hmv_phase2(
     const int64_t* __restrict__ tiles, const int64_t* __restrict__ tile_ptr,
     const int64_t* __restrict__ items, const T* __restrict__ pool,
     const double* __restrict__ v1, const double* __restrict__ v2, const double* __restrict__ t,
     double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3, int64_t ybs) {
   constexpr int U = p2_unroll<NB>();
+  constexpr int H = p2_halves<NB>();
+  constexpr int NW = NB / H;  // right-hand sides per wavefront
+  static_assert(NW * H == NB && 64 * kTileWaves * H <= 1024, "bad split");
   __shared__ double red[3 * NB][64];
   const int64_t tile = xcd_order(blockIdx.x, gridDim.x);
   const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
   const int w = wave_id();
+  const int g = w % kTileWaves;  // item group
+  const int h = w / kTileWaves;  // right-hand-side group
   const int lane = threadIdx.x & 63;
   const bool live = lane < mt;
   // lanes past the tile's last row repeat that row (in bounds, never stored),
   // which keeps the item loop free of divergent branches
   const int row = live ? lane : (int)mt - 1;
-  double acc[3 * NB];
+  double acc[3 * NW];
 #pragma unroll
-  for (int q = 0; q < 3 * NB; ++q) acc[q] = 0.0;
-  for (int64_t q = tile_ptr[tile] + w; q < tile_ptr[tile + 1]; q += kTileWaves) {
+  for (int q = 0; q < 3 * NW; ++q) acc[q] = 0.0;
+  for (int64_t q = tile_ptr[tile] + g; q < tile_ptr[tile + 1]; q += kTileWaves) {
     const int64_t* it = items + q * IT_W;
     const int64_t kind = it[IT_KIND], base = it[IT_BASE], ld = it[IT_LD], plane = it[IT_PLANE];
     const int64_t cnt = it[IT_COUNT], xo = it[IT_XOFF];
     const T* p = pool + base;  // wave-uniform; lanes add their row
-    const double* xs = (kind == 0 ? v1 : kind == 1 ? t : v2) + xo * NB;
+    const double* xs = (kind == 0 ? v1 : kind == 1 ? t : v2) + xo * NB + h * NW;
     int64_t c = 0;
     if constexpr (p2_prefetch<NB>()) {
       // software pipeline: the next group's loads are in flight while the
@@ -589,7 +618,7 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
           const bool more = c + 2 * U <= cnt;
           T an[U][3];
           if (more) p2_load<T, U>(p, ld, plane, row, c + U, an);
-          p2_madd<T, NB, U>(a, xs + c * NB, acc);
+          p2_madd<T, NB, NW, U>(a, xs + c * NB, acc);
           c += U;
           if (!more) break;
 #pragma unroll
@@ -602,35 +631,32 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_hmv_phase2(
       for (; c + U <= cnt; c += U) {
         T a[U][3];
         p2_load<T, U>(p, ld, plane, row, c, a);
-        p2_madd<T, NB, U>(a, xs + c * NB, acc);
+        p2_madd<T, NB, NW, U>(a, xs + c * NB, acc);
       }
     }
     for (; c < cnt; ++c) {
-      const double a0 = (double)ld_op(p + c * ld + row);
-      const double a1 = (double)ld_op(p + c * ld + plane + row);
-      const double a2 = (double)ld_op(p + c * ld + 2 * plane + row);
-      const double* x = xs + c * NB;
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        acc[3 * b + 0] = fma(a0, x[b], acc[3 * b + 0]);
-        acc[3 * b + 1] = fma(a1, x[b], acc[3 * b + 1]);
-        acc[3 * b + 2] = fma(a2, x[b], acc[3 * b + 2]);
-      }
+      T a[1][3];
+      p2_load<T, 1>(p, ld, plane, row, c, a);
+      p2_madd<T, NB, NW, 1>(a, xs + c * NB, acc);
     }
   }
-  // deterministic cross-wave sum: wave 0 stores, waves 1.. add in order
-  for (int ww = 0; ww < kTileWaves; ++ww) {
-    if (w == ww) {
+  // deterministic cross-group sum: group 0 stores, groups 1.. add in order
+  for (int gg = 0; gg < kTileWaves; ++gg) {
+    if (g == gg) {
 #pragma unroll
-      for (int q = 0; q < 3 * NB; ++q) red[q][lane] = (ww == 0) ? acc[q] : red[q][lane] + acc[q];
+      for (int q = 0; q < 3 * NW; ++q) {
+        double* r = &red[3 * h * NW + q][lane];
+        *r = (gg == 0) ? acc[q] : *r + acc[q];
+      }
     }
     __syncthreads();
   }
-  if (w == 0 && live) {
+  if (g == 0 && live) {
     const int64_t r = R0 + lane;
     const int64_t sr = ybs > 0 ? 1 : NB, sb = ybs > 0 ? ybs : 1;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
+    for (int bb = 0; bb < NW; ++bb) {
+      const int b = h * NW + bb;
       y1[r * sr + b * sb] = red[3 * b + 0][lane];
       y2[r * sr + b * sb] = red[3 * b + 1][lane];
       y3[r * sr + b * sb] = red[3 * b + 2][lane];
@@ -651,9 +677,8 @@ static void launch_matvec(const int64_t* units, int64_t nunits, const int64_t* m
   if (nred > 0)
     k_hmv_phase1b<NB><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(red, nred, part, t);
   if (ntiles > 0)
-    k_hmv_phase2<T, NB><<<(unsigned)ntiles, 64 * kTileWaves, 0, s>>>(tiles, tile_ptr, items, pool,
-                                                                     v1, v2, t, y1, y2, y3,
-                                                                     ybs);
+    k_hmv_phase2<T, NB><<<(unsigned)ntiles, 64 * kTileWaves * p2_halves<NB>(), 0, s>>>(
+        tiles, tile_ptr, items, pool, v1, v2, t, y1, y2, y3, ybs);
 }
 
 }  // namespace pq3d
