@@ -415,7 +415,7 @@ def bench_ragged(args):
             one.step()
             one.sync()
             vmax = float(one.max_slipv()[0])
-            if armed and vmax > args.vup:
+            if armed and vmax > 1e-3:
                 events.append(float(one.time[0]))
                 armed = False
             elif vmax < 1e-4:
@@ -480,14 +480,16 @@ def bench_accuracy(args):
     for name, dtype in (("fp64", torch.float64), ("fp32", torch.float32)):
         H, _ = build_hmatrix(problem, dtype)
         ens = DeviceEnsemble([problem], H, device=H.device, rounds=8)
-        events, armed, peak = [], True, 0.0
+        # armed once max V is below --vdown (the initial nucleation patch
+        # starts above a slow-slip threshold)
+        events, armed, peak = [], False, 0.0
         w0 = time.perf_counter()
         for i in range(args.steps):
             ens.step()
             ens.sync()
             vmax = float(ens.max_slipv()[0])
             peak = max(peak, vmax)
-            if armed and vmax > 1e-3:
+            if armed and vmax > args.vup:
                 events.append(float(ens.time[0]) / YEAR)
                 armed = False
                 log(f"accuracy {name}: event {len(events)} at {events[-1]:.4f} yr, step {i}")

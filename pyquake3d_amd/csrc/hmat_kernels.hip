@@ -44,10 +44,10 @@ enum { UN_MEM = 0, UN_NMEM = 1, UN_CB = 2, UN_CC = 3, UN_N = 4, UN_C0 = 5, UN_DI
        UN_KTOT = 7, UN_W = 8 };
 // phase-1 unit member: V^T base, k, output row offset, first lane
 enum { MB_VT = 0, MB_K = 1, MB_OUT = 2, MB_LANE = 3, MB_W = 4 };
-// phase-1b reduction (blocks split over several units): t row offset, k,
-// first partial row, number of units (partial rows of unit u start at
-// first + u*k)
-enum { RD_T = 0, RD_K = 1, RD_P = 2, RD_NU = 3, RD_W = 4 };
+// phase-1b row (a rank chunk of a block split over several units): t row
+// offset, k, first partial row, number of units (partial rows of unit u start
+// at first + u*k), first rank, ranks
+enum { RD_T = 0, RD_K = 1, RD_P = 2, RD_NU = 3, RD_L0 = 4, RD_LC = 5, RD_W = 6 };
 // tile: first row, rows
 enum { TL_W = 2 };
 constexpr int kRecvW = 12;  // receiver record: centroid + ev1, ev2, ev3
@@ -506,7 +506,8 @@ __global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ 
 
 // Phase 1b: blocks split over several units: t = sum of their partial rows in
 // unit order (deterministic). Element e = l*NB + b of unit q sits at
-// part[(p0 + q*k)*NB + e]; loads are issued eight units at a time.
+// part[(p0 + q*k)*NB + e]; one wavefront per rank chunk [l0, l0 + lc) of a
+// block (about 64 elements); loads are issued eight units at a time.
 template <int NB>
 __global__ void __launch_bounds__(256) k_hmv_phase1b(const int64_t* __restrict__ red,
                                                      int64_t nred, const double* __restrict__ part,
@@ -516,8 +517,9 @@ __global__ void __launch_bounds__(256) k_hmv_phase1b(const int64_t* __restrict__
   if (r >= nred) return;
   const int64_t* it = red + r * RD_W;
   const int64_t to = it[RD_T], k = it[RD_K], p0 = it[RD_P], nu = it[RD_NU];
+  const int64_t l0 = it[RD_L0], lc = it[RD_LC];
   const int64_t qs = k * NB;
-  for (int64_t e = lane; e < qs; e += 64) {
+  for (int64_t e = l0 * NB + lane; e < (l0 + lc) * NB; e += 64) {
     const double* pp = part + p0 * NB + e;
     double s = 0.0;
     int64_t q = 0;

@@ -128,6 +128,24 @@ def unpack_u(flat, m, k, seg):
     return U.reshape(3 * m, k)
 
 
+def reduce_rows(reduce, nrhs):
+    """Phase-1b rows for ``nrhs`` right-hand sides: each reduction (t row, k,
+    first partial row, units) split into chunks of max(1, 64 // nrhs) ranks,
+    so that a wavefront sums about 64 elements whatever nrhs is. Row:
+    (t row, k, first partial row, units, first rank, ranks)."""
+    reduce = np.asarray(reduce, dtype=np.int64).reshape(-1, 4)
+    if len(reduce) == 0:
+        return np.zeros((0, 6), dtype=np.int64)
+    lc = max(1, 64 // nrhs)
+    k = reduce[:, 1]
+    nch = -(-k // lc)
+    rows = np.repeat(reduce, nch, axis=0)
+    first = np.concatenate([[0], np.cumsum(nch)[:-1]]).astype(np.int64)
+    l0 = (np.arange(int(nch.sum()), dtype=np.int64) - np.repeat(first, nch)) * lc
+    lcnt = np.minimum(lc, rows[:, 1] - l0)
+    return np.ascontiguousarray(np.column_stack([rows, l0, lcnt]).astype(np.int64))
+
+
 def pack_units(groups):
     """Phase-1 units from low-rank blocks grouped by source columns:
     ``groups[(c0, n)]`` lists (V base, k, t row, first partial row) per
@@ -186,7 +204,8 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
             V^T of a block is 2n x k row-major at its V base, U is in leaf
             panels (``pack_u``). Output rows are the block's t rows (direct)
             or its partial rows for this column chunk.
-    reduce  (R, 4): phase-1b sums (t row, k, first partial row, units)
+    reduce  (R, 4): phase-1b sums (t row, k, first partial row, units); the
+            kernel reads them as ``reduce_rows`` for the call's nrhs
     nt, npart: rows of t and of the partial buffer
     """
     tiles, tile_of_row = leaf_tables(tree)
@@ -416,16 +435,21 @@ class HMatrix:
         _lib.check_width("item", self.items)
         _lib.check_width("unit", self.units)
         _lib.check_width("member", self.members)
-        _lib.check_width("reduce", self.reduce)
         _lib.check_width("tile", self.tiles)
         # algorithmic bytes of one matvec: stored operators + 2 input + 3 output vectors
         self.matvec_bytes = int(self.pool.numel()) * self.esize + 5 * 8 * self.n
 
     def _bufs(self, nrhs):
+        """Per right-hand-side count: t and partial scratch, and the phase-1b
+        rows (one wavefront per row)."""
         if nrhs not in self._scratch:
+            red = reduce_rows(self.reduce.cpu().numpy(), nrhs)
+            red_d = torch.from_numpy(red).to(self.device)
+            _lib.check_width("reduce", red_d)
             self._scratch[nrhs] = (
                 torch.zeros(max(self.nt, 1) * nrhs, dtype=torch.float64, device=self.device),
-                torch.zeros(max(self.npart, 1) * nrhs, dtype=torch.float64, device=self.device))
+                torch.zeros(max(self.npart, 1) * nrhs, dtype=torch.float64, device=self.device),
+                red_d)
         return self._scratch[nrhs]
 
     # ----------------------------------------------------------------- apply
@@ -448,10 +472,10 @@ class HMatrix:
             raise ValueError("v1, v2 must hold n*nrhs values")
         if tuple(out.shape) != (3, n * nrhs) or not out.is_contiguous() or out.dtype != torch.float64:
             raise ValueError("out must be a contiguous (3, n*nrhs) float64 tensor")
-        t, part = self._bufs(nrhs)
+        t, part, red = self._bufs(nrhs)
         _lib.call("pq3d_hmat_matvec", _lib.ptr(self.units), self.nunits, _lib.ptr(self.members),
-                  _lib.ptr(self.reduce),
-                  self.nred, _lib.ptr(self.tiles), _lib.ptr(self.tile_ptr), self.ntiles,
+                  _lib.ptr(red), int(red.shape[0]), _lib.ptr(self.tiles), _lib.ptr(self.tile_ptr),
+                  self.ntiles,
                   _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
                   _lib.ptr(v2), _lib.ptr(t), _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]),
                   _lib.ptr(out[2]), n if (instance_major and nrhs > 1) else 0,
@@ -555,7 +579,7 @@ class MultiHMatrix:
         self.nt, self.npart = int(t_off[-1]), int(p_off[-1])
         self._scratch = {}
         for name, tab in (("item", self.items), ("unit", self.units), ("member", self.members),
-                          ("reduce", self.reduce), ("tile", self.tiles)):
+                          ("tile", self.tiles)):
             _lib.check_width(name, tab)
         self.matvec_bytes = int(self.pool.numel()) * self.esize + 5 * 8 * self.n
 

@@ -148,8 +148,10 @@ class Checker:
                     assert out >= 0 and out + k <= (nt if direct else npart)
         if nred:
             rd = rows(self.T(red), "reduce")
-            for to, k, p0, nu in rd[:nred]:
+            assert len(rd) >= nred
+            for to, k, p0, nu, l0, lc in rd[:nred]:
                 assert to + k <= nt and p0 >= 0 and p0 + nu * k <= npart
+                assert 0 <= l0 and lc >= 1 and l0 + lc <= k
         tl = rows(self.T(tiles), "tile")
         tp = self.T(tile_ptr).numpy()
         it = rows(self.T(items), "item")

@@ -263,3 +263,22 @@ def test_unit_packing_covers_every_block_chunk_once():
     key = units[:, 5] * 10 ** 6 + units[:, 2]
     assert np.all(np.diff(key) >= 0)
     assert len(units) < sum(nu for (k, to, po, nu) in expect.values())   # packing happened
+
+
+@pytest.mark.parametrize("nrhs", [1, 2, 4, 8, 16])
+def test_reduce_rows_cover_each_reduction_once(nrhs):
+    """Phase-1b rows: every rank of every reduction in exactly one row, rows of
+    max(1, 64 // nrhs) ranks, reduction fields copied."""
+    from pyquake3d_amd.hmatrix import reduce_rows
+    base = np.array([[0, 5, 0, 2], [5, 64, 10, 3], [69, 130, 202, 4], [199, 1, 722, 2]])
+    rows = reduce_rows(base, nrhs)
+    lc = max(1, 64 // nrhs)
+    seen = {}
+    for to, k, p0, nu, l0, cnt in rows:
+        match = [r for r in base if r[0] == to]
+        assert len(match) == 1 and tuple(match[0]) == (to, k, p0, nu)
+        assert 1 <= cnt <= lc and l0 % lc == 0 and l0 + cnt <= k
+        for l in range(l0, l0 + cnt):
+            seen[(to, l)] = seen.get((to, l), 0) + 1
+    assert seen == {(r[0], l): 1 for r in base for l in range(r[1])}
+    assert reduce_rows(np.zeros((0, 4), dtype=np.int64), nrhs).shape == (0, 6)
