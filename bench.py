@@ -53,7 +53,7 @@ YEAR = 3.15576e7
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of the matvec workload,
 # FETCH scaled by 2.0 and WRITE by 1.0: the factors the same counters read
 # for a 2 GiB device-to-device copy (profiles/hmatvec_traffic_N100352.md).
-MEASURED_TRAFFIC = {("f32", 1): 6_242_980_245, ("f32", 16): 7_433_304_896}
+MEASURED_TRAFFIC = {("f32", 1): 6_289_516_096, ("f32", 16): 7_465_723_168}
 
 # FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
 # (unit strike and dip slip, both projected), counted operator by operator
@@ -462,38 +462,54 @@ def bench_ragged(args):
 
 # ------------------------------------------------------------- accuracy
 def bench_accuracy(args):
-    """Event times of the hstep problem (draw 0, N = 100,352) with FP32 and
-    with FP64 operator storage: the device controller, one accepted step per
-    iteration, max slip rate read after every step; an event starts when it
-    rises through --vup (re-armed below --vdown). Runs until --events events
-    or --tmax years or --steps steps, whichever comes first. With the
-    defaults (1e-8 / 3e-9 m/s) the events are the slow-slip transients this
-    fault produces (max V ~2e-8 m/s every ~140 years; no seismic rates within
-    600 years); --vup 1e-3 --vdown 1e-4 counts seismic events."""
+    """Event times of the hstep problem (draw 0, N = 100,352; Dc scaled by
+    --dc-scale) with FP32 and with FP64 operator storage: the device
+    controller, one accepted step per iteration, max slip rate read after
+    every step; an event starts when it rises through --vup (armed once it
+    has been below --vdown). Runs until --events events (0: no cap), --tmax years,
+    --steps steps, or the reference's abort rule (dt below 1e-15 s or more
+    than 20 rejections, src/QDsim.py:956-958), whichever comes first. Every
+    FP64 event is paired with the nearest FP32 event.
+
+    With Dc as given (0.14 m mean) this fault produces slow-slip transients
+    (max V ~1e-8..2e-8 m/s) and no seismic rates within 600 years: use
+    --vup 1e-8 --vdown 3e-9 (the defaults). With --dc-scale 0.7 a seismic
+    event nucleates after ~2.4 years (--vup 1e-3 --vdown 1e-4); the
+    reference controller then aborts during the dynamic phase in either
+    precision."""
+    import numpy as np
     import torch
+    from dataclasses import replace
 
     from pyquake3d_amd.problem import undulating_c3
-    from pyquake3d_amd.qdsim import DeviceEnsemble
+    from pyquake3d_amd.qdsim import DeviceEnsemble, StepAborted
 
     problem = undulating_c3()
+    if args.dc_scale != 1.0:
+        problem = replace(problem, dc=problem.dc * args.dc_scale)
     runs = {}
     for name, dtype in (("fp64", torch.float64), ("fp32", torch.float32)):
         H, _ = build_hmatrix(problem, dtype)
         ens = DeviceEnsemble([problem], H, device=H.device, rounds=8)
         # armed once max V is below --vdown (the initial nucleation patch
         # starts above a slow-slip threshold)
-        events, armed, peak = [], False, 0.0
+        events, armed, peak, status = [], False, 0.0, "ok"
         w0 = time.perf_counter()
         for i in range(args.steps):
             ens.step()
-            ens.sync()
+            try:
+                ens.sync()
+            except StepAborted:
+                status = f"reference abort rule at {float(ens.time[0]) / YEAR:.4f} yr"
+                log(f"accuracy {name}: {status}")
+                break
             vmax = float(ens.max_slipv()[0])
             peak = max(peak, vmax)
             if armed and vmax > args.vup:
                 events.append(float(ens.time[0]) / YEAR)
                 armed = False
                 log(f"accuracy {name}: event {len(events)} at {events[-1]:.4f} yr, step {i}")
-                if len(events) >= args.events:
+                if 0 < args.events <= len(events):
                     break
             elif vmax < args.vdown:
                 armed = True
@@ -505,27 +521,32 @@ def bench_accuracy(args):
         wall = time.perf_counter() - w0
         runs[name] = {"event_years": events, "steps": int(ens.naccepted[0]),
                       "sim_years": float(ens.time[0]) / YEAR, "wall_s": wall,
-                      "peak_slip_rate": peak}
+                      "peak_slip_rate": peak, "status": status}
         del H, ens
         torch.cuda.empty_cache()
     e64, e32 = runs["fp64"]["event_years"], runs["fp32"]["event_years"]
-    ne = min(len(e64), len(e32))
-    errs = [abs(a - b) / b for a, b in zip(e32[:ne], e64[:ne])]
+    pairs = []
+    for t64 in e64:
+        if e32:
+            t32 = min(e32, key=lambda t: abs(t - t64))
+            pairs.append({"fp64_yr": t64, "fp32_yr": t32, "rel_err": abs(t32 - t64) / t64})
+    unmatched32 = [t for t in e32 if all(abs(t - p["fp32_yr"]) > 0 for p in pairs)]
     return {
         "metric": "event-time relative error, FP32 vs FP64 operator storage (N=1e5)",
-        "value": max(errs) if errs else None, "unit": "relative",
+        "value": max(p["rel_err"] for p in pairs) if pairs else None, "unit": "relative",
         "n_gpus": 1, "steps": args.steps, "warmup": 0,
         "ms_per_step": 1e3 * runs["fp32"]["wall_s"] / max(runs["fp32"]["steps"], 1),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 vs f64 operator storage, f64 state",
         "data": "synthetic undulating fault mesh and seeded friction fields",
         "config": {"workload": f"event times, undulating fault 100x25 km, N={problem.n}, leaf 64, "
-                               f"eta 2, ACA eps 1e-4, rtol 1e-4, event = max V up through "
-                               f"{args.vup:g} m/s (re-armed below {args.vdown:g}), up to "
-                               f"{args.events} events / "
-                               f"{args.tmax} yr / {args.steps} steps (host reads max V every "
-                               f"step)"},
-        "event_time_rel_err": errs, "fp64": runs["fp64"], "fp32": runs["fp32"],
+                               f"eta 2, ACA eps 1e-4, rtol 1e-4, Dc x {args.dc_scale:g}, event = "
+                               f"max V up through {args.vup:g} m/s (armed below "
+                               f"{args.vdown:g}), up to {args.events or 'any number of'} events / "
+                               f"{args.tmax} yr / "
+                               f"{args.steps} steps (host reads max V every step)"},
+        "pairs": pairs, "fp32_events_without_fp64_partner": unmatched32,
+        "fp64": runs["fp64"], "fp32": runs["fp32"],
     }
 
 
@@ -613,10 +634,12 @@ def main():
     ap.add_argument("--nb", type=int, default=None, choices=[1, 2, 4, 8, 16],
                     help="friction draws in the batched variant (default 16; learned: 8)")
     ap.add_argument("--count", type=int, default=64, help="ragged: number of simulations")
-    ap.add_argument("--events", type=int, default=3, help="accuracy: events to reach")
+    ap.add_argument("--events", type=int, default=0,
+                    help="accuracy: stop after this many events (0: run to --tmax)")
     ap.add_argument("--vup", type=float, default=1e-8, help="accuracy: event onset max V (m/s)")
     ap.add_argument("--vdown", type=float, default=3e-9, help="accuracy: re-arm max V (m/s)")
-    ap.add_argument("--tmax", type=float, default=600.0, help="accuracy: simulated years cap")
+    ap.add_argument("--tmax", type=float, default=150.0, help="accuracy: simulated years cap")
+    ap.add_argument("--dc-scale", type=float, default=1.0, help="accuracy: factor on Dc")
     ap.add_argument("--record", type=int, default=800,
                     help="learned: steps of the reference controller logged for training")
     args = ap.parse_args()
