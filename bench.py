@@ -15,6 +15,9 @@ Workloads
                         same mesh stepped together, one multi-RHS H-matvec
                         per step; rate = sum over the draws
             ``value`` is the batched variant; all three are reported.
+  example   The reference's BP5-QD example model (N = 9,250, half space)
+            from its initial state: dense FP64 vs FP64 / FP32 H-matrices,
+            event times and simulated years per wall-hour.
   accuracy  Event times of the hstep problem with FP32 and with FP64
             operator storage (one draw, device controller): the relative
             event-time error behind "matched accuracy".
@@ -550,6 +553,86 @@ def bench_accuracy(args):
     }
 
 
+# ------------------------------------------------------------- example
+def bench_example(args):
+    """The reference's BP5-QD example model (examples/BP5-QD: 9,250
+    triangles, half space, nucleation disc at 3e-2 m/s; committed as
+    tests/golden/bp5_example.npz) run from its initial state for --steps
+    accepted steps with the device controller, three ways: dense FP64
+    operators (the reference's plain method), FP64 and FP32 H-matrices (leaf
+    32, eta 2, ACA eps 1e-6). Max slip rate and time are copied on the
+    device after every step (no host synchronisation in the timed loop);
+    events are up-crossings of 1e-3 m/s (re-armed below 1e-4 m/s)."""
+    import numpy as np
+    import torch
+
+    from pyquake3d_amd.hmatrix import HMatrix
+    from pyquake3d_amd.problem import load_problem
+    from pyquake3d_amd.qdsim import DenseOperator, DeviceEnsemble, StepAborted
+
+    p, hs = load_problem(str(REPO / "tests" / "golden" / "bp5_example.npz"))
+    dev = torch.device("cuda:0")
+    builds = (("dense_fp64", lambda: DenseOperator(p.mesh, p.mu, p.lambda_, device=dev,
+                                                   halfspace=hs)),
+              ("hmatrix_fp64", lambda: HMatrix.from_mesh(p.mesh, p.mu, p.lambda_, leaf=32,
+                                                         eta=2.0, eps=1e-6, dtype=torch.float64,
+                                                         device=dev, halfspace=hs)),
+              ("hmatrix_fp32", lambda: HMatrix.from_mesh(p.mesh, p.mu, p.lambda_, leaf=32,
+                                                         eta=2.0, eps=1e-6, dtype=torch.float32,
+                                                         device=dev, halfspace=hs)))
+    runs = {}
+    for name, build in builds:
+        t0 = time.perf_counter()
+        op = build()
+        torch.cuda.synchronize()
+        build_s = time.perf_counter() - t0
+        ens = DeviceEnsemble([p], op, device=dev, rounds=8)
+        vh = torch.zeros(args.steps, dtype=torch.int64, device=dev)
+        th = torch.zeros(args.steps, dtype=torch.float64, device=dev)
+        w0 = time.perf_counter()
+        for i in range(args.steps):
+            ens.step()
+            vh[i].copy_(ens.vmax[0])
+            th[i].copy_(ens.tdev[0])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        status = "ok"
+        try:
+            ens.sync()
+        except StepAborted as e:
+            status = str(e)
+        v = vh.cpu().numpy().view(np.float64)
+        t = th.cpu().numpy() / YEAR
+        events, armed = [], True
+        for vi, ti in zip(v, t):
+            if armed and vi > 1e-3:
+                events.append(float(ti))
+                armed = False
+            elif vi < 1e-4:
+                armed = True
+        yrs = float(ens.time[0]) / YEAR
+        runs[name] = {"ms_per_step": 1e3 * wall / args.steps, "wall_s": wall, "sim_years": yrs,
+                      "years_per_hour": yrs / (wall / 3600.0), "accepted_steps":
+                          int(ens.naccepted[0]), "event_years": events,
+                      "peak_slip_rate": float(v.max()), "build_s": build_s, "status": status}
+        log(f"example {name}: {json.dumps(runs[name])}")
+        del op, ens
+        torch.cuda.empty_cache()
+    head = runs["hmatrix_fp32"]
+    return {
+        "metric": "sim-years/wall-hour, reference BP5-QD example (N=9250, half space)",
+        "value": head["years_per_hour"], "unit": "sim-years/wall-hour",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 H-matrix storage, f64 state (dense f64 and H f64 alongside)",
+        "data": "the reference's examples/BP5-QD model files (mesh, parameters), as archived",
+        "config": {"workload": f"BP5-QD example, N={p.n}, half space, {args.steps} accepted "
+                               f"steps from the model's initial state, device controller"},
+        "speedup_vs_dense_fp64": head["years_per_hour"] / runs["dense_fp64"]["years_per_hour"],
+        **runs,
+    }
+
+
 # ------------------------------------------------------------- assembly
 def bench_assembly(args):
     import numpy as np
@@ -627,7 +710,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="hstep", choices=["hstep", "assembly", "matvec", "learned", "ragged",
-                                                   "accuracy"])
+                                                   "accuracy", "example"])
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"],
                     help="operator storage of the matvec workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -653,6 +736,9 @@ def main():
         args.steps = 3000 if args.steps is None else args.steps
         args.warmup = 200 if args.warmup is None else args.warmup
         res = bench_hstep(args)
+    elif args.workload == "example":
+        args.steps = 20000 if args.steps is None else args.steps
+        res = bench_example(args)
     elif args.workload == "accuracy":
         args.steps = 60000 if args.steps is None else args.steps
         res = bench_accuracy(args)

@@ -65,7 +65,9 @@ class Problem:
         """(Tno, Tt1o, Tt2o, state, slipv1, slipv2, slipv) in MPa / m/s."""
         a, b, f0, V0 = self.a, self.b, self.f0, self.V0
         V, Tno = self.slipv, self.Tno
-        Tt = Tno * a * np.arcsinh(V / (2.0 * V0) * np.exp((f0 + b * np.log(V0 / self.Vpl)) / a))
+        Tt = self.meta.get("Tt")   # shear traction given by a stress tensor, else steady state
+        if Tt is None:
+            Tt = Tno * a * np.arcsinh(V / (2.0 * V0) * np.exp((f0 + b * np.log(V0 / self.Vpl)) / a))
         state = np.log(np.sinh(Tt / Tno / a) * 2.0 * V0 / V) * a
         Tt1o = Tt * np.cos(self.rake)
         Tt2o = Tt * np.sin(self.rake)
@@ -153,6 +155,35 @@ def undulating_c3(nx=448, nz=112, seed=0, nucleation=True, mu=32e9, nu=0.25, Vs=
         V = np.where((xg[:, 0] - cx) ** 2 + (xg[:, 2] - cz) ** 2 < (3e3) ** 2, 100 * Vpl, V)
     return Problem(mesh, mu, lam, mu / Vs ** 2, 1e-6, a, b, dc, f0, np.full(n, sigma), V,
                    np.zeros(n), np.full(n, Vpl), name=f"undulating_c3_N{n}")
+
+
+_ARRAYS = ("a", "b", "dc", "f0", "Tno", "slipv", "rake", "Vpl", "shear_loading",
+           "normal_loading")
+
+
+def save_problem(path, p: Problem, halfspace: bool = False) -> None:
+    """A model as one compressed .npz (mesh, per-element fields, scalars):
+    what ``load_problem`` reads back, for runs where the original model files
+    are not at hand."""
+    extra = {} if p.meta.get("Tt") is None else {"Tt": np.asarray(p.meta["Tt"])}
+    np.savez_compressed(
+        path, nodelst=p.mesh.nodelst, elelst=p.mesh.elelst.astype(np.int32),
+        scalars=np.array([p.mu, p.lambda_, p.density, p.V0, float(halfspace), float(p.fix_Tn)]),
+        name=np.array(p.name), **{k: getattr(p, k) for k in _ARRAYS}, **extra)
+
+
+def load_problem(path):
+    """(Problem, halfspace) from a ``save_problem`` archive."""
+    with np.load(path, allow_pickle=False) as z:
+        mesh = FaultMesh(np.array(z["nodelst"]), np.array(z["elelst"]).astype(np.int64),
+                         name=str(z["name"]))
+        mu, lam, rho, V0, hs, fix = (float(v) for v in z["scalars"])
+        arr = {k: np.array(z[k]) for k in _ARRAYS}
+        Tt = np.array(z["Tt"]) if "Tt" in z.files else None
+    p = Problem(mesh, mu, lam, rho, V0, arr["a"], arr["b"], arr["dc"], arr["f0"], arr["Tno"],
+                arr["slipv"], arr["rake"], arr["Vpl"], arr["shear_loading"],
+                arr["normal_loading"], fix_Tn=bool(fix), name=mesh.name, meta={"Tt": Tt})
+    return p, bool(hs)
 
 
 def with_friction(p: Problem, **kw) -> Problem:

@@ -35,9 +35,11 @@ class DenseOperator:
     """FP64 dense six-operator matrix (the reference's plain method):
     [[A1s, A1d], [A2s, A2d], [Bs, Bd]] (3N x 2N), applied with one GEMV."""
 
-    def __init__(self, mesh, mu, lambda_, device="cuda", dtype=torch.float64):
+    def __init__(self, mesh, mu, lambda_, device="cuda", dtype=torch.float64, halfspace=False,
+                 jud_ele_order=False):
         from .tde import DeviceMesh
-        dm = DeviceMesh.from_mesh(mesh, mu, lambda_, device=device)
+        dm = DeviceMesh.from_mesh(mesh, mu, lambda_, device=device, halfspace=halfspace,
+                                  jud_ele_order=jud_ele_order)
         K = dm.traction_block()
         n = dm.n
         self.n = n

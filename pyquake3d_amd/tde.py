@@ -113,8 +113,10 @@ class DeviceMesh:
                      else None)
 
     @classmethod
-    def from_mesh(cls, mesh: FaultMesh, mu, lambda_, device="cuda", halfspace=False):
-        return cls(mesh.nodelst, mesh.elelst, mu, lambda_, device, halfspace=halfspace)
+    def from_mesh(cls, mesh: FaultMesh, mu, lambda_, device="cuda", halfspace=False,
+                  jud_ele_order=False):
+        return cls(mesh.nodelst, mesh.elelst, mu, lambda_, device, jud_ele_order=jud_ele_order,
+                   halfspace=halfspace)
 
     def _index(self, idx):
         if idx is None:
