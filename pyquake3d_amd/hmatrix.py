@@ -489,6 +489,83 @@ class HMatrix:
     def from_cluster(self, x: torch.Tensor) -> torch.Tensor:
         return x[..., self.iperm_t]
 
+    # ------------------------------------------------------------- storage
+    FORMAT = "pq3d-hmatrix"
+    VERSION = 1
+    _BLOCK_KEYS = ("r0", "m", "c0", "n", "dense_idx", "d_base", "lr_idx", "ranks", "l_ub", "l_vb")
+
+    def save(self, path, **meta) -> None:
+        """Write the built operator (storage pool, work lists, permutation,
+        block tables) to one safetensors file with a versioned header: the
+        cache that replaces the reference's per-block pickles
+        (src/Hmatrix.py:1240-1241, src/QDsim.py:199-218). ``meta`` (strings)
+        is stored alongside, e.g. the model and build parameters."""
+        from safetensors.torch import save_file
+        cpu = lambda t: t.detach().to("cpu").contiguous()
+        tens = {"pool": cpu(self.pool), "units": cpu(self.units), "members": cpu(self.members),
+                "reduce": cpu(self.reduce), "tiles": cpu(self.tiles),
+                "tile_ptr": cpu(self.tile_ptr), "items": cpu(self.items),
+                "perm": torch.from_numpy(np.ascontiguousarray(self.perm, dtype=np.int64)),
+                "leaf_tiles": torch.from_numpy(np.ascontiguousarray(self.leaf_tiles)),
+                "tile_of_row": torch.from_numpy(np.ascontiguousarray(self.tile_of_row))}
+        for k in self._BLOCK_KEYS:
+            tens["block_" + k] = torch.from_numpy(np.ascontiguousarray(self.blocks[k],
+                                                                       dtype=np.int64))
+        st = self.stats
+        header = {"format": self.FORMAT, "version": str(self.VERSION), "n": str(self.n),
+                  "nt": str(self.nt), "npart": str(self.npart), "esize": str(self.esize),
+                  "halfspace": str(int(self.halfspace)), "mu": repr(self.mu),
+                  "lambda": repr(self.lambda_), "n_dense": str(st.n_dense),
+                  "n_lowrank": str(st.n_lowrank), "dense_entries": str(st.dense_entries),
+                  "lowrank_entries": str(st.lowrank_entries), "ranks_mean": repr(st.ranks_mean),
+                  "ranks_max": str(st.ranks_max), "fallback_dense": str(st.fallback_dense)}
+        header.update({"user_" + k: str(v) for k, v in meta.items()})
+        save_file(tens, str(path), metadata=header)
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        """An H-matrix written by ``save``, ready for ``matvec`` on
+        ``device`` (nothing is re-assembled). Raises ValueError for another
+        format or version."""
+        from safetensors import safe_open
+        with safe_open(str(path), framework="pt") as f:
+            h = f.metadata() or {}
+            if h.get("format") != cls.FORMAT or h.get("version") != str(cls.VERSION):
+                raise ValueError(f"{path}: not a {cls.FORMAT} v{cls.VERSION} file")
+            t = {k: f.get_tensor(k) for k in f.keys()}
+        self = cls.__new__(cls)
+        dev = torch.device(device)
+        self.device = dev
+        self.esize = int(h["esize"])
+        self.dtype = torch.float32 if self.esize == 4 else torch.float64
+        self.n, self.nt, self.npart = int(h["n"]), int(h["nt"]), int(h["npart"])
+        self.halfspace = h["halfspace"] == "1"
+        self.mu, self.lambda_ = float(h["mu"]), float(h["lambda"])
+        self.meta = {k[5:]: v for k, v in h.items() if k.startswith("user_")}
+        self.pool = t["pool"].to(dev)
+        for k in ("units", "members", "reduce", "tiles", "tile_ptr", "items"):
+            setattr(self, k, t[k].to(dev))
+        self.nunits, self.nred, self.ntiles = len(t["units"]), len(t["reduce"]), len(t["tiles"])
+        for name in ("unit", "member", "tile", "item"):
+            _lib.check_width(name, getattr(self, name + "s"))
+        self.perm = t["perm"].numpy()
+        self.iperm = np.empty_like(self.perm)
+        self.iperm[self.perm] = np.arange(len(self.perm))
+        self.perm_t = torch.from_numpy(self.perm).to(dev)
+        self.iperm_t = torch.from_numpy(self.iperm).to(dev)
+        self.leaf_tiles = t["leaf_tiles"].numpy()
+        self.tile_of_row = t["tile_of_row"].numpy()
+        self.blocks = {k: t["block_" + k].numpy() for k in cls._BLOCK_KEYS}
+        self.stats = HStats(n=self.n, n_dense=int(h["n_dense"]), n_lowrank=int(h["n_lowrank"]),
+                            dense_entries=int(h["dense_entries"]),
+                            lowrank_entries=int(h["lowrank_entries"]),
+                            ranks_mean=float(h["ranks_mean"]), ranks_max=int(h["ranks_max"]),
+                            fallback_dense=int(h["fallback_dense"]),
+                            bytes=int(self.pool.numel()) * self.esize, build_s={})
+        self._scratch = {}
+        self.matvec_bytes = int(self.pool.numel()) * self.esize + 5 * 8 * self.n
+        return self
+
     def lowrank_factors(self, q):
         """(U (3m x k), V (k x 2n)) of the q-th stored low-rank block, FP64 numpy."""
         B = self.blocks
