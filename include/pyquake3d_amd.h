@@ -126,6 +126,26 @@ int pq3d_tde_traction_block_hs_f32(const double* d_recv, const double* d_rec,
                                    double mu, double lambda_, float* d_out, int64_t si,
                                    int64_t sj, int64_t plane, int64_t out_len, void* stream);
 
+/* ------------------------------------------------------------ displacement
+ * Displacement of a triangular dislocation, full space (TDdispFS,
+ * src/DH_greenfunction.py:99-220) and half space (TDdispHS, :402-443: main
+ * + free-surface correction + image). The reference binds no C symbol for
+ * these (they are numpy functions); nu is Poisson's ratio. A side whose
+ * direction is vertical adds no correction, as in the published method. */
+/* TDdispFS / TDdispHS: n points x one triangle (d_hrec1 null -> full space).
+ * d_disp n x 3 row-major (east, north, up); NaN at points on a side of the
+ * triangle in its own plane, as the reference. */
+int pq3d_tde_disp_points(const double* d_X, const double* d_Y, const double* d_Z, int64_t n,
+                         const double* d_rec1, const double* d_hrec1, double Ss, double Ds,
+                         double Ts, double nu, double* d_disp, void* stream);
+/* get_sumS's displacement (src/QDsim.py:25-53) with per-triangle slip:
+ * d_out[3 i + c] = sum_j u_c(d_obs[3 i .. 3 i + 2]; triangle j, d_ss[j],
+ * d_ds[j], d_ts ? d_ts[j] : 0). d_obs nobs x 3; d_hrec null -> full space.
+ * Deterministic: the same inputs give the same bits. */
+int pq3d_tde_disp_sum(const double* d_obs, int64_t nobs, const double* d_rec,
+                      const double* d_hrec, int64_t ns, const double* d_ss, const double* d_ds,
+                      const double* d_ts, double nu, double* d_out, void* stream);
+
 /* ------------------------------------------------------------ H-matrix
  * Index space: cluster order (the cluster tree's permutation); blocks are
  * contiguous ranges. Pool element type: float (dtype_bytes 4) or double (8).

@@ -56,6 +56,11 @@ SIGNATURES = {
                                                c_int64, c_void_p, c_int64, c_int64, c_double,
                                                c_double, c_void_p, c_int64, c_int64, c_int64,
                                                c_int64, c_void_p]),
+    "pq3d_tde_disp_points": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                     c_double, c_double, c_double, c_double, c_void_p,
+                                     c_void_p]),
+    "pq3d_tde_disp_sum": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p,
+                                  c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     # H-matrix
     "pq3d_layout_constants": (c_int, [_L, c_int64]),
     "pq3d_hmat_dense_assemble": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,

@@ -13,6 +13,10 @@
 //  * *_hs variants             the same for the half space (free surface z = 0,
 //                              src/TDstressFS_C.cpp:1911-1999), with a second
 //                              per-triangle record (pq3d_tde_hs_records).
+//  * pq3d_tde_disp_points / pq3d_tde_disp_sum
+//                              displacement (src/DH_greenfunction.py TDdispFS,
+//                              TDdispHS): n points x 1 triangle, and points x
+//                              triangles summed with per-triangle slip.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdarg>
@@ -24,6 +28,7 @@
 #include "pq3d_common.h"
 #include "tde_device.h"
 #include "tde_hs_device.h"
+#include "tde_disp_device.h"
 
 namespace pq3d {
 
@@ -131,6 +136,77 @@ __global__ void __launch_bounds__(256) k_traction_block(
   }
 }
 
+// Displacement of one triangle at n points (TDdispFS / TDdispHS of
+// src/DH_greenfunction.py:99, :402); out n x 3 row-major (east, north, up).
+template <bool HS>
+__global__ void __launch_bounds__(256) k_disp_points(const double* __restrict__ X,
+                                                     const double* __restrict__ Y,
+                                                     const double* __restrict__ Z, int64_t n,
+                                                     const double* __restrict__ rec1,
+                                                     const double* __restrict__ hrec1, double Ss,
+                                                     double Ds, double Ts, double nu,
+                                                     double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r[kSrcRec];
+  load_rec(rec1, r);
+  double u[3];
+  if constexpr (HS)
+    tde_disp_point_hs(r, hrec1, X[i], Y[i], Z[i], Ss, Ds, Ts, nu, u);
+  else
+    tde_disp_point(r, X[i], Y[i], Z[i], Ss, Ds, Ts, nu, u);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) out[3 * i + k] = u[k];
+}
+
+constexpr int kDsWaves = 4;  // waves per workgroup of k_disp_sum
+
+// Displacement at nobs points summed over ns triangles, each with its own
+// slip: out[3 i + c] = sum_j u_c(obs_i; triangle j, ss[j], ds[j], ts[j])
+// (get_sumS, src/QDsim.py:25-53, with per-triangle slip). Lane <-> point (64
+// per workgroup); wave w takes triangles w, w + 4, ..., an index uniform in
+// the wave, so the records are read by scalar loads. The four partial sums
+// are added in wave order, which fixes the result bit for bit.
+template <bool HS>
+__global__ void __launch_bounds__(64 * kDsWaves) k_disp_sum(
+    const double* __restrict__ obs, int64_t nobs, const double* __restrict__ rec,
+    const double* __restrict__ hrec, int64_t ns, const double* __restrict__ ss,
+    const double* __restrict__ ds, const double* __restrict__ ts, double nu,
+    double* __restrict__ out) {
+  __shared__ double part[kDsWaves][3][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  const bool live = i < nobs;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (live) {
+    const double x = obs[3 * i], y = obs[3 * i + 1], z = obs[3 * i + 2];
+    for (int64_t j = w; j < ns; j += kDsWaves) {
+      const double Ts = ts ? ts[j] : 0.0;
+      double u[3];
+      if constexpr (HS)
+        tde_disp_point_hs(rec + kSrcRec * j, hrec + kHsRec * j, x, y, z, ss[j], ds[j], Ts, nu,
+                          u);
+      else
+        tde_disp_point(rec + kSrcRec * j, x, y, z, ss[j], ds[j], Ts, nu, u);
+      acc[0] += u[0];
+      acc[1] += u[1];
+      acc[2] += u[2];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) part[w][c][lane] = acc[c];
+  __syncthreads();
+  if (w == 0 && live) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double s = part[0][c][lane];
+      for (int k = 1; k < kDsWaves; ++k) s += part[k][c][lane];
+      out[3 * i + c] = s;
+    }
+  }
+}
+
 static inline double poisson(double mu, double lam) { return 1.0 / (1.0 + lam / mu) / 2.0; }
 
 // ---------------------------------------------------------------- host pool
@@ -204,7 +280,7 @@ static int stress_points_launch(const double* d_X, const double* d_Y, const doub
   PQ3D_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return PQ3D_OK;
   PQ3D_REQUIRE(d_X && d_Y && d_Z && d_rec1 && d_stress && d_strain, "null pointer");
-  PQ3D_REQUIRE(n <= (int64_t)1 << 40, "n too large");
+  PQ3D_REQUIRE(n <= (int64_t)1 << 31, "n too large (at most 2^31 receivers)");
   if (d_hrec1)
     k_stress_points<true><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
         d_X, d_Y, d_Z, n, d_rec1, d_hrec1, Ss, Ds, Ts, mu, lam, poisson(mu, lam), d_stress,
@@ -242,6 +318,7 @@ static int stress_sources_launch(double x, double y, double z, const double* d_r
   PQ3D_REQUIRE(ns >= 0, "n_sources must be >= 0");
   if (ns == 0) return PQ3D_OK;
   PQ3D_REQUIRE(d_rec && d_stress && d_strain, "null pointer");
+  PQ3D_REQUIRE(ns <= (int64_t)1 << 31, "n_sources too large (at most 2^31)");
   if (d_hrec)
     k_stress_sources<true><<<blocks_for(ns, 256), 256, 0, as_stream(stream)>>>(
         x, y, z, d_rec, d_hrec, ns, Ss, Ds, Ts, mu, lam, poisson(mu, lam), d_stress, d_strain);
@@ -361,6 +438,48 @@ int pq3d_tde_traction_block_hs_f32(const double* d_recv, const double* d_rec,
   PQ3D_REQUIRE(d_hrec, "null half-space records");
   return traction_block_launch(d_recv, d_rec, d_hrec, d_rows, row0, m, d_cols, col0, n, mu, lam,
                                d_out, si, sj, plane, out_len, 1, stream);
+}
+
+// ------------------------------------------------------------ displacement
+
+int pq3d_tde_disp_points(const double* d_X, const double* d_Y, const double* d_Z, int64_t n,
+                         const double* d_rec1, const double* d_hrec1, double Ss, double Ds,
+                         double Ts, double nu, double* d_disp, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0, "n must be >= 0");
+  if (n == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_X && d_Y && d_Z && d_rec1 && d_disp, "null pointer");
+  PQ3D_REQUIRE(n <= (int64_t)1 << 31, "n too large (at most 2^31 points)");
+  PQ3D_REQUIRE(nu > -1.0 && nu < 0.5, "Poisson's ratio must lie in (-1, 0.5)");
+  if (d_hrec1)
+    k_disp_points<true><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
+        d_X, d_Y, d_Z, n, d_rec1, d_hrec1, Ss, Ds, Ts, nu, d_disp);
+  else
+    k_disp_points<false><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
+        d_X, d_Y, d_Z, n, d_rec1, nullptr, Ss, Ds, Ts, nu, d_disp);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_tde_disp_sum(const double* d_obs, int64_t nobs, const double* d_rec,
+                      const double* d_hrec, int64_t ns, const double* d_ss, const double* d_ds,
+                      const double* d_ts, double nu, double* d_out, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(nobs >= 0 && ns >= 0, "sizes must be >= 0");
+  if (nobs == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_obs && d_out, "null pointer");
+  PQ3D_REQUIRE(ns == 0 || (d_rec && d_ss && d_ds), "null source pointer");
+  PQ3D_REQUIRE(nobs <= (int64_t)1 << 30, "nobs too large (at most 2^30 points)");
+  PQ3D_REQUIRE(nu > -1.0 && nu < 0.5, "Poisson's ratio must lie in (-1, 0.5)");
+  const unsigned int g = blocks_for(nobs, 64);
+  if (d_hrec)
+    k_disp_sum<true><<<g, 64 * kDsWaves, 0, as_stream(stream)>>>(d_obs, nobs, d_rec, d_hrec, ns,
+                                                                 d_ss, d_ds, d_ts, nu, d_out);
+  else
+    k_disp_sum<false><<<g, 64 * kDsWaves, 0, as_stream(stream)>>>(d_obs, nobs, d_rec, nullptr, ns,
+                                                                  d_ss, d_ds, d_ts, nu, d_out);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
 }
 
 // ------------------------------------------------ drop-in host entry points

@@ -10,6 +10,9 @@ Two layers:
   traction operators of any (rows x cols) block in one launch. This replaces
   the per-column / per-row ctypes loops of the reference evaluators
   (src/Hmatrix.py:318-511) and their Python traction projection (:295-313).
+* ``TDdispFS`` / ``TDdispHS`` and ``DeviceMesh.displacement`` -- displacement
+  of triangular dislocations (src/DH_greenfunction.py:99, :402) and its sum
+  over the mesh with per-element slip (get_sumS, src/QDsim.py:25-53).
 """
 from __future__ import annotations
 
@@ -63,6 +66,38 @@ def TDstressEachSourceAtReceiver_C(x, y, z, P1_flat, P2_flat, P3_flat, Ss, Ds, T
     if err:
         raise RuntimeError(err.decode())
     return stress.reshape(ns, 6), strain.reshape(ns, 6)
+
+
+def _disp(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, nu, halfspace, device):
+    dev = torch.device(device)
+    X, Y, Z = (np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (X, Y, Z))
+    n = X.shape[0]
+    if Y.shape[0] != n or Z.shape[0] != n:
+        raise ValueError("X, Y and Z must have the same length")
+    tri = np.concatenate([np.asarray(P, dtype=np.float64).ravel() for P in (P1, P2, P3)])
+    if tri.shape[0] != 9:
+        raise ValueError("P1, P2 and P3 must each hold 3 coordinates")
+    rec = torch.from_numpy(source_records(tri)).to(dev)
+    hrec = torch.from_numpy(hs_records(tri)).to(dev) if halfspace else None
+    dX, dY, dZ = (torch.from_numpy(v).to(dev) for v in (X, Y, Z))
+    out = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    _lib.call("pq3d_tde_disp_points", _lib.ptr(dX), _lib.ptr(dY), _lib.ptr(dZ), n,
+              _lib.ptr(rec), _lib.ptr(hrec), float(Ss), float(Ds), float(Ts), float(nu),
+              _lib.ptr(out), _lib.stream_handle())
+    u = out.cpu().numpy()
+    return u[:, 0].copy(), u[:, 1].copy(), u[:, 2].copy()
+
+
+def TDdispFS(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, nu, device="cuda"):
+    """Full-space displacement (ue, un, uv) at points (X, Y, Z) from one
+    triangle with strike, dip and tensile slip (src/DH_greenfunction.py:99)."""
+    return _disp(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, nu, False, device)
+
+
+def TDdispHS(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, nu, device="cuda"):
+    """Half-space displacement (ue, un, uv), free surface z = 0, points and
+    triangle at z <= 0 (src/DH_greenfunction.py:402)."""
+    return _disp(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, nu, True, device)
 
 
 def source_records(tri: np.ndarray) -> np.ndarray:
@@ -161,4 +196,38 @@ class DeviceMesh:
         _lib.call(fn, _lib.ptr(self.recv), _lib.ptr(self.rec), *hs, _lib.ptr(r), row0, m,
                   _lib.ptr(c), col0, n, self.mu, self.lambda_, _lib.ptr(out), si, sj, m * n,
                   out.numel(), _lib.stream_handle(stream))
+        return out
+
+    @property
+    def nu(self) -> float:
+        return 1.0 / (1.0 + self.lambda_ / self.mu) / 2.0
+
+    def displacement(self, obs, ss, ds, ts=None, *, nu=None, out=None, stream=None):
+        """Displacement (nobs, 3: east, north, up) at points ``obs`` (nobs x 3)
+        summed over every element with slip ``ss``, ``ds`` and opening ``ts``
+        (length-N arrays or scalars; ``ts`` None means no opening), in the
+        mesh's full or half space (get_sumS, src/QDsim.py:25-53, sums unit
+        dip slip with nu = 0.25). ``nu`` defaults to the mesh's Poisson ratio."""
+        def dev(a, name):
+            t = torch.as_tensor(a, dtype=torch.float64)
+            if t.ndim == 0:
+                t = t.expand(self.n)
+            if t.shape != (self.n,):
+                raise ValueError(f"{name} must be a scalar or have one value per element")
+            return t.to(self.device).contiguous()
+        p = torch.as_tensor(obs, dtype=torch.float64).to(self.device).contiguous()
+        if p.ndim != 2 or p.shape[1] != 3:
+            raise ValueError("obs must be (nobs, 3)")
+        nobs = p.shape[0]
+        s, d = dev(ss, "ss"), dev(ds, "ds")
+        t = dev(ts, "ts") if ts is not None else None
+        if out is None:
+            out = torch.empty((nobs, 3), dtype=torch.float64, device=self.device)
+        if tuple(out.shape) != (nobs, 3) or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 (nobs, 3) tensor")
+        _lib.check_width("rec", self.rec)
+        _lib.call("pq3d_tde_disp_sum", _lib.ptr(p), nobs, _lib.ptr(self.rec),
+                  _lib.ptr(self.hrec), self.n, _lib.ptr(s), _lib.ptr(d), _lib.ptr(t),
+                  float(self.nu if nu is None else nu), _lib.ptr(out),
+                  _lib.stream_handle(stream))
         return out

@@ -69,3 +69,20 @@ extern "C" void host_tde_block_hs(const double* recv, int64_t m, const double* t
     }
   }
 }
+
+#include "tde_disp_device.h"
+
+extern "C" void host_tde_disp_points(const double* pts, int64_t n, const double* tri9, double Ss,
+                                     double Ds, double Ts, double nu, int hs, double* disp) {
+  double rec[kSrcRec], h[kHsRec];
+  tde_source_record_host(tri9, tri9 + 3, tri9 + 6, rec);
+  if (hs) tde_hs_record_host(tri9, tri9 + 3, tri9 + 6, h);
+  for (int64_t i = 0; i < n; ++i) {
+    if (hs)
+      tde_disp_point_hs(rec, h, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], Ss, Ds, Ts, nu,
+                        disp + 3 * i);
+    else
+      tde_disp_point(rec, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], Ss, Ds, Ts, nu,
+                     disp + 3 * i);
+  }
+}

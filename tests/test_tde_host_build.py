@@ -29,6 +29,8 @@ def host_lib(tmp_path_factory):
     lib.host_tde_points_hs.argtypes = [D, ctypes.c_int64, D] + [ctypes.c_double] * 5 + [D, D]
     lib.host_tde_block_hs.argtypes = [D, ctypes.c_int64, D, ctypes.c_int64, ctypes.c_double,
                                       ctypes.c_double, D]
+    lib.host_tde_disp_points.argtypes = [D, ctypes.c_int64, D] + [ctypes.c_double] * 4 + \
+        [ctypes.c_int, D]
     return lib
 
 
@@ -104,3 +106,30 @@ def test_host_build_halfspace_block(host_lib, golden):
     host_lib.host_tde_block_hs(p(recv), n, p(tri), n, 32e9, 32e9, p(out))
     scale = np.abs(K).max(axis=(0, 2))[None, :, None]
     assert (np.abs(out - K) / scale).max() < 1e-10
+
+
+@pytest.mark.parametrize("hs", [0, 1])
+def test_host_build_displacement_matches_oracle(host_lib, hs):
+    """The device displacement arithmetic (full and half space) against the
+    displacement oracle, to 1e-10 of each point's largest component plus
+    1e-13 |b| (far from a small triangle the displacement is a small
+    difference of terms of size |b|, so rounding there is relative to |b|),
+    on random buried triangles (every fifth with a vertical side) and points
+    down to the free surface."""
+    from oracle.tde_disp_oracle import td_disp_fs, td_disp_hs
+    rng = np.random.default_rng(11 + hs)
+    for t in range(20):
+        c = np.array([rng.uniform(-1e3, 1e3), rng.uniform(-1e3, 1e3), -rng.uniform(200, 3000)])
+        P = c + rng.uniform(-800, 800, (3, 3))
+        P[:, 2] = np.minimum(P[:, 2], -10.0)
+        if t % 5 == 0:
+            P[1, :2] = P[0, :2]
+        pts = c + rng.uniform(-3e3, 3e3, (100, 3))
+        pts[:, 2] = np.minimum(pts[:, 2], 0.0)
+        b = rng.normal(size=3)
+        out = np.zeros((100, 3))
+        host_lib.host_tde_disp_points(p(np.ascontiguousarray(pts)), 100,
+                                      p(np.ascontiguousarray(P.ravel())), *b, 0.25, hs, p(out))
+        ref = (td_disp_hs if hs else td_disp_fs)(pts, P[0], P[1], P[2], *b, 0.25)
+        tol = 1e-10 * np.abs(ref).max(1) + 1e-13 * np.linalg.norm(b)
+        assert (np.abs(out - ref).max(1) <= tol).all()
