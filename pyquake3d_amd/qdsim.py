@@ -258,12 +258,41 @@ class QDsim:
                     slipv=g(self.slipv), rake=g(self.rake), slip1=g(self.slip1),
                     slip2=g(self.slip2))
 
+    SNAPSHOT_FIELDS = ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "rake",
+                       "slip1", "slip2")
+
+    def restore(self, snap: dict) -> None:
+        """Continue from a snapshot in original element order (``snapshot()``,
+        or ``writers.restart_fields`` of a VTK file, as the reference's
+        restart from a VTK file, src/QDsim.py:656-682). The simulated time is
+        kept unless the snapshot holds one."""
+        for k in self.SNAPSHOT_FIELDS:
+            x = np.asarray(snap[k], dtype=np.float64)
+            if x.shape != (self.n,):
+                raise ValueError(f"snapshot field {k} must have one value per element")
+            getattr(self, k).copy_(torch.from_numpy(np.ascontiguousarray(x[self.perm])))
+        if "time" in snap:
+            self.time = float(snap["time"])
+        self.maxslipv0 = float(np.max(snap["slipv"]))
+
+    def ouputVTK(self, fname) -> None:
+        """The current state as the reference's VTK file (QDsim.ouputVTK,
+        src/QDsim.py:1142-1289; the method keeps the reference's name)."""
+        from .writers import vtk_cells, write_vtk
+        write_vtk(fname, self.p.mesh.nodelst, self.p.mesh.elelst,
+                  vtk_cells(self.snapshot(), self.p))
+
     def run(self, nsteps: int, dttry=None, log=None, vmax_every=1):
-        """Run ``nsteps`` accepted steps; returns rows (step, h, t, maxV)."""
+        """Run ``nsteps`` accepted steps; returns rows (step, h, t, maxV).
+        ``log``: a ``writers.StateLog`` that gets the reference's state.txt
+        line of every step (src/main.py:177)."""
         dt = self.htry if dttry is None else dttry
         rows = []
         for i in range(nsteps):
             h, dt = self.simu_forward(dt)
+            if log is not None:
+                log.step(self.nsteps - 1, h, float(self.slipv1.abs().max()),
+                         float(self.slipv2.abs().max()), self.time)
             if vmax_every and (i % vmax_every == 0):
                 rows.append((self.nsteps, h, self.time, self.max_slipv()))
         self.htry = dt

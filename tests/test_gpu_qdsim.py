@@ -285,3 +285,37 @@ def test_ragged_ensemble_matches_solo_runs(gpu, pq3d_lib):
                                 dtype=torch.float64, device=gpu)
         t64, _ = solo(p, H64)
         assert abs(snaps[k][0] - t64) <= 1e-3 * t64, (k, snaps[k][0], t64)
+
+
+def test_restore_and_vtk_restart(gpu, pq3d_lib, small, tmp_path):
+    """Restore from a snapshot continues bit for bit; restart from the
+    reference's VTK file (6-decimal tractions and state) continues within
+    the rounding of that format; the state log gets one line per step."""
+    from pyquake3d_amd import writers
+    p, _ = small
+    op = DenseOperator(p.mesh, p.mu, p.lambda_, device=gpu)
+    a = QDsim(p, op, device=gpu)
+    log = writers.StateLog(tmp_path / "state.txt", p)
+    a.run(40, log=log)
+    log.close(0.0)
+    lines = (tmp_path / "state.txt").read_text().split("\n")
+    steps = [ln for ln in lines[15:] if ln and not ln.startswith(("Program", "Time"))]
+    assert len(steps) == 40 and steps[0].startswith("0 ") and steps[-1].startswith("39 ")
+    snap = a.snapshot()
+    a.ouputVTK(tmp_path / "step40.vtk")
+    b = QDsim(p, op, device=gpu)
+    b.restore(snap)
+    c = QDsim(p, op, device=gpu)
+    c.restore(writers.restart_fields(writers.read_vtk(tmp_path / "step40.vtk")["cells"]))
+    c.time = a.time
+    dt = a.htry
+    for _ in range(10):
+        ha, _ = a.simu_forward(dt)
+        hb, _ = b.simu_forward(dt)
+        hc, _ = c.simu_forward(dt)
+        assert ha == hb
+    sa, sb, sc = a.snapshot(), b.snapshot(), c.snapshot()
+    for k in ("slipv", "Tt1o", "state", "slip1"):
+        assert np.array_equal(sa[k], sb[k]), k
+    assert np.abs(sc["Tt1o"] - sa["Tt1o"]).max() <= 1e-5
+    assert np.abs(np.log(sc["slipv"] / sa["slipv"])).max() <= 1e-2
