@@ -146,6 +146,32 @@ int pq3d_tde_disp_sum(const double* d_obs, int64_t nobs, const double* d_rec,
                       const double* d_hrec, int64_t ns, const double* d_ss, const double* d_ds,
                       const double* d_ts, double nu, double* d_out, void* stream);
 
+/* ------------------------------------------------------- cluster tree
+ * Cluster tree and block partition on the device, one level per call
+ * (src/Hmatrix.py:133-182 build_block_tree, :58-103 admissibility,
+ * :1611-1672 create_recursive_blocks). The same comparisons and IEEE
+ * expressions as the host restatement: identical trees and partitions. */
+/* One tree level. d_pts n x 3 (original order); d_perm (n) holds the current
+ * order and is partitioned in place (d_tmp: n scratch). d_nodes nnodes x 2 =
+ * (first index, points). Out: d_box nnodes x 6 = (lo, hi) of each node's
+ * points; d_nl[q] = points of the left child, or -1 when node q is not split
+ * (at most `leaf` points, or one side empty). */
+int pq3d_tree_level(const double* d_pts, int64_t n, int64_t* d_perm, int64_t* d_tmp,
+                    const int64_t* d_nodes, int64_t nnodes, int64_t leaf, double* d_box,
+                    int64_t* d_nl, void* stream);
+/* One block-tree level. d_pairs npairs x 3 = (row node, col node, path);
+ * d_node nodes x 4 = (first, points, left, right; -1 for a leaf node);
+ * d_box nodes x 6. Admissible pairs and pairs with a leaf node or a node of
+ * at most min_size points go to d_leafs (x 4: row, col, admissible, path),
+ * the others put their four children in d_next (x 3); d_counts[0..1] (zero
+ * on entry) count both. Capacity: npairs leafs, 4 npairs next. The path of a
+ * child has its index 0..3 in bits 2 (30 - depth) (order (ll, lr, rl, rr));
+ * depth < 30. */
+int pq3d_block_level(const int64_t* d_pairs, int64_t npairs, const int64_t* d_node,
+                     const double* d_box, double eta, int64_t min_size, int depth,
+                     int64_t* d_leafs, int64_t* d_next, unsigned long long* d_counts,
+                     void* stream);
+
 /* ------------------------------------------------------------ H-matrix
  * Index space: cluster order (the cluster tree's permutation); blocks are
  * contiguous ranges. Pool element type: float (dtype_bytes 4) or double (8).

@@ -61,6 +61,10 @@ SIGNATURES = {
                                      c_void_p]),
     "pq3d_tde_disp_sum": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p,
                                   c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "pq3d_tree_level": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
+                                c_int64, c_void_p, c_void_p, c_void_p]),
+    "pq3d_block_level": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_double, c_int64,
+                                 c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     # H-matrix
     "pq3d_layout_constants": (c_int, [_L, c_int64]),
     "pq3d_hmat_dense_assemble": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,

@@ -24,6 +24,7 @@ SOURCES = [
     "hmat_kernels.hip",
     "rk_kernels.hip",
     "mlp_kernels.hip",
+    "tree_kernels.hip",
 ]
 
 ARCH = os.environ.get("PQ3D_OFFLOAD_ARCH", "gfx950")

@@ -11,12 +11,22 @@ The block recursion follows ``create_recursive_blocks`` (src/Hmatrix.py:1611-167
 Differences by design (MI355X storage): the tree is built over a permutation
 so that every cluster is a contiguous index range; all device data and
 vectors live in that permuted order.
+
+``build_cluster_tree_device`` / ``build_block_partition_device`` build the
+same tree and partition on the GPU, one level per launch
+(csrc/tree_kernels.hip), bit for bit equal to the host functions here; the
+H-matrix uses them. Device trees number their nodes level by level (root 0,
+the children of the j-th split node at 2j + 1 and 2j + 2); ranges, boxes and
+the block order are those of the host functions.
 """
 from __future__ import annotations
 
 from dataclasses import dataclass, field
 
 import numpy as np
+import torch
+
+from . import _lib
 
 
 @dataclass
@@ -125,3 +135,86 @@ def build_block_partition(t: ClusterTree, eta: float = 2.0,
         stack.extend([(ra, rb), (ra, lb), (la, rb), (la, lb)])
     return BlockPartition(np.array(rows, np.int64), np.array(cols, np.int64),
                           np.array(adm, bool), t)
+
+
+# ------------------------------------------------------------------ device
+def _tree_level(pts, perm, tmp, nodes, leaf, box, nl):
+    _lib.call("pq3d_tree_level", _lib.ptr(pts), int(pts.shape[0]), _lib.ptr(perm),
+              _lib.ptr(tmp), _lib.ptr(nodes), int(nodes.shape[0]), int(leaf), _lib.ptr(box),
+              _lib.ptr(nl), _lib.stream_handle())
+
+
+def _block_level(pairs, node, box, eta, min_size, depth, leafs, nxt, counts):
+    _lib.call("pq3d_block_level", _lib.ptr(pairs), int(pairs.shape[0]), _lib.ptr(node),
+              _lib.ptr(box), float(eta), int(min_size), int(depth), _lib.ptr(leafs),
+              _lib.ptr(nxt), _lib.ptr(counts), _lib.stream_handle())
+
+
+def build_cluster_tree_device(points: np.ndarray, leaf: int = 64, device="cuda") -> ClusterTree:
+    """``build_cluster_tree`` on the GPU: one launch per tree level (each
+    node's box, split and stable partition of the permutation); the host
+    reads back only the level's boxes and left-child sizes."""
+    dev = torch.device(device)
+    pts = torch.from_numpy(np.ascontiguousarray(points, dtype=np.float64)).to(dev)
+    if pts.ndim != 2 or pts.shape[1] != 3:
+        raise ValueError("points must be (n, 3)")
+    n = int(pts.shape[0])
+    if n == 0:
+        raise ValueError("no points")
+    perm = torch.arange(n, dtype=torch.int64, device=dev)
+    tmp = torch.empty_like(perm)
+    level = np.array([[0, n]], dtype=np.int64)
+    parts = []
+    while len(level):
+        nodes = torch.from_numpy(level).to(dev)
+        box = torch.empty((len(level), 6), dtype=torch.float64, device=dev)
+        nl = torch.empty(len(level), dtype=torch.int64, device=dev)
+        _tree_level(pts, perm, tmp, nodes, leaf, box, nl)
+        b, k = box.cpu().numpy(), nl.cpu().numpy()
+        parts.append((level, b, k))
+        sp = k > 0
+        s, m, l = level[sp, 0], level[sp, 1], k[sp]
+        level = np.stack([np.stack([s, l], 1), np.stack([s + l, m - l], 1)],
+                         axis=1).reshape(-1, 2)
+    start = np.concatenate([p[0][:, 0] for p in parts])
+    size = np.concatenate([p[0][:, 1] for p in parts])
+    box = np.concatenate([p[1] for p in parts])
+    split = np.concatenate([p[2] for p in parts]) > 0
+    left = np.full(len(start), -1, dtype=np.int64)
+    right = np.full(len(start), -1, dtype=np.int64)
+    j = np.arange(int(split.sum()), dtype=np.int64)
+    left[split], right[split] = 2 * j + 1, 2 * j + 2
+    return ClusterTree(perm.cpu().numpy(), start, size, left, right,
+                       np.ascontiguousarray(box[:, :3]), np.ascontiguousarray(box[:, 3:]), leaf)
+
+
+def build_block_partition_device(t: ClusterTree, eta: float = 2.0, min_size: int | None = None,
+                                 device="cuda") -> BlockPartition:
+    """``build_block_partition`` on the GPU: one launch per level of the block
+    tree; leaf blocks carry their path from the root and are put in the
+    recursion's depth-first order by sorting the paths."""
+    min_size = t.leaf if min_size is None else min_size
+    dev = torch.device(device)
+    node = torch.from_numpy(np.ascontiguousarray(
+        np.stack([t.start, t.size, t.left, t.right], 1).astype(np.int64))).to(dev)
+    box = torch.from_numpy(np.ascontiguousarray(np.concatenate([t.lo, t.hi], 1))).to(dev)
+    pairs = torch.zeros((1, 3), dtype=torch.int64, device=dev)   # (root, root), empty path
+    counts = torch.zeros(2, dtype=torch.int64, device=dev)
+    out = []
+    depth = 0
+    while pairs.shape[0]:
+        P = int(pairs.shape[0])
+        leafs = torch.empty((P, 4), dtype=torch.int64, device=dev)
+        nxt = torch.empty((4 * P, 3), dtype=torch.int64, device=dev)
+        counts.zero_()
+        _block_level(pairs, node, box, eta, min_size, depth, leafs, nxt, counts)
+        c = counts.cpu().numpy()
+        if not (0 <= c[0] <= P and 0 <= c[1] <= 4 * P):
+            raise RuntimeError("block level counts out of range")
+        out.append(leafs[:int(c[0])])
+        pairs = nxt[:int(c[1])]
+        depth += 1
+    L = torch.cat(out).cpu().numpy()
+    L = L[np.argsort(L[:, 3], kind="stable")]
+    return BlockPartition(np.ascontiguousarray(L[:, 0]), np.ascontiguousarray(L[:, 1]),
+                          L[:, 2].astype(bool), t)

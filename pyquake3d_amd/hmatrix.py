@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .cluster import build_block_partition, build_cluster_tree
+from .cluster import build_block_partition_device, build_cluster_tree_device
 from .mesh import FaultMesh, get_eleVec, receivers
 from .tde import hs_records, source_records
 
@@ -272,8 +272,8 @@ class HMatrix:
         self.n = n = tri.shape[0]
         tm = {}
         t0 = time.perf_counter()
-        self.tree = build_cluster_tree(self.xg, leaf)
-        self.part = build_block_partition(self.tree, eta)
+        self.tree = build_cluster_tree_device(self.xg, leaf, self.device)
+        self.part = build_block_partition_device(self.tree, eta, device=self.device)
         self.leaf_tiles, self.tile_of_row = leaf_tables(self.tree)
         tm["tree"] = time.perf_counter() - t0
         self.perm = self.tree.perm

@@ -13,6 +13,8 @@ from pyquake3d_amd import _lib
 from pyquake3d_amd import hmatrix as hm
 from pyquake3d_amd.mesh import undulating_fault
 
+from cluster_emulation import fake_block_level, fake_tree_level
+
 
 LC = None
 
@@ -69,6 +71,44 @@ class Checker:
             r0, m, c0, n, base = tab[b, :5]
             assert 0 <= r0 and r0 + m <= N and 0 <= c0 and c0 + n <= N and m > 0 and n > 0
             assert base >= 0 and base + 6 * m * n <= P
+
+    def pq3d_tree_level(self, pts, n, perm, tmp, nodes, nnodes, leaf, box, nl, stream):
+        # k_tree_level: workgroup q reads nodes[2q..2q+1], perm/tmp[s, s+m),
+        # pts[3 perm[i] + k], writes box[6q..6q+5] and nl[q]
+        P, pm, tp = self.T(pts), self.T(perm), self.T(tmp)
+        nd = self.T(nodes).reshape(-1).numpy()
+        assert P.numel() == 3 * n and pm.numel() == n and tp.numel() == n
+        assert nd.size >= 2 * nnodes and leaf >= 1
+        assert self.T(box).numel() >= 6 * nnodes and self.T(nl).numel() >= nnodes
+        s, m = nd[0:2 * nnodes:2], nd[1:2 * nnodes:2]
+        assert (s >= 0).all() and (m >= 1).all() and (s + m <= n).all()
+        order = np.argsort(s)                     # the nodes' ranges are disjoint
+        assert (s[order][1:] >= (s + m)[order][:-1]).all()
+        pv = pm.numpy()
+        for a, b in zip(s.tolist(), m.tolist()):
+            assert pv[a:a + b].min() >= 0 and pv[a:a + b].max() < n
+        fake_tree_level(P.view(-1, 3), pm, tp, self.T(nodes).view(-1, 2), leaf,
+                        self.T(box).view(-1, 6), self.T(nl))
+
+    def pq3d_block_level(self, pairs, npairs, node, box, eta, min_size, depth, leafs, nxt,
+                         counts, stream):
+        # k_block_level: thread p reads pairs[3p..3p+2], node[4a..], box[6a..]
+        # of both nodes, and writes at most one leaf row (< npairs) or four
+        # child rows (< 4 npairs)
+        pr = self.T(pairs).reshape(-1, 3).numpy()[:npairs]
+        nd = self.T(node).reshape(-1, 4).numpy()
+        assert self.T(box).numel() == 6 * len(nd)
+        assert ((pr[:, :2] >= 0) & (pr[:, :2] < len(nd))).all()
+        assert 0 <= depth < 30
+        assert self.T(leafs).numel() >= 4 * npairs and self.T(nxt).numel() >= 12 * npairs
+        assert self.T(counts).numel() >= 2
+        assert (self.T(counts) == 0).all()
+        kids = nd[pr[:, :2]][:, :, 2:]
+        split = (kids >= 0).all(axis=(1, 2))
+        assert (kids[split] < len(nd)).all()
+        fake_block_level(self.T(pairs).view(-1, 3)[:npairs], self.T(node).view(-1, 4),
+                         self.T(box).view(-1, 6), eta, min_size, depth,
+                         self.T(leafs).view(-1, 4), self.T(nxt).view(-1, 3), self.T(counts))
 
     def pq3d_hmat_aca(self, tab, wsoff, nb, recv, rec, hrec, mu, lam, eps, kmax, wU, wV, flags,
                       rank, stream):
