@@ -539,13 +539,42 @@ constexpr int kTileWaves = 4;
 
 // Phase-2 helpers: load U columns (three planes) of an item starting at
 // column c, and multiply-add them into the 3*NB accumulators.
+// Phase-2 operator loads are raw buffer loads: one buffer resource per item
+// (its first element and byte extent), a per-lane 32-bit byte offset for the
+// lane's three plane elements (off[o] = (o plane + row) sizeof(T), fixed for
+// the item) and a wave-uniform byte offset per column ((c + u) ld sizeof(T),
+// a scalar register). No 64-bit per-lane address arithmetic remains in the
+// column loop, and a load past the item's extent would read zero.
+constexpr int kBufDword3 = 0x00020000;  // raw 32-bit data, gfx9 family
+constexpr int kBufAux = PQ3D_NT_LOADS ? 2 : 0;  // non-temporal
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t item_rsrc(const void* p, uint64_t bytes) {
+  const uint32_t nrec = bytes > 0xffffffffull ? 0xffffffffu : (uint32_t)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nrec,
+                                           kBufDword3);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                 int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, kBufAux));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                   int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kBufAux));
+}
+
 template <typename T, int U>
-__device__ __forceinline__ void p2_load(const T* p, int64_t ld, int64_t plane, int row,
-                                        int64_t c, T (&a)[U][3]) {
+__device__ __forceinline__ void p2_load(__amdgpu_buffer_rsrc_t r, int ldb,
+                                        const uint32_t (&off)[3], int64_t c, T (&a)[U][3]) {
 #pragma unroll
-  for (int u = 0; u < U; ++u)
+  for (int u = 0; u < U; ++u) {
+    const int soff = (int)(c + u) * ldb;
 #pragma unroll
-    for (int o = 0; o < 3; ++o) a[u][o] = ld_op(p + (c + u) * ld + o * plane + row);
+    for (int o = 0; o < 3; ++o) a[u][o] = buf_load<T>(r, off[o], soff);
+  }
 }
 
 // x0: the weights of the group's first column; NB per column, of which this
@@ -609,7 +638,13 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
     const int64_t* it = items + q * IT_W;
     const int64_t kind = it[IT_KIND], base = it[IT_BASE], ld = it[IT_LD], plane = it[IT_PLANE];
     const int64_t cnt = it[IT_COUNT], xo = it[IT_XOFF];
-    const T* p = pool + base;  // wave-uniform; lanes add their row
+    // the item spans (cnt - 1) ld + 2 plane + mt elements from pool + base
+    // (host-checked: under 2^31 bytes)
+    const __amdgpu_buffer_rsrc_t rs =
+        item_rsrc(pool + base, (uint64_t)((cnt - 1) * ld + 2 * plane + mt) * sizeof(T));
+    const int ldb = (int)(ld * (int64_t)sizeof(T));
+    const uint32_t off[3] = {(uint32_t)(row * sizeof(T)), (uint32_t)((plane + row) * sizeof(T)),
+                             (uint32_t)((2 * plane + row) * sizeof(T))};
     const double* xs = (kind == 0 ? v1 : kind == 1 ? t : v2) + xo * NB + h * NW;
     int64_t c = 0;
     if constexpr (p2_prefetch<NB>()) {
@@ -617,11 +652,11 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
       // current group is multiplied
       if (U <= cnt) {
         T a[U][3];
-        p2_load<T, U>(p, ld, plane, row, c, a);
+        p2_load<T, U>(rs, ldb, off, c, a);
         for (;;) {
           const bool more = c + 2 * U <= cnt;
           T an[U][3];
-          if (more) p2_load<T, U>(p, ld, plane, row, c + U, an);
+          if (more) p2_load<T, U>(rs, ldb, off, c + U, an);
           p2_madd<T, NB, NW, U>(a, xs + c * NB, acc);
           c += U;
           if (!more) break;
@@ -634,13 +669,13 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
     } else {
       for (; c + U <= cnt; c += U) {
         T a[U][3];
-        p2_load<T, U>(p, ld, plane, row, c, a);
+        p2_load<T, U>(rs, ldb, off, c, a);
         p2_madd<T, NB, NW, U>(a, xs + c * NB, acc);
       }
     }
     for (; c < cnt; ++c) {
       T a[1][3];
-      p2_load<T, 1>(p, ld, plane, row, c, a);
+      p2_load<T, 1>(rs, ldb, off, c, a);
       p2_madd<T, NB, NW, 1>(a, xs + c * NB, acc);
     }
   }

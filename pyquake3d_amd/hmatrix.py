@@ -182,6 +182,20 @@ def pack_units(groups):
             np.ascontiguousarray(np.array(members, dtype=np.int64).reshape(-1, 4)))
 
 
+def check_item_extents(items, tiles, tile_ptr, esize) -> None:
+    """Raise unless every phase-2 item spans less than 2^31 bytes: the kernel
+    addresses an item through one buffer resource with 32-bit byte offsets
+    (column offset (count - 1) ld, plane offset 2 plane + rows)."""
+    it = np.asarray(items, dtype=np.int64).reshape(-1, 6)
+    tp = np.asarray(tile_ptr, dtype=np.int64)
+    mt = np.repeat(np.asarray(tiles, dtype=np.int64).reshape(-1, 2)[:, 1], np.diff(tp))
+    if len(it) != len(mt):
+        raise ValueError("items and tile_ptr disagree")
+    span = ((it[:, 4] - 1) * it[:, 2] + 2 * it[:, 3] + mt) * int(esize)
+    if len(span) and int(span.max()) >= 2 ** 31:
+        raise ValueError("a matvec item spans 2 GiB or more")
+
+
 def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_vb, k_keep):
     """Matvec work lists.
 
@@ -420,6 +434,7 @@ class HMatrix:
     def _build_worklists(self, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_vb, k_keep):
         W = build_worklists(self.tree, self.n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub,
                             l_vb, k_keep)
+        check_item_extents(W["items"], W["tiles"], W["tile_ptr"], self.esize)
         dev = self.device
         self.items = torch.from_numpy(W["items"]).to(dev)
         self.tile_ptr = torch.from_numpy(W["tile_ptr"]).to(dev)
@@ -548,6 +563,8 @@ class HMatrix:
         self.nunits, self.nred, self.ntiles = len(t["units"]), len(t["reduce"]), len(t["tiles"])
         for name in ("unit", "member", "tile", "item"):
             _lib.check_width(name, getattr(self, name + "s"))
+        check_item_extents(t["items"].numpy(), t["tiles"].numpy(), t["tile_ptr"].numpy(),
+                           self.esize)
         self.perm = t["perm"].numpy()
         self.iperm = np.empty_like(self.perm)
         self.iperm[self.perm] = np.arange(len(self.perm))
