@@ -32,6 +32,10 @@ Workloads
   assembly  Kernel assembly only: all N x N pairs at N = 20,000 (FP64, six
             traction operators, 19.2 GB of output in HBM). One step = one full
             assembly.
+  disp      Surface displacement: a 256 x 256 grid on the free surface
+            (x in [-60, 60] km, y in [-20, 20] km) from every element of the
+            N = 20,000 undulating fault with seeded slip, half space, FP64
+            (pq3d_tde_disp_sum). One step = one full grid.
 
 Prints ONE JSON line. Inputs are resident in HBM before the timed region;
 the timed region is bracketed by device synchronisation; kernel times come
@@ -66,6 +70,13 @@ MEASURED_TRAFFIC = {("f32", 1): 6_289_516_096, ("f32", 16): 7_465_723_168}
 # + 2 x (129 global rotation + 20 Hooke + 30 projection) + 24 (summing the
 # three angular dislocations); divisions and square roots count as one.
 FLOPS_PER_PAIR = 1571.0
+# displacement of one (point, triangle) pair, counted per operation in the
+# device source (+ - * / sqrt log atan atan2 each 1): full space 570 (three
+# angular dislocations 3 x 127, their frame rotations 3 x 25, point transform
+# and configuration 46, Burgers function and output ~68); half space on a
+# vertical ruled fault 3,370 (main + image 2 x 570, two corrected sides
+# 2 x 1,105, slip vector and sums 21)
+DISP_FLOPS_PER_PAIR_HS = 3370.0
 
 
 def log(msg):
@@ -684,6 +695,78 @@ def bench_assembly(args):
     return res
 
 
+def bench_disp(args):
+    import numpy as np
+    import torch
+
+    from pyquake3d_amd import tde
+    from pyquake3d_amd.mesh import undulating_fault
+
+    dev = torch.device("cuda:0")
+    mesh = undulating_fault(length=100e3, depth=25e3, nx=200, nz=50, amplitude=300.0,
+                            wavelength=5e3, seed=0)
+    N = mesh.n
+    dm = tde.DeviceMesh.from_mesh(mesh, 32e9, 32e9, device=dev, halfspace=True)
+    gx, gy = np.meshgrid(np.linspace(-60e3, 60e3, 256), np.linspace(-20e3, 20e3, 256) + 37.0)
+    obs_np = np.stack([gx.ravel(), gy.ravel(), np.zeros(gx.size)], 1)
+    rng = np.random.default_rng(0)
+    ss_np, ds_np = rng.normal(size=N), rng.normal(size=N)
+    obs = torch.from_numpy(obs_np).to(dev)
+    ss, ds = torch.from_numpy(ss_np).to(dev), torch.from_numpy(ds_np).to(dev)
+    out = torch.empty((len(obs_np), 3), dtype=torch.float64, device=dev)
+    log(f"disp: {len(obs_np)} surface points x {N} elements, half space")
+    for _ in range(args.warmup):
+        dm.displacement(obs, ss, ds, out=out)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        dm.displacement(obs, ss, ds, out=out)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = ev0.elapsed_time(ev1) / args.steps
+    u = out.cpu().numpy()
+    assert np.isfinite(u).all()
+    pairs = float(len(obs_np)) * N
+    achieved = pairs * DISP_FLOPS_PER_PAIR_HS / (kern_ms * 1e-3) / 1e12
+    res = {
+        "metric": "displacement pair evaluations/s (surface grid x elements, half space, FP64)",
+        "value": pairs * args.steps / wall,
+        "unit": "pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic undulating fault mesh and seeded slip",
+        "config": {"workload": "surface displacement, 256x256 grid at z=0 from the undulating "
+                               "fault 100x25 km, N=20000, half space, FP64"},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": MI355X_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_TFLOPS, "traffic": None,
+                     "kernel_ms": kern_ms},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_disp(mesh, obs_np, ss_np, ds_np)
+    return res
+
+
+def cpu_baseline_disp(mesh, obs, ss, ds):
+    """The displacement oracle (numpy, one thread per call, vectorised over
+    points) on the first elements of the workload until ~10 s have passed."""
+    from oracle.tde_disp_oracle import td_disp_hs
+    tri = mesh.triangles()
+    t = time.perf_counter()
+    j = 0
+    while time.perf_counter() - t < 10.0 and j < mesh.n:
+        td_disp_hs(obs, tri[j, 0:3], tri[j, 3:6], tri[j, 6:9], ss[j], ds[j], 0.0, 0.25)
+        j += 1
+    dt = time.perf_counter() - t
+    return {"value": len(obs) * j / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": f"numpy displacement oracle, {j} elements x {len(obs)} surface points, "
+                      f"{dt:.1f} s"}
+
+
 def cpu_baseline_assembly(mesh, dm):
     import numpy as np
 
@@ -710,7 +793,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="hstep", choices=["hstep", "assembly", "matvec", "learned", "ragged",
-                                                   "accuracy", "example"])
+                                                   "accuracy", "example", "disp"])
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"],
                     help="operator storage of the matvec workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -750,6 +833,10 @@ def main():
         args.steps = 1500 if args.steps is None else args.steps
         args.nb = 8 if args.nb is None else args.nb
         res = bench_learned(args)
+    elif args.workload == "disp":
+        args.steps = 3 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        res = bench_disp(args)
     elif args.workload == "matvec":
         args.steps = 20 if args.steps is None else args.steps
         args.warmup = 3 if args.warmup is None else args.warmup

@@ -19,20 +19,23 @@ PQ3D_HD inline void ang_dis_disp(double x, double y, double z, double cA, double
   const double r = sqrt(x * x + y * y + z * z);
   if (zeta > r) zeta = r;  // keeps the logarithms real
   if (z > r) z = r;
-  const double k = 1.0 / 8.0 / kPi / (1.0 - nu);
+  const double k = (1.0 / (8.0 * kPi)) / (1.0 - nu);
   const double N = 1.0 - 2.0 * nu;
   const double rz = r - z, rzeta = r - zeta;
   const double Lz = log(rz), Lzeta = log(rzeta);
-  const double ux = bx * k * (x * y / r / rz - x * eta / r / rzeta);
-  const double vx = bx * k * (eta * sA / rzeta - y * eta / r / rzeta + y * y / r / rz +
+  // reciprocals shared by the terms: 1/r, 1/(r - z), 1/(r - zeta)
+  const double ir = 1.0 / r, irz = 1.0 / rz, ize = 1.0 / rzeta;
+  const double irrz = ir * irz, irze = ir * ize;
+  const double ux = bx * k * (x * y * irrz - x * eta * irze);
+  const double vx = bx * k * (eta * sA * ize - y * eta * irze + y * y * irrz +
                               N * (cA * Lzeta - Lz));
-  const double wx = bx * k * (eta * cA / rzeta - y / r - eta * z / r / rzeta - N * sA * Lzeta);
-  const double uy = by * k * (x * x * cA / r / rzeta - x * x / r / rz - N * (cA * Lzeta - Lz));
-  const double vy = by * x * k * (y * cA / r / rzeta - sA * cA / rzeta - y / r / rz);
-  const double wy = by * x * k * (z * cA / r / rzeta - cA * cA / rzeta + 1.0 / r);
-  const double uz = bz * sA * k * (N * Lzeta - x * x / r / rzeta);
-  const double vz = bz * x * sA * k * (sA / rzeta - y / r / rzeta);
-  const double wz = bz * x * sA * k * (cA / rzeta - z / r / rzeta);
+  const double wx = bx * k * (eta * cA * ize - y * ir - eta * z * irze - N * sA * Lzeta);
+  const double uy = by * k * (x * x * cA * irze - x * x * irrz - N * (cA * Lzeta - Lz));
+  const double vy = by * x * k * (y * cA * irze - sA * cA * ize - y * irrz);
+  const double wy = by * x * k * (z * cA * irze - cA * cA * ize + ir);
+  const double uz = bz * sA * k * (N * Lzeta - x * x * irze);
+  const double vz = bz * x * sA * k * (sA * ize - y * irze);
+  const double wz = bz * x * sA * k * (cA * ize - z * irze);
   u = ux + uy + uz;
   v = vx + vy + vz;
   w = wx + wy + wz;
@@ -90,7 +93,7 @@ PQ3D_HD inline void tde_disp_point(const double* rec, double x, double y, double
 // Free-surface correction displacement of one angular dislocation at
 // (y1, y2, y3) in its frame, angle with (sinB, cosB, cotB, tan(B/2)), depth
 // a of its vertex (AngDisDispFSC, src/DH_greenfunction.py:313-399).
-PQ3D_HD inline __attribute__((noinline)) void fsc_disp(double y1, double y2, double y3,
+PQ3D_HD inline __attribute__((always_inline)) void fsc_disp(double y1, double y2, double y3,
                                                        const double* trig, double b1, double b2,
                                                        double b3, double nu, double a,
                                                        double v[3]) {
@@ -101,60 +104,64 @@ PQ3D_HD inline __attribute__((noinline)) void fsc_disp(double y1, double y2, dou
   const double z3b = -y1 * sinB + y3b * cosB;
   const double r2b = y1 * y1 + y2 * y2 + y3b * y3b;
   const double rb = sqrt(r2b);
-  const double r3b = rb * r2b;
   const double W6 = rb + y3b, W7 = rb + z3b;
-  const double Fib = 2.0 * atan(-y2 / (-W6 / tanH + y1));
+  const double Fib = 2.0 * atan(-y2 / (y1 - W6 / tanH));
   const double L6 = log(W6), L7 = log(W7);
-  const double ar = a / rb;
+  // reciprocals shared by the terms
+  const double irb = 1.0 / rb, iW6 = 1.0 / W6, iW7 = 1.0 / W7, icB = 1.0 / cosB;
+  const double ir2b = irb * irb, ir3b = ir2b * irb;
+  const double ar = a * irb;
   const double h = y3b - a;
   const double q = rb * cosB + y3b;
   const double c2 = cotB * cotB;
-  const double u1 = -2.0 * M1 * N1 * Fib * c2 + N1 * y2 / W6 * ((N1 - ar) * cotB - y1 / W6 * (nu + ar)) +
-                    N1 * y2 * cosB * cotB / W7 * (cosB + ar) + a * y2 * h * cotB / r3b +
-                    y2 * h / (rb * W6) * (-N1 * cotB + y1 / W6 * (2.0 * nu + ar) + a * y1 / r2b) +
-                    y2 * h / (rb * W7) *
-                        (cosB / W7 * (q * (N1 * cosB - ar) * cotB + 2.0 * M1 * (rb * sinB - y1) * cosB) -
-                         a * y3b * cosB * cotB / r2b);
+  const double hrb6 = h * irb * iW6, hrb7 = h * irb * iW7;
+  const double S = rb * sinB - y1;
+  const double u1 = -2.0 * M1 * N1 * Fib * c2 + N1 * y2 * iW6 * ((N1 - ar) * cotB - y1 * iW6 * (nu + ar)) +
+                    N1 * y2 * cosB * cotB * iW7 * (cosB + ar) + a * y2 * h * cotB * ir3b +
+                    y2 * hrb6 * (-N1 * cotB + y1 * iW6 * (2.0 * nu + ar) + a * y1 * ir2b) +
+                    y2 * hrb7 *
+                        (cosB * iW7 * (q * (N1 * cosB - ar) * cotB + 2.0 * M1 * S * cosB) -
+                         a * y3b * cosB * cotB * ir2b);
   const double u2 = N1 * ((2.0 * M1 * c2 - nu) * L6 - (2.0 * M1 * c2 + N1) * cosB * L7) -
-                    N1 / W6 * (y1 * cotB * (N1 - ar) + nu * y3b - a + y2 * y2 / W6 * (nu + ar)) -
-                    N1 * z1b * cotB / W7 * (cosB + ar) - a * y1 * h * cotB / r3b +
-                    h / W6 * (-2.0 * nu + 1.0 / rb * (N1 * y1 * cotB - a) +
-                              y2 * y2 / (rb * W6) * (2.0 * nu + ar) + a * y2 * y2 / r3b) +
-                    h / W7 * (cosB * cosB - 1.0 / rb * (N1 * z1b * cotB + a * cosB) +
-                              a * y3b * z1b * cotB / r3b -
-                              1.0 / (rb * W7) * (y2 * y2 * cosB * cosB - a * z1b * cotB / rb * q));
-  const double u3 = 2.0 * M1 * (N1 * Fib * cotB + y2 / W6 * (2.0 * nu + ar) - y2 * cosB / W7 * (cosB + ar)) +
-                    y2 * h / rb * (2.0 * nu / W6 + a / r2b) +
-                    y2 * h * cosB / (rb * W7) * (N1 - q / W7 * (cosB + ar) - a * y3b / r2b);
+                    N1 * iW6 * (y1 * cotB * (N1 - ar) + nu * y3b - a + y2 * y2 * iW6 * (nu + ar)) -
+                    N1 * z1b * cotB * iW7 * (cosB + ar) - a * y1 * h * cotB * ir3b +
+                    h * iW6 * (-2.0 * nu + irb * (N1 * y1 * cotB - a) +
+                               y2 * y2 * irb * iW6 * (2.0 * nu + ar) + a * y2 * y2 * ir3b) +
+                    h * iW7 * (cosB * cosB - irb * (N1 * z1b * cotB + a * cosB) +
+                               a * y3b * z1b * cotB * ir3b -
+                               irb * iW7 * (y2 * y2 * cosB * cosB - a * z1b * cotB * irb * q));
+  const double u3 = 2.0 * M1 * (N1 * Fib * cotB + y2 * iW6 * (2.0 * nu + ar) - y2 * cosB * iW7 * (cosB + ar)) +
+                    y2 * h * irb * (2.0 * nu * iW6 + a * ir2b) +
+                    y2 * cosB * hrb7 * (N1 - q * iW7 * (cosB + ar) - a * y3b * ir2b);
   const double w1 = N1 * ((2.0 * M1 * c2 + nu) * L6 - (2.0 * M1 * c2 + 1.0) * cosB * L7) +
-                    N1 / W6 * (-N1 * y1 * cotB + nu * y3b - a + a * y1 * cotB / rb + y1 * y1 / W6 * (nu + ar)) -
-                    N1 * cotB / W7 * (z1b * cosB - a * (rb * sinB - y1) / (rb * cosB)) -
-                    a * y1 * h * cotB / r3b +
-                    h / W6 * (2.0 * nu + 1.0 / rb * (N1 * y1 * cotB + a) -
-                              y1 * y1 / (rb * W6) * (2.0 * nu + ar) - a * y1 * y1 / r3b) +
-                    h * cotB / W7 *
-                        (-cosB * sinB + a * y1 * y3b / (r3b * cosB) +
-                         (rb * sinB - y1) / rb * (2.0 * M1 * cosB - q / W7 * (1.0 + a / (rb * cosB))));
-  const double w2 = 2.0 * M1 * N1 * Fib * c2 + N1 * y2 / W6 * (-(N1 - ar) * cotB + y1 / W6 * (nu + ar)) -
-                    N1 * y2 * cotB / W7 * (1.0 + a / (rb * cosB)) - a * y2 * h * cotB / r3b +
-                    y2 * h / (rb * W6) * (N1 * cotB - 2.0 * nu * y1 / W6 - a * y1 / rb * (1.0 / rb + 1.0 / W6)) +
-                    y2 * h * cotB / (rb * W7) *
-                        (-2.0 * M1 * cosB + q / W7 * (1.0 + a / (rb * cosB)) + a * y3b / (r2b * cosB));
-  const double w3 = -2.0 * M1 * N1 * cotB * (L6 - cosB * L7) - 2.0 * M1 * y1 / W6 * (2.0 * nu + ar) +
-                    2.0 * M1 * z1b / W7 * (cosB + ar) +
-                    h / rb * (N1 * cotB - 2.0 * nu * y1 / W6 - a * y1 / r2b) -
-                    h / W7 * (cosB * sinB + q * cotB / rb * (2.0 * M1 * cosB - q / W7) +
-                              ar * (sinB - y3b * z1b / r2b - z1b * q / (rb * W7)));
-  const double s1 = N1 * (y2 / W6 * (1.0 + ar) - y2 * cosB / W7 * (cosB + ar)) -
-                    y2 * h / rb * (a / r2b + 1.0 / W6) +
-                    y2 * h * cosB / (rb * W7) * (q / W7 * (cosB + ar) + a * y3b / r2b);
-  const double s2 = N1 * (-sinB * L7 - y1 / W6 * (1.0 + ar) + z1b / W7 * (cosB + ar)) +
-                    y1 * h / rb * (a / r2b + 1.0 / W6) -
-                    h / W7 * (sinB * (cosB - ar) + z1b / rb * (1.0 + a * y3b / r2b) -
-                              1.0 / (rb * W7) * (y2 * y2 * cosB * sinB - a * z1b / rb * q));
-  const double s3 = 2.0 * M1 * Fib + 2.0 * M1 * (y2 * sinB / W7 * (cosB + ar)) +
-                    y2 * h * sinB / (rb * W7) * (1.0 + q / W7 * (cosB + ar) + a * y3b / r2b);
-  const double c = 1.0 / 4.0 / kPi / (1.0 - nu);
+                    N1 * iW6 * (-N1 * y1 * cotB + nu * y3b - a + a * y1 * cotB * irb + y1 * y1 * iW6 * (nu + ar)) -
+                    N1 * cotB * iW7 * (z1b * cosB - a * S * irb * icB) -
+                    a * y1 * h * cotB * ir3b +
+                    h * iW6 * (2.0 * nu + irb * (N1 * y1 * cotB + a) -
+                               y1 * y1 * irb * iW6 * (2.0 * nu + ar) - a * y1 * y1 * ir3b) +
+                    h * cotB * iW7 *
+                        (-cosB * sinB + a * y1 * y3b * ir3b * icB +
+                         S * irb * (2.0 * M1 * cosB - q * iW7 * (1.0 + a * irb * icB)));
+  const double w2 = 2.0 * M1 * N1 * Fib * c2 + N1 * y2 * iW6 * (-(N1 - ar) * cotB + y1 * iW6 * (nu + ar)) -
+                    N1 * y2 * cotB * iW7 * (1.0 + a * irb * icB) - a * y2 * h * cotB * ir3b +
+                    y2 * hrb6 * (N1 * cotB - 2.0 * nu * y1 * iW6 - a * y1 * irb * (irb + iW6)) +
+                    y2 * cotB * hrb7 *
+                        (-2.0 * M1 * cosB + q * iW7 * (1.0 + a * irb * icB) + a * y3b * ir2b * icB);
+  const double w3 = -2.0 * M1 * N1 * cotB * (L6 - cosB * L7) - 2.0 * M1 * y1 * iW6 * (2.0 * nu + ar) +
+                    2.0 * M1 * z1b * iW7 * (cosB + ar) +
+                    h * irb * (N1 * cotB - 2.0 * nu * y1 * iW6 - a * y1 * ir2b) -
+                    h * iW7 * (cosB * sinB + q * cotB * irb * (2.0 * M1 * cosB - q * iW7) +
+                               ar * (sinB - y3b * z1b * ir2b - z1b * q * irb * iW7));
+  const double s1 = N1 * (y2 * iW6 * (1.0 + ar) - y2 * cosB * iW7 * (cosB + ar)) -
+                    y2 * h * irb * (a * ir2b + iW6) +
+                    y2 * cosB * hrb7 * (q * iW7 * (cosB + ar) + a * y3b * ir2b);
+  const double s2 = N1 * (-sinB * L7 - y1 * iW6 * (1.0 + ar) + z1b * iW7 * (cosB + ar)) +
+                    y1 * h * irb * (a * ir2b + iW6) -
+                    h * iW7 * (sinB * (cosB - ar) + z1b * irb * (1.0 + a * y3b * ir2b) -
+                               irb * iW7 * (y2 * y2 * cosB * sinB - a * z1b * irb * q));
+  const double s3 = 2.0 * M1 * Fib + 2.0 * M1 * (y2 * sinB * iW7 * (cosB + ar)) +
+                    y2 * sinB * hrb7 * (1.0 + q * iW7 * (cosB + ar) + a * y3b * ir2b);
+  const double c = (1.0 / (4.0 * kPi)) / (1.0 - nu);
   v[0] = c * (b1 * u1 + b2 * w1 + b3 * s1);
   v[1] = c * (b1 * u2 + b2 * w2 + b3 * s2);
   v[2] = c * (b1 * u3 + b2 * w3 + b3 * s3);
@@ -200,13 +207,6 @@ PQ3D_HD inline void harm_disp_point(const double* h, double x, double y, double 
   }
 }
 
-// Full-space point evaluation kept out of line in the half-space path.
-PQ3D_HD inline __attribute__((noinline)) void tde_disp_point_ol(const double* rec, double x,
-                                                        double y, double z, double Ss, double Ds,
-                                                        double Ts, double nu, double out[3]) {
-  tde_disp_point(rec, x, y, z, Ss, Ds, Ts, nu, out);
-}
-
 // Half-space displacement (TDdispHS): main + correction + image; for a
 // triangle with all vertices at z = 0 the image's vertical component and
 // then the whole sum change sign.
@@ -214,8 +214,8 @@ PQ3D_HD inline void tde_disp_point_hs(const double* rec, const double* h, double
                                       double z, double Ss, double Ds, double Ts, double nu,
                                       double out[3]) {
   double um[3], ui[3], uh[3];
-  tde_disp_point_ol(rec, x, y, z, Ss, Ds, Ts, nu, um);
-  tde_disp_point_ol(h + H_IMG, x, y, z, Ss, Ds, Ts, nu, ui);
+  tde_disp_point(rec, x, y, z, Ss, Ds, Ts, nu, um);
+  tde_disp_point(h + H_IMG, x, y, z, Ss, Ds, Ts, nu, ui);
   harm_disp_point(h, x, y, z, Ss, Ds, Ts, nu, uh);
   const bool surf = h[H_SURF] != 0.0;
   if (surf) ui[2] = -ui[2];
