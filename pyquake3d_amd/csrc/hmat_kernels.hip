@@ -340,18 +340,19 @@ __device__ __forceinline__ int64_t xcd_order(int64_t b, int64_t g) {
 }
 
 // Column loops are unrolled so that a group's loads are all issued before its
-// multiply-adds (several loads in flight per wave). A larger NB means more
-// arithmetic and more live registers per loaded value, hence less unrolling.
-// Build-time tuning knobs for 8 and 16 right-hand sides (defaults: the best
-// of a sweep on MI355X at N = 100,352, FP32 and FP64 storage).
+// multiply-adds (several loads in flight per wave). Phase 1 (one accumulator
+// per right-hand side) gains from deep groups: at 16 right-hand sides a group
+// of 16 columns is fastest although its weights no longer fit in scalar
+// registers. Phase 2 (3 NB accumulators per lane) unrolls less. Build-time
+// tuning knobs (defaults: the best of sweeps on MI355X at N = 100,352).
 #ifndef PQ3D_P1_U8
-#define PQ3D_P1_U8 4
+#define PQ3D_P1_U8 8
 #endif
 #ifndef PQ3D_P2_U8
 #define PQ3D_P2_U8 4
 #endif
 #ifndef PQ3D_P1_U16
-#define PQ3D_P1_U16 4
+#define PQ3D_P1_U16 16
 #endif
 #ifndef PQ3D_P2_U16
 #define PQ3D_P2_U16 2
@@ -377,9 +378,12 @@ __device__ __forceinline__ T ld_op(const T* p) {
 #endif
 }
 
+#ifndef PQ3D_P1_U1
+#define PQ3D_P1_U1 8
+#endif
 template <int NB>
 constexpr int p1_unroll() {
-  return NB <= 2 ? 8 : NB <= 4 ? 4 : NB <= 8 ? PQ3D_P1_U8 : PQ3D_P1_U16;
+  return NB <= 2 ? PQ3D_P1_U1 : NB <= 4 ? 4 : NB <= 8 ? PQ3D_P1_U8 : PQ3D_P1_U16;
 }
 template <int NB>
 constexpr int p2_unroll() {
