@@ -60,7 +60,7 @@ YEAR = 3.15576e7
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of the matvec workload,
 # FETCH scaled by 2.0 and WRITE by 1.0: the factors the same counters read
 # for a 2 GiB device-to-device copy (profiles/hmatvec_traffic_N100352.md).
-MEASURED_TRAFFIC = {("f32", 1): 6_289_516_096, ("f32", 16): 7_465_723_168}
+MEASURED_TRAFFIC = {("f32", 1): 6_276_266_539, ("f32", 16): 7_350_784_928}
 
 # FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
 # (unit strike and dip slip, both projected), counted operator by operator
