@@ -656,9 +656,10 @@ def bench_assembly(args):
     mesh = undulating_fault(length=100e3, depth=25e3, nx=200, nz=50, amplitude=300.0,
                             wavelength=5e3, seed=0)
     N = mesh.n
-    dm = tde.DeviceMesh.from_mesh(mesh, 32e9, 32e9, device=dev)
+    hs = bool(args.halfspace)
+    dm = tde.DeviceMesh.from_mesh(mesh, 32e9, 32e9, device=dev, halfspace=hs)
     out = torch.empty((6, N, N), dtype=torch.float64, device=dev)
-    log(f"assembly: N={N}, output {out.numel() * 8 / 1e9:.1f} GB")
+    log(f"assembly: N={N}, output {out.numel() * 8 / 1e9:.1f} GB, half space {hs}")
     for _ in range(args.warmup):
         dm.traction_block(out=out)
     torch.cuda.synchronize()
@@ -675,6 +676,22 @@ def bench_assembly(args):
     pairs = float(N) * N
     sample = out[:, :64, :64].cpu().numpy()
     assert np.isfinite(sample).all()
+    if hs:
+        res = {
+            "metric": "TDE evaluations/s (N x N assembly, FP64, half space)",
+            "value": 2.0 * pairs * args.steps / wall, "unit": "evaluations/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall * 1e3 / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic undulating fault mesh, seed 0",
+            "config": {"workload": "kernel assembly, undulating fault 100x25 km, N=20000, unit "
+                                   "Ss and Ds, six traction operators, FP64, half space "
+                                   "(free surface z=0)"},
+            "roofline": {"bound": "fp64-valu", "achieved": None, "peak": MI355X_FP64_TFLOPS,
+                         "unit": "TFLOP/s", "frac": None, "traffic": None,
+                         "kernel_ms": kern_ms},
+        }
+        return res
     achieved = pairs * FLOPS_PER_PAIR / (kern_ms * 1e-3) / 1e12
     res = {
         "metric": "TDE evaluations/s (N x N assembly, FP64)",
@@ -797,6 +814,8 @@ def main():
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"],
                     help="operator storage of the matvec workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--halfspace", action="store_true",
+                    help="assembly: half-space operators (free surface z=0)")
     ap.add_argument("--nb", type=int, default=None, choices=[1, 2, 4, 8, 16],
                     help="friction draws in the batched variant (default 16; learned: 8)")
     ap.add_argument("--count", type=int, default=64, help="ragged: number of simulations")

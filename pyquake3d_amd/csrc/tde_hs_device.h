@@ -147,7 +147,14 @@ PQ3D_HD inline D3 d3sqrt(D3 a) {
 }
 // Terms that enter the displacement only with constant coefficients need
 // their gradient alone: log(a) and 2 atan(n/d) carry value 0 here.
-PQ3D_HD inline D3 d3log_grad(D3 a) { return {0.0, a.g0 / a.v, a.g1 / a.v, a.g2 / a.v}; }
+// (ia = 1 / a.v, computed once by the caller)
+PQ3D_HD inline D3 d3log_grad_r(D3 a, double ia) { return {0.0, a.g0 * ia, a.g1 * ia, a.g2 * ia}; }
+// 1 / b
+PQ3D_HD inline D3 d3inv(D3 b) {
+  const double r = 1.0 / b.v;
+  const double m = -r * r;
+  return {r, b.g0 * m, b.g1 * m, b.g2 * m};
+}
 PQ3D_HD inline D3 d3atan2x2_grad(D3 n, D3 d) {
   const double w = 2.0 / (n.v * n.v + d.v * d.v);
   return {0.0, (d.v * n.g0 - n.v * d.g0) * w, (d.v * n.g1 - n.v * d.g1) * w,
@@ -174,76 +181,81 @@ PQ3D_HD inline __attribute__((noinline)) void fsc_grad(double y1v, double y2v, d
   const D3 z3b = (-y1) * sinB + y3b * cosB;
   const D3 r2b = y1 * y1 + y2 * y2 + y3b * y3b;
   const D3 rb = d3sqrt(r2b);
-  const D3 r3b = rb * r2b;
   const D3 W6 = rb + y3b;
   const D3 W7 = rb + z3b;
-  const D3 ar = a / rb;
   const D3 hh = y3b - a;
   const D3 q = rb * cosB + y3b;
   const double c2 = cotB * cotB;
+  // reciprocals shared by the terms (one division each instead of four per
+  // forward-mode quotient)
+  const D3 irb = d3inv(rb), iW6 = d3inv(W6), iW7 = d3inv(W7);
+  const D3 ir2b = irb * irb, ir3b = ir2b * irb;
+  const D3 irbW6 = irb * iW6, irbW7 = irb * iW7;
+  const D3 ar = a * irb;
+  const double icB = 1.0 / cosB, itH = 1.0 / tanH;
+  const D3 S = rb * sinB - y1;
   D3 o1, o2, o3;
   if constexpr (K == 0) {
-    const D3 Fib = d3atan2x2_grad(-y2, (-W6) / tanH + y1);
-    const D3 L6 = d3log_grad(W6), L7 = d3log_grad(W7);
-    o1 = (-2.0 * M1 * N1 * c2) * Fib + N1 * y2 / W6 * ((N1 - ar) * cotB - y1 / W6 * (nu + ar)) +
-         (N1 * cosB * cotB) * y2 / W7 * (cosB + ar) + a * cotB * y2 * hh / r3b +
-         y2 * hh / (rb * W6) * ((-N1 * cotB) + y1 / W6 * (2.0 * nu + ar) + a * y1 / r2b) +
-         y2 * hh / (rb * W7) *
-             (cosB / W7 * (q * (N1 * cosB - ar) * cotB + (2.0 * M1 * cosB) * (rb * sinB - y1)) -
-              (a * cosB * cotB) * y3b / r2b);
+    const D3 Fib = d3atan2x2_grad(-y2, (-W6) * itH + y1);
+    const D3 L6 = d3log_grad_r(W6, iW6.v), L7 = d3log_grad_r(W7, iW7.v);
+    o1 = (-2.0 * M1 * N1 * c2) * Fib + N1 * y2 * iW6 * ((N1 - ar) * cotB - y1 * iW6 * (nu + ar)) +
+         (N1 * cosB * cotB) * y2 * iW7 * (cosB + ar) + (a * cotB) * y2 * hh * ir3b +
+         y2 * hh * irbW6 * ((-N1 * cotB) + y1 * iW6 * (2.0 * nu + ar) + a * y1 * ir2b) +
+         y2 * hh * irbW7 *
+             (cosB * iW7 * (q * (N1 * cosB - ar) * cotB + (2.0 * M1 * cosB) * S) -
+              (a * cosB * cotB) * y3b * ir2b);
     const D3 y2s = y2 * y2;
     o2 = N1 * ((2.0 * M1 * c2 - nu) * L6 - ((2.0 * M1 * c2 + N1) * cosB) * L7) -
-         N1 / W6 * (cotB * y1 * (N1 - ar) + nu * y3b - a + y2s / W6 * (nu + ar)) -
-         (N1 * cotB) * z1b / W7 * (cosB + ar) - (a * cotB) * y1 * hh / r3b +
-         hh / W6 *
-             ((-2.0 * nu) + 1.0 / rb * (N1 * cotB * y1 - a) + y2s / (rb * W6) * (2.0 * nu + ar) +
-              a * y2s / r3b) +
-         hh / W7 *
-             (cosB * cosB - 1.0 / rb * (N1 * cotB * z1b + a * cosB) +
-              (a * cotB) * y3b * z1b / r3b -
-              1.0 / (rb * W7) * ((cosB * cosB) * y2s - (a * cotB) * z1b / rb * q));
-    o3 = 2.0 * M1 * ((N1 * cotB) * Fib + y2 / W6 * (2.0 * nu + ar) - cosB * y2 / W7 * (cosB + ar)) +
-         y2 * hh / rb * ((2.0 * nu) / W6 + a / r2b) +
-         cosB * y2 * hh / (rb * W7) * (N1 - q / W7 * (cosB + ar) - a * y3b / r2b);
+         N1 * iW6 * (cotB * y1 * (N1 - ar) + nu * y3b - a + y2s * iW6 * (nu + ar)) -
+         (N1 * cotB) * z1b * iW7 * (cosB + ar) - (a * cotB) * y1 * hh * ir3b +
+         hh * iW6 *
+             ((-2.0 * nu) + irb * (N1 * cotB * y1 - a) + y2s * irbW6 * (2.0 * nu + ar) +
+              a * y2s * ir3b) +
+         hh * iW7 *
+             (cosB * cosB - irb * (N1 * cotB * z1b + a * cosB) +
+              (a * cotB) * y3b * z1b * ir3b -
+              irbW7 * ((cosB * cosB) * y2s - (a * cotB) * z1b * irb * q));
+    o3 = 2.0 * M1 * ((N1 * cotB) * Fib + y2 * iW6 * (2.0 * nu + ar) - cosB * y2 * iW7 * (cosB + ar)) +
+         y2 * hh * irb * ((2.0 * nu) * iW6 + a * ir2b) +
+         cosB * y2 * hh * irbW7 * (N1 - q * iW7 * (cosB + ar) - a * y3b * ir2b);
   } else if constexpr (K == 1) {
-    const D3 Fib = d3atan2x2_grad(-y2, (-W6) / tanH + y1);
-    const D3 L6 = d3log_grad(W6), L7 = d3log_grad(W7);
+    const D3 Fib = d3atan2x2_grad(-y2, (-W6) * itH + y1);
+    const D3 L6 = d3log_grad_r(W6, iW6.v), L7 = d3log_grad_r(W7, iW7.v);
     const D3 y1s = y1 * y1;
     o1 = N1 * ((2.0 * M1 * c2 + nu) * L6 - ((2.0 * M1 * c2 + 1.0) * cosB) * L7) +
-         N1 / W6 * ((-N1 * cotB) * y1 + nu * y3b - a + (a * cotB) * y1 / rb + y1s / W6 * (nu + ar)) -
-         (N1 * cotB) / W7 * (cosB * z1b - a * (rb * sinB - y1) / (rb * cosB)) -
-         (a * cotB) * y1 * hh / r3b +
-         hh / W6 *
-             ((2.0 * nu) + 1.0 / rb * (N1 * cotB * y1 + a) - y1s / (rb * W6) * (2.0 * nu + ar) -
-              a * y1s / r3b) +
-         cotB * hh / W7 *
-             ((-cosB * sinB) + a * y1 * y3b / (r3b * cosB) +
-              (rb * sinB - y1) / rb * ((2.0 * M1 * cosB) - q / W7 * (1.0 + a / (rb * cosB))));
-    o2 = (2.0 * M1 * N1 * c2) * Fib + N1 * y2 / W6 * ((-(N1 - ar)) * cotB + y1 / W6 * (nu + ar)) -
-         (N1 * cotB) * y2 / W7 * (1.0 + a / (rb * cosB)) - (a * cotB) * y2 * hh / r3b +
-         y2 * hh / (rb * W6) *
-             (N1 * cotB - (2.0 * nu) * y1 / W6 - a * y1 / rb * (1.0 / rb + 1.0 / W6)) +
-         cotB * y2 * hh / (rb * W7) *
-             ((-2.0 * M1 * cosB) + q / W7 * (1.0 + a / (rb * cosB)) + a * y3b / (r2b * cosB));
-    o3 = (-2.0 * M1 * N1 * cotB) * (L6 - cosB * L7) - (2.0 * M1) * y1 / W6 * (2.0 * nu + ar) +
-         (2.0 * M1) * z1b / W7 * (cosB + ar) +
-         hh / rb * (N1 * cotB - (2.0 * nu) * y1 / W6 - a * y1 / r2b) -
-         hh / W7 *
-             (cosB * sinB + cotB * q / rb * ((2.0 * M1 * cosB) - q / W7) +
-              ar * (sinB - y3b * z1b / r2b - z1b * q / (rb * W7)));
+         N1 * iW6 * ((-N1 * cotB) * y1 + nu * y3b - a + (a * cotB) * y1 * irb + y1s * iW6 * (nu + ar)) -
+         (N1 * cotB) * iW7 * (cosB * z1b - a * S * irb * icB) -
+         (a * cotB) * y1 * hh * ir3b +
+         hh * iW6 *
+             ((2.0 * nu) + irb * (N1 * cotB * y1 + a) - y1s * irbW6 * (2.0 * nu + ar) -
+              a * y1s * ir3b) +
+         cotB * hh * iW7 *
+             ((-cosB * sinB) + a * y1 * y3b * ir3b * icB +
+              S * irb * ((2.0 * M1 * cosB) - q * iW7 * (1.0 + a * irb * icB)));
+    o2 = (2.0 * M1 * N1 * c2) * Fib + N1 * y2 * iW6 * ((-(N1 - ar)) * cotB + y1 * iW6 * (nu + ar)) -
+         (N1 * cotB) * y2 * iW7 * (1.0 + a * irb * icB) - (a * cotB) * y2 * hh * ir3b +
+         y2 * hh * irbW6 * (N1 * cotB - (2.0 * nu) * y1 * iW6 - a * y1 * irb * (irb + iW6)) +
+         cotB * y2 * hh * irbW7 *
+             ((-2.0 * M1 * cosB) + q * iW7 * (1.0 + a * irb * icB) + a * y3b * ir2b * icB);
+    o3 = (-2.0 * M1 * N1 * cotB) * (L6 - cosB * L7) - (2.0 * M1) * y1 * iW6 * (2.0 * nu + ar) +
+         (2.0 * M1) * z1b * iW7 * (cosB + ar) +
+         hh * irb * (N1 * cotB - (2.0 * nu) * y1 * iW6 - a * y1 * ir2b) -
+         hh * iW7 *
+             (cosB * sinB + cotB * q * irb * ((2.0 * M1 * cosB) - q * iW7) +
+              ar * (sinB - y3b * z1b * ir2b - z1b * q * irbW7));
   } else {
-    const D3 Fib = d3atan2x2_grad(-y2, (-W6) / tanH + y1);
-    const D3 L7 = d3log_grad(W7);
-    o1 = N1 * (y2 / W6 * (1.0 + ar) - cosB * y2 / W7 * (cosB + ar)) -
-         y2 * hh / rb * (a / r2b + 1.0 / W6) +
-         cosB * y2 * hh / (rb * W7) * (q / W7 * (cosB + ar) + a * y3b / r2b);
-    o2 = N1 * ((-sinB) * L7 - y1 / W6 * (1.0 + ar) + z1b / W7 * (cosB + ar)) +
-         y1 * hh / rb * (a / r2b + 1.0 / W6) -
-         hh / W7 *
-             (sinB * (cosB - ar) + z1b / rb * (1.0 + a * y3b / r2b) -
-              1.0 / (rb * W7) * ((cosB * sinB) * (y2 * y2) - a * z1b / rb * q));
-    o3 = (2.0 * M1) * Fib + (2.0 * M1 * sinB) * y2 / W7 * (cosB + ar) +
-         sinB * y2 * hh / (rb * W7) * (1.0 + q / W7 * (cosB + ar) + a * y3b / r2b);
+    const D3 Fib = d3atan2x2_grad(-y2, (-W6) * itH + y1);
+    const D3 L7 = d3log_grad_r(W7, iW7.v);
+    o1 = N1 * (y2 * iW6 * (1.0 + ar) - cosB * y2 * iW7 * (cosB + ar)) -
+         y2 * hh * irb * (a * ir2b + iW6) +
+         cosB * y2 * hh * irbW7 * (q * iW7 * (cosB + ar) + a * y3b * ir2b);
+    o2 = N1 * ((-sinB) * L7 - y1 * iW6 * (1.0 + ar) + z1b * iW7 * (cosB + ar)) +
+         y1 * hh * irb * (a * ir2b + iW6) -
+         hh * iW7 *
+             (sinB * (cosB - ar) + z1b * irb * (1.0 + a * y3b * ir2b) -
+              irbW7 * ((cosB * sinB) * (y2 * y2) - a * z1b * irb * q));
+    o3 = (2.0 * M1) * Fib + (2.0 * M1 * sinB) * y2 * iW7 * (cosB + ar) +
+         sinB * y2 * hh * irbW7 * (1.0 + q * iW7 * (cosB + ar) + a * y3b * ir2b);
   }
   const double c = 1.0 / 4.0 / kPi / (1.0 - nu);
   g[0][0] = c * o1.g0;
