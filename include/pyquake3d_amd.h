@@ -140,8 +140,8 @@ int pq3d_tde_disp_points(const double* d_X, const double* d_Y, const double* d_Z
                          double Ts, double nu, double* d_disp, void* stream);
 /* get_sumS's displacement (src/QDsim.py:25-53) with per-triangle slip:
  * d_out[3 i + c] = sum_j u_c(d_obs[3 i .. 3 i + 2]; triangle j, d_ss[j],
- * d_ds[j], d_ts ? d_ts[j] : 0). d_obs nobs x 3; d_hrec null -> full space.
- * Deterministic: the same inputs give the same bits. */
+ * d_ds[j], d_ts ? d_ts[j] : 0). d_obs nobs x 3 (nobs <= 2^29); d_hrec null
+ * -> full space. Deterministic: the same inputs give the same bits. */
 int pq3d_tde_disp_sum(const double* d_obs, int64_t nobs, const double* d_rec,
                       const double* d_hrec, int64_t ns, const double* d_ss, const double* d_ds,
                       const double* d_ts, double nu, double* d_out, void* stream);

@@ -749,7 +749,7 @@ int pq3d_hmat_dense_assemble(const int64_t* d_tab, int64_t nblocks, const double
   if (nblocks == 0) return PQ3D_OK;
   PQ3D_REQUIRE(d_tab && d_recv && d_rec && d_pool, "null pointer");
   PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
-  PQ3D_REQUIRE(nblocks < (1LL << 31), "too many blocks");
+  PQ3D_REQUIRE(nblocks < (1LL << 24), "too many blocks");
   const double nu = 1.0 / (1.0 + lam / mu) / 2.0;
   hipStream_t s = as_stream(stream);
   const unsigned g = (unsigned)nblocks;
@@ -784,7 +784,7 @@ int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
                "null pointer");
   PQ3D_REQUIRE(kmax >= 1 && kmax <= 4096, "kmax out of range");
   PQ3D_REQUIRE(eps > 0.0, "eps must be > 0");
-  PQ3D_REQUIRE(nblocks < (1LL << 31), "too many blocks");
+  PQ3D_REQUIRE(nblocks < (1LL << 24), "too many blocks");
   const double nu = 1.0 / (1.0 + lam / mu) / 2.0;
   if (d_hrec)
     k_aca<true><<<(unsigned)nblocks, kAcaThreads, 0, as_stream(stream)>>>(
@@ -807,7 +807,7 @@ int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nbl
   PQ3D_REQUIRE(d_tab && d_wsoff && d_tiles && d_tile_of_row && d_wU && d_wV && d_pool,
                "null pointer");
   PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
-  PQ3D_REQUIRE(nblocks < (1LL << 31), "too many blocks");
+  PQ3D_REQUIRE(nblocks < (1LL << 24), "too many blocks");
   hipStream_t s = as_stream(stream);
   if (dtype_bytes == 4)
     k_aca_pack<float><<<(unsigned)nblocks, 256, 0, s>>>(d_tab, d_wsoff, d_tiles, d_tile_of_row,
@@ -840,7 +840,8 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_me
                "null pointer");
   PQ3D_REQUIRE(nunits == 0 || (d_units && d_members && d_t), "null pointer");
   PQ3D_REQUIRE(nred == 0 || (d_red && d_part), "null pointer");
-  PQ3D_REQUIRE(nunits < (1LL << 33) && nred < (1LL << 33) && ntiles < (1LL << 31),
+  // launch sizes: grid x workgroup size stays below 2^32 threads
+  PQ3D_REQUIRE(nunits < (1LL << 26) && nred < (1LL << 26) && ntiles < (1LL << 22),
                "too many work items");
   hipStream_t s = as_stream(stream);
 #define PQ3D_MV(NBV)                                                                        \

@@ -469,7 +469,7 @@ int pq3d_tde_disp_sum(const double* d_obs, int64_t nobs, const double* d_rec,
   if (nobs == 0) return PQ3D_OK;
   PQ3D_REQUIRE(d_obs && d_out, "null pointer");
   PQ3D_REQUIRE(ns == 0 || (d_rec && d_ss && d_ds), "null source pointer");
-  PQ3D_REQUIRE(nobs <= (int64_t)1 << 30, "nobs too large (at most 2^30 points)");
+  PQ3D_REQUIRE(nobs <= (int64_t)1 << 29, "nobs too large (at most 2^29 points)");
   PQ3D_REQUIRE(nu > -1.0 && nu < 0.5, "Poisson's ratio must lie in (-1, 0.5)");
   const unsigned int g = blocks_for(nobs, 64);
   if (d_hrec)

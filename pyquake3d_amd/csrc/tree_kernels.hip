@@ -223,7 +223,7 @@ int pq3d_tree_level(const double* d_pts, int64_t n, int64_t* d_perm, int64_t* d_
   if (nnodes == 0) return PQ3D_OK;
   PQ3D_REQUIRE(d_pts && d_perm && d_tmp && d_nodes && d_box && d_nl, "null pointer");
   PQ3D_REQUIRE(leaf >= 1, "leaf must be >= 1");
-  PQ3D_REQUIRE(nnodes <= ((int64_t)1 << 31) - 1, "too many nodes in one level");
+  PQ3D_REQUIRE(nnodes < ((int64_t)1 << 24), "too many nodes in one level (at most 2^24 - 1)");
   k_tree_level<<<(unsigned)nnodes, kTreeThreads, 0, as_stream(stream)>>>(
       d_pts, d_perm, d_tmp, d_nodes, leaf, d_box, d_nl);
   PQ3D_LAUNCH_CHECK();
@@ -240,7 +240,7 @@ int pq3d_block_level(const int64_t* d_pairs, int64_t npairs, const int64_t* d_no
   PQ3D_REQUIRE(d_pairs && d_node && d_box && d_leafs && d_next && d_counts, "null pointer");
   PQ3D_REQUIRE(depth >= 0 && depth < kMaxBlockDepth - 1, "block tree deeper than %d levels",
                kMaxBlockDepth - 1);
-  PQ3D_REQUIRE(npairs <= ((int64_t)1 << 40), "too many pairs in one level");
+  PQ3D_REQUIRE(npairs <= ((int64_t)1 << 31), "too many pairs in one level (at most 2^31)");
   k_block_level<<<blocks_for(npairs, 256), 256, 0, as_stream(stream)>>>(
       d_pairs, npairs, d_node, d_box, eta, min_size, depth, d_leafs, d_next, d_counts);
   PQ3D_LAUNCH_CHECK();
