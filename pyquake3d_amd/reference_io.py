@@ -86,6 +86,43 @@ def _min_dist(points, targets, chunk=4096):
     return out
 
 
+def lab_edge_normal_stress(Tno, xg, nodelst, elelst, seed=42):
+    """Normal stress of the laboratory model (Tn_edge, src/QDsim.py:309-377):
+    within 2 mm of the mesh boundary away from the free surface it ramps
+    linearly from 0.7 Tno[0] at the boundary to Tno[0]; elsewhere it is a
+    smoothed random field: the ramped stress times (1 + U(-0.5, 0.5))
+    (legacy numpy generator seeded with 42: a permutation of the cells, then
+    the uniform draws), sampled by nearest neighbour on a 100 x 100 (x, y)
+    grid, Gaussian-filtered (sigma 4 cells) and sampled back by nearest
+    neighbour."""
+    from scipy.interpolate import griddata
+    from scipy.ndimage import gaussian_filter
+    rs = np.random.RandomState(seed)
+    Tno = np.array(Tno, dtype=np.float64)
+    bn = boundary_nodes(elelst)
+    bc = nodelst[bn - 1]
+    bc_sb = bc[~(np.abs(bc[:, 2]) < 1e-5)]
+    wedge = 0.1 * 2.0 * 1e-2
+    ed, top = Tno[0] * 0.7, Tno[0]
+    dis = _min_dist(xg, bc_sb) / wedge
+    edge = dis < 1.0
+    Tno[edge] = ed - (ed - top) * dis[edge]
+    n = len(Tno)
+    sample = rs.choice(np.arange(n), size=n, replace=False)
+    arr = rs.uniform(low=-0.5, high=0.5, size=n)
+    noisy = Tno.copy()
+    noisy[sample] = Tno[sample] + Tno[sample] * arr
+    xi = np.linspace(xg[:, 0].min(), xg[:, 0].max(), 100)
+    yi = np.linspace(xg[:, 1].min(), xg[:, 1].max(), 100)
+    Xi, Yi = np.meshgrid(xi, yi)
+    Si = griddata((xg[:, 0], xg[:, 1]), noisy, (Xi, Yi), method="nearest")
+    smooth = gaussian_filter(Si, sigma=4.0)
+    val = griddata(np.stack((Xi.ravel(), Yi.ravel()), axis=1), smooth.ravel(), xg[:, [0, 1]],
+                   method="nearest")
+    val = np.where(np.isnan(val), Tno, val)
+    return np.where(edge, Tno, val)
+
+
 def problem_from_parameters(param_path, mesh_path=None):
     """Problem, half-space flag and parameter dict of a reference model.
 
@@ -94,8 +131,9 @@ def problem_from_parameters(param_path, mesh_path=None):
     Dc, f0, -, Tno, V, shear and normal loading per element,
     src/QDsim.py:687-740) and False (principal stresses projected on each
     element, friction zones by distance to the mesh boundary, optional
-    nucleation disc, src/QDsim.py:387-655). Initlab (the laboratory model's
-    smoothed edge stresses) is not supported."""
+    nucleation disc, src/QDsim.py:387-655), with the laboratory model's
+    edge-tapered, smoothed random normal stress when Initlab is set
+    (``lab_edge_normal_stress``)."""
     p = read_para(param_path)
     d = os.path.dirname(os.path.abspath(param_path))
     if mesh_path is None:
@@ -114,8 +152,6 @@ def problem_from_parameters(param_path, mesh_path=None):
     rho = float(p["Rock density"])
     V0 = float(p["Reference slip rate"])
     Vpl = float(p["Plate loading rate"])
-    if _flag(p, "Initlab"):
-        raise NotImplementedError("Initlab (laboratory edge stresses) is not supported")
     ev1, ev2, ev3 = eleVec[:, 0:3], eleVec[:, 3:6], eleVec[:, 6:9]
 
     # stresses projected on the element (src/QDsim.py:387-456)
@@ -150,6 +186,8 @@ def problem_from_parameters(param_path, mesh_path=None):
         rake = np.full(N, float(p["Fix_rake"])) / 180.0 * np.pi
     if not solve_normal and not depth_var:
         Tno = np.full(N, trac_nor)
+    if _flag(p, "Initlab"):
+        Tno = lab_edge_normal_stress(Tno, xg, nodelst, elelst)
 
     slipvC = np.full(N, Vpl)            # Grad_slpv_con(const=True)
     f0 = np.full(N, float(p["Reference friction coefficient"]))

@@ -1,18 +1,22 @@
 """Readers of the reference's model files (gmsh 2.2 mesh, "key: value"
-parameters, the 10-column element parameter file) and the compact model
-archive. The BP5-QD example is checked against its committed archive when
-the reference tree is present."""
+parameters, the 10-column element parameter file, the laboratory model's
+normal stress) and the compact model archive. The example models are checked
+against their committed archives when the reference tree is present."""
 import os
 
 import numpy as np
 import pytest
 
 from pyquake3d_amd.problem import load_problem, planar_c1, save_problem
-from pyquake3d_amd.reference_io import (boundary_nodes, problem_from_parameters, read_msh_v2,
-                                        read_para)
+from pyquake3d_amd.mesh import get_eleVec, planar_fault
+from pyquake3d_amd.reference_io import (boundary_nodes, lab_edge_normal_stress,
+                                        problem_from_parameters, read_msh_v2, read_para)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 BP5 = "/root/reference/examples/BP5-QD/parameter.txt"
+EXAMPLES = "/root/reference/examples"
+needs_reference = pytest.mark.skipif(not os.path.isdir(EXAMPLES),
+                                     reason="the reference's example files are not here")
 
 MSH = """$MeshFormat\r
 2.2 0 8\r
@@ -70,6 +74,7 @@ def test_model_archive_round_trip(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.exists(BP5), reason="reference example tree not present")
+@needs_reference
 def test_bp5_example_matches_its_archive():
     """The reader on the reference's BP5-QD files (9,250 triangles, half
     space, 10-column element file) equals the committed archive; the initial
@@ -87,3 +92,62 @@ def test_bp5_example_matches_its_archive():
     slow = q.slipv == 1e-9
     # steady state at V = Vpl: state = f0 + b ln(V0/Vpl)
     assert np.allclose(state[slow], (q.f0 + q.b * np.log(q.V0 / q.Vpl))[slow], rtol=1e-12)
+
+
+@needs_reference
+@pytest.mark.parametrize("d,name,n,hs", [("Lab-model", "lab_example.npz", 14336, False),
+                                         ("EAFZ-model", "eafz_example.npz", 41800, True)])
+def test_examples_match_their_archives(d, name, n, hs):
+    p, h, para = problem_from_parameters(os.path.join(EXAMPLES, d, "parameter.txt"))
+    q, h2 = load_problem(os.path.join(GOLDEN, name))
+    assert h == h2 == hs and p.n == q.n == n
+    assert np.array_equal(p.mesh.elelst, q.mesh.elelst)
+    assert np.array_equal(p.mesh.nodelst, q.mesh.nodelst)
+    for k in ("a", "b", "dc", "f0", "Tno", "slipv", "rake", "Vpl"):
+        assert np.array_equal(getattr(p, k), getattr(q, k)), k
+
+
+def _dipping_lab_mesh():
+    """A 4 x 2 cm fault dipping 30 degrees, its top edge at z = 0."""
+    m = planar_fault(length=0.04, depth=0.02, nx=24, nz=12)
+    nodes = m.nodelst.copy()
+    nodes[:, 1] = -nodes[:, 2] / np.tan(np.radians(30.0))
+    return nodes, m.elelst
+
+
+def test_lab_normal_stress_edge_taper_and_smoothing():
+    """Within 2 mm of the boundary (the free-surface edge excluded) the
+    stress ramps from 0.7 Tno[0] to Tno[0] with the distance; elsewhere it
+    is the random field smoothed: inside the range of the unsmoothed field
+    and far less spread than it; fixed by the seed."""
+    nodes, ele = _dipping_lab_mesh()
+    ev, xg = get_eleVec(nodes, ele)
+    T0 = np.full(len(xg), 60.0)
+    out = lab_edge_normal_stress(T0, xg, nodes, ele)
+    bn = boundary_nodes(ele)
+    bc = nodes[bn - 1]
+    bc = bc[np.abs(bc[:, 2]) >= 1e-5]
+    dist = np.sqrt(((xg[:, None, :] - bc[None, :, :]) ** 2).sum(-1)).min(1)
+    edge = dist < 0.002
+    assert 0 < edge.sum() < len(xg)
+    ramp = 42.0 + 18.0 * dist / 0.002
+    assert np.allclose(out[edge], ramp[edge], rtol=0, atol=1e-12)
+    inner = out[~edge]
+    assert inner.min() >= 0.5 * 42.0 and inner.max() <= 1.5 * 60.0
+    # the unsmoothed field has the spread of U(-0.5, 0.5) (std 0.29 of the mean)
+    assert inner.std() < 0.1 * inner.mean()
+    assert np.array_equal(out, lab_edge_normal_stress(T0, xg, nodes, ele))
+    assert not np.array_equal(out, lab_edge_normal_stress(T0, xg, nodes, ele, seed=7))
+
+
+def test_legacy_seeded_draws_equal_a_seeded_generator_object():
+    """The reference seeds numpy's global generator with 42, then draws a
+    permutation (choice without replacement) and uniforms; a RandomState(42)
+    object gives the same sequence."""
+    n = 1000
+    np.random.seed(42)
+    s1 = np.random.choice(np.arange(n), size=n, replace=False)
+    a1 = np.random.uniform(low=-0.5, high=0.5, size=n)
+    rs = np.random.RandomState(42)
+    assert np.array_equal(s1, rs.choice(np.arange(n), size=n, replace=False))
+    assert np.array_equal(a1, rs.uniform(low=-0.5, high=0.5, size=n))
