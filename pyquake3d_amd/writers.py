@@ -14,6 +14,10 @@ build's results unchanged.
                                            pyvista
   restart_fields(cells)                    the simulator state those fields
                                            hold (rake in radians)
+  element_file_from_vtk(vtk, out)          the 10-column element file made from
+                                           a run's VTK output, as
+                                           examples/HF-model/create_init/
+                                           initcond.py makes it, without pyvista
   model_summary(problem)                   element sizes, nucleation size and
                                            cohesive zone (src/QDsim.py:258-295)
   StateLog                                 state.txt: header, one line per
@@ -156,6 +160,25 @@ def restart_fields(cells: dict) -> dict:
         "rake": cells["rake[Degree]"] / 180.0 * np.pi, "slip1": cells["slip1"],
         "slip2": cells["slip2"],
     }
+
+
+def element_file_from_vtk(vtk_path, out_path, f0=0.6) -> int:
+    """Write the 10-column element file (rake, a, b, Dc, f0, shear traction,
+    normal traction, slip rate, shear loading 0, normal loading 0) that
+    examples/HF-model/create_init/initcond.py writes from a run's VTK output,
+    with its formats; returns the number of rows. As in that script, the
+    rake column holds the VTK's degrees, while the element-file reader takes
+    it in radians (src/QDsim.py:698, :729-730)."""
+    c = read_vtk(vtk_path)["cells"]
+    rows = zip(c["rake[Degree]"].tolist(), c["a"].tolist(), c["b"].tolist(), c["dc"].tolist(),
+               c["Shear_[MPa]"].tolist(), c["Normal_[MPa]"].tolist(),
+               c["Slipv[m/s]"].tolist())
+    n = 0
+    with open(out_path, "w") as f:
+        for rake, a, b, dc, tt, tn, v in rows:
+            f.write("%f %f %f %f %g %f %f %.20f 0 0\n" % (rake, a, b, dc, f0, tt, tn, v))
+            n += 1
+    return n
 
 
 def model_summary(problem) -> dict:

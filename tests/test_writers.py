@@ -133,3 +133,22 @@ def test_state_log_lines(tmp_path, small):
     assert lines[15] == "3 0.500000 0.000000 2.0000000000000001e-09 86400.000000 1.000000"
     assert lines[16] == "Program end time: T1"
     assert lines[17] == "Time taken: 12.35 seconds"
+
+
+def test_element_file_from_vtk(tmp_path, small):
+    """The HF-model workflow: a run's VTK output -> 10-column element file
+    with the columns and formats of examples/HF-model/create_init/initcond.py."""
+    p = small
+    snap = _snapshot(p.n, np.random.default_rng(3))
+    cells = writers.vtk_cells(snap, p)
+    writers.write_vtk(tmp_path / "step0.vtk", p.mesh.nodelst, p.mesh.elelst, cells)
+    n = writers.element_file_from_vtk(tmp_path / "step0.vtk", tmp_path / "initcondion.txt")
+    assert n == p.n
+    lines = (tmp_path / "initcondion.txt").read_text().splitlines()
+    f = lines[0].split()
+    assert len(f) == 10 and f[4] == "0.6" and f[8] == "0" and f[9] == "0"
+    assert f[0] == "%f" % float("%f" % (snap["rake"][0] * 180 / np.pi))
+    assert f[7] == "%.20f" % float("%.40f" % snap["slipv"][0])
+    vals = np.loadtxt(tmp_path / "initcondion.txt")
+    assert np.allclose(vals[:, 1], p.a, atol=5e-7) and np.allclose(vals[:, 6], snap["Tno"], atol=1e-6)
+    assert np.allclose(vals[:, 5], np.hypot(snap["Tt1o"], snap["Tt2o"]), atol=1e-6)
