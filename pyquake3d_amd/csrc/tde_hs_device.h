@@ -400,24 +400,46 @@ PQ3D_HD inline void tde_point_hs(const double* rec, const double* h, double x, d
   }
 }
 
+// Fused full-space strike + dip pair kept out of line in the half-space path.
+PQ3D_HD inline __attribute__((noinline)) void tde_pair_traction_sd_ol(
+    const double* rec, const double* p, const double* f, double mu, double lam, double nu,
+    double ts[3], double td[3]) {
+  tde_pair_traction_sd(rec, p, f, mu, lam, nu, ts, td);
+}
+
 // Unit strike and unit dip slip, half space, projected on the receiver frame
-// f: ts = (A1s, A2s, Bs), td = (A1d, A2d, Bd).
+// f: ts = (A1s, A2s, Bs), td = (A1d, A2d, Bd). Tractions are linear in the
+// stress, so main, image and correction are projected separately; main and
+// image use the fused strike + dip evaluation. A triangle with all vertices
+// at z = 0 (its image's xz and yz components change sign) takes the
+// component-wise path.
 PQ3D_HD inline void tde_pair_traction_sd_hs(const double* rec, const double* h, const double* p,
                                             const double* f, double mu, double lam, double nu,
                                             double ts[3], double td[3]) {
+  const double b[2][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}};
+  double sh[2][6], eh[2][6];
+  if (h[H_SURF] == 0.0) {
+    double tsm[3], tdm[3], tsi[3], tdi[3], tsh[3], tdh[3];
+    tde_pair_traction_sd_ol(rec, p, f, mu, lam, nu, tsm, tdm);
+    tde_pair_traction_sd_ol(h + H_IMG, p, f, mu, lam, nu, tsi, tdi);
+    harm_point<2>(h, p[0], p[1], p[2], b, mu, lam, nu, sh, eh);
+    project(sh[0], f, tsh);
+    project(sh[1], f, tdh);
+    for (int k = 0; k < 3; ++k) {
+      ts[k] = tsm[k] + tsh[k] + tsi[k];
+      td[k] = tdm[k] + tdh[k] + tdi[k];
+    }
+    return;
+  }
   double sm[6], em[6], dm[6], fm[6], si[6], ei[6], di[6], fi[6];
   tde_point_ol(rec, p[0], p[1], p[2], 1.0, 0.0, 0.0, mu, lam, nu, sm, em);
   tde_point_ol(rec, p[0], p[1], p[2], 0.0, 1.0, 0.0, mu, lam, nu, dm, fm);
   tde_point_ol(h + H_IMG, p[0], p[1], p[2], 1.0, 0.0, 0.0, mu, lam, nu, si, ei);
   tde_point_ol(h + H_IMG, p[0], p[1], p[2], 0.0, 1.0, 0.0, mu, lam, nu, di, fi);
-  if (h[H_SURF] != 0.0) {
-    si[4] = -si[4];
-    si[5] = -si[5];
-    di[4] = -di[4];
-    di[5] = -di[5];
-  }
-  const double b[2][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}};
-  double sh[2][6], eh[2][6];
+  si[4] = -si[4];
+  si[5] = -si[5];
+  di[4] = -di[4];
+  di[5] = -di[5];
   harm_point<2>(h, p[0], p[1], p[2], b, mu, lam, nu, sh, eh);
   double s1[6], s2[6];
   for (int k = 0; k < 6; ++k) {
