@@ -189,6 +189,15 @@ def bench_hstep(args):
                          for k in range(1, args.nb)]
     batched, _ = run_variant(draws, H32, args.steps, args.warmup, f"batched_fp32_nb{args.nb}")
     head = batched
+    batched_mfma = None
+    if args.nb == 16:
+        # the same ensemble with the 16-RHS block products on FP32 matrix cores
+        H32.accumulate = "mfma"
+        batched_mfma, _ = run_variant(draws, H32, args.steps, args.warmup, "batched_fp32_mfma")
+        H32.accumulate = "f64"
+        if args.accumulate == "mfma":
+            head = batched_mfma
+    mf = head is batched_mfma
     achieved = head["matvec_bytes"] / (head["matvec_ms"] * 1e-3) / 1e9
     res = {
         "metric": "sim-years/wall-hour at N=1e5 (fp32+batched+learned dt vs plain fp64)",
@@ -201,25 +210,31 @@ def bench_hstep(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 operator storage, f64 state and accumulation",
+        "dtype": ("f32 operator storage and matrix-core products (f32 sums per item group), "
+                  "f64 low-rank V products, state and RK45" if mf else
+                  "f32 operator storage, f64 state and accumulation"),
         "data": "synthetic undulating fault mesh and seeded friction fields",
         "config": {"workload": f"H-matrix RK45 time stepping, undulating fault 100x25 km, "
                                f"N={problem.n}, leaf 64, eta 2, ACA eps 1e-4, BP5-like friction, "
                                f"{args.steps} steps after {args.warmup} from the seeded initial "
                                f"state (a step: one accepted RK45 step of every draw; a draw "
                                f"that needs more than 8 attempts finishes in the next step)",
-                   "variant": f"batched_fp32: {args.nb} friction draws sharing the operator, "
-                              f"one {args.nb}-RHS H-matvec per step, reference step controller "
-                              f"per draw run on the device (learned step size not in this line)"},
+                   "variant": (("batched_fp32_mfma" if mf else "batched_fp32") +
+                               f": {args.nb} friction draws sharing the operator, "
+                               f"one {args.nb}-RHS H-matvec per step"
+                               + (" (dense and U products on FP32 matrix cores)" if mf else "")
+                               + ", reference step controller per draw run on the device "
+                                 "(learned step size not in this line)")},
         "speedup_vs_plain_fp64": head["years_per_hour"] / plain["years_per_hour"],
         "plain_fp64": plain,
         "fast_fp32": fast,
         "batched_fp32": batched,
+        "batched_fp32_mfma": batched_mfma,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / MI355X_HBM_GBS,
-                     "traffic": MEASURED_TRAFFIC.get(("f32", args.nb)),
+                     "traffic": MEASURED_TRAFFIC.get(("f32-mfma" if mf else "f32", args.nb)),
                      "kernel": f"H-matvec, {args.nb} right-hand sides (k_hmv_phase1 + "
-                               f"k_hmv_phase1b + k_hmv_phase2)",
+                               f"k_hmv_phase1b + " + ("k_hmv_phase2_mfma16)" if mf else "k_hmv_phase2)"),
                      "kernel_ms": head["matvec_ms"], "algorithmic_bytes": head["matvec_bytes"]},
     }
     if not args.no_cpu_baseline:
@@ -255,6 +270,7 @@ def bench_matvec(args):
     problem = undulating_c3()
     dtype = torch.float32 if args.dtype == "f32" else torch.float64
     H, build_s = build_hmatrix(problem, dtype)
+    H.accumulate = args.accumulate
     n, nb, dev = H.n, args.nb, H.device
     gen = torch.Generator(device=dev)
     gen.manual_seed(0)
@@ -287,7 +303,10 @@ def bench_matvec(args):
         "value": achieved, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": f"{args.dtype} operator storage, f64 vectors and accumulation",
+        "dtype": (f"{args.dtype} operator storage, f32 matrix-core products for the dense and U "
+                  f"blocks, f64 V products and vectors"
+                  if (H.accumulate == "mfma" and nb == 16 and args.dtype == "f32") else
+                  f"{args.dtype} operator storage, f64 vectors and accumulation"),
         "data": "synthetic undulating fault mesh, seeded random slip-rate vectors",
         "config": {"workload": f"H-matvec, undulating fault 100x25 km, N={n}, leaf 64, eta 2, "
                                f"ACA eps 1e-4, {nb} right-hand sides"},
@@ -814,6 +833,9 @@ def main():
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"],
                     help="operator storage of the matvec workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--accumulate", default="f64", choices=["f64", "mfma"],
+                    help="16 right-hand sides over FP32 storage: 'mfma' runs the dense and U "
+                         "products on FP32 matrix cores (hstep headline and matvec)")
     ap.add_argument("--halfspace", action="store_true",
                     help="assembly: half-space operators (free surface z=0)")
     ap.add_argument("--nb", type=int, default=None, choices=[1, 2, 4, 8, 16],

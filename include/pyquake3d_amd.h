@@ -237,6 +237,19 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_me
                      double* d_y1, double* d_y2, double* d_y3, int64_t y_bstride,
                      void* stream);
 
+/* pq3d_hmat_matvec with the dense and U-block products on FP32 matrix cores
+ * (same replaced reference path, src/Hmatrix.py:1445-1538): FP32 storage
+ * (dtype_bytes 4) and nrhs 16 only. Phase 1 (V^T x) is the FP64 one of
+ * pq3d_hmat_matvec; phase 2 rounds the weights to FP32 and sums in FP32
+ * within each of a tile's four item groups, the groups added in FP32 in
+ * a fixed order; outputs FP64. */
+int pq3d_hmat_matvec_mfma(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
+                          const int64_t* d_red, int64_t nred, const int64_t* d_tiles,
+                          const int64_t* d_tile_ptr, int64_t ntiles, const int64_t* d_items,
+                          const void* d_pool, int dtype_bytes, int nrhs, const double* d_v1,
+                          const double* d_v2, double* d_t, double* d_part, double* d_y1,
+                          double* d_y2, double* d_y3, int64_t y_bstride, void* stream);
+
 /* ---------------------------------------------------------- time step
  * Cells may belong to n_inst independent instances (d_inst[i], or NULL for
  * one instance); d_h, d_cellpar (V0, mu, Cs per instance), d_err (4 per

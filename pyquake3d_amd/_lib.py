@@ -78,6 +78,10 @@ SIGNATURES = {
                                  c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_int64, c_void_p]),
+    "pq3d_hmat_matvec_mfma": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64,
+                                      c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int,
+                                      c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_int64, c_void_p]),
     # time step
     "pq3d_rk45_attempt": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -707,6 +707,177 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
   }
 }
 
+// Phase 2 on FP32 matrix cores, 16 right-hand sides, FP32 operator storage:
+// the same tiles, items and item groups as k_hmv_phase2<float, 16>, but each
+// item is a (rows x columns) x (columns x 16) product on
+// v_mfma_f32_16x16x4f32 with the weights rounded to FP32 and FP32
+// accumulation inside an item group (over all of its items); the four
+// groups' sums are added in LDS in group order (FP32) and written as FP64.
+// A tile's 64 rows are four 16-row blocks per output plane, held by one
+// wavefront per item group (kMfRb = 4: 12 accumulator tiles of 16 x 16) or
+// split over 2 or 4 wavefronts.
+//   A (16 x 4):  lane l holds row (l & 15), column (l >> 4) of the step
+//   B (4 x 16):  lane l holds column (l >> 4), right-hand side (l & 15)
+//   D (16 x 16): lane l, register r holds row 4 (l >> 4) + r,
+//                right-hand side (l & 15)
+// Measured on MI355X at N = 100,352 (16-RHS product, phase 1 included):
+// kMfRb 4 1.79-1.80 ms, 2 2.15 ms, 1 2.71 ms; the FP64-accumulating
+// k_hmv_phase2 1.71-1.74 ms.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+
+// 16-row blocks per wavefront: a tile's 64 rows are split over 4 / kMfRb
+// wavefronts per item group (fewer accumulators per wave, more waves).
+#ifndef PQ3D_MF_RB
+#define PQ3D_MF_RB 4
+#endif
+constexpr int kMfRb = PQ3D_MF_RB;
+constexpr int kMfWaves = kTileWaves * (4 / kMfRb);
+static_assert(kMfRb == 1 || kMfRb == 2 || kMfRb == 4, "bad row split");
+
+// One 4-column step of an item: off[o][rb] is the lane's byte offset of
+// plane o, row block rb, column kl of the step (per lane, voffset); the
+// step's first column c goes in soffset, the same for the whole wavefront.
+// In the masked last step a column past the item's end reads the item's last
+// column (in bounds), and its A and B values are zeroed.
+template <bool MASK>
+__device__ __forceinline__ void mf_load(__amdgpu_buffer_rsrc_t rs,
+                                        const uint32_t (&off)[3][kMfRb], int ldb, int64_t c,
+                                        int kl, int il, int64_t cnt,
+                                        const double* __restrict__ xs, float (&a)[3][kMfRb],
+                                        double& b) {
+  const bool in = !MASK || c + kl < cnt;
+  const int64_t col = in ? c + kl : cnt - 1;
+  const uint32_t back = MASK ? (uint32_t)((c + kl - col) * ldb) : 0u;
+  const int soff = (int)c * ldb;
+#pragma unroll
+  for (int o = 0; o < 3; ++o)
+#pragma unroll
+    for (int rb = 0; rb < kMfRb; ++rb) {
+      const float v = buf_load<float>(rs, off[o][rb] - back, soff);
+      a[o][rb] = in ? v : 0.0f;
+    }
+  const double x = xs[col * 16 + il];
+  b = in ? x : 0.0;
+}
+
+__device__ __forceinline__ void mf_mma(const float (&a)[3][kMfRb], double b,
+                                       f32x4 (&acc)[3][kMfRb]) {
+  const float bf = (float)b;
+#pragma unroll
+  for (int o = 0; o < 3; ++o)
+#pragma unroll
+    for (int rb = 0; rb < kMfRb; ++rb)
+      acc[o][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[o][rb], bf, acc[o][rb], 0, 0, 0);
+}
+
+// The item group's items of one tile, for the wavefront's row blocks
+// rb0 .. rb0 + kMfRb - 1 (rows past the tile's last repeat it: in bounds,
+// their sums are not stored). Each step's loads are issued one step ahead of
+// its products.
+__device__ __forceinline__ void mf_items(const int64_t* __restrict__ items, int64_t q0,
+                                         int64_t q1, const float* __restrict__ pool,
+                                         const double* __restrict__ v1,
+                                         const double* __restrict__ v2,
+                                         const double* __restrict__ t, int64_t mt, int rb0,
+                                         int kl, int il, f32x4 (&acc)[3][kMfRb]) {
+  constexpr int NB = 16;
+  for (int64_t q = q0; q < q1; q += kTileWaves) {
+    const int64_t* it = items + q * IT_W;
+    const int64_t kind = it[IT_KIND], base = it[IT_BASE], ld = it[IT_LD], plane = it[IT_PLANE];
+    const int64_t cnt = it[IT_COUNT], xo = it[IT_XOFF];
+    // the item spans (cnt - 1) ld + 2 plane + mt elements from pool + base
+    // (host-checked: under 2^31 bytes)
+    const __amdgpu_buffer_rsrc_t rs =
+        item_rsrc(pool + base, (uint64_t)((cnt - 1) * ld + 2 * plane + mt) * sizeof(float));
+    const int ldb = (int)(ld * (int64_t)sizeof(float));
+    uint32_t off[3][kMfRb];
+#pragma unroll
+    for (int rb = 0; rb < kMfRb; ++rb) {
+      const int64_t r0 = (int64_t)(rb0 + rb) * 16 + il;
+      const int64_t r = r0 < mt ? r0 : mt - 1;
+#pragma unroll
+      for (int o = 0; o < 3; ++o)
+        off[o][rb] = (uint32_t)((o * plane + r) * sizeof(float)) + (uint32_t)(kl * ldb);
+    }
+    const double* xs = (kind == 0 ? v1 : kind == 1 ? t : v2) + xo * NB;
+    int64_t c = 0;
+    if (cnt >= 4) {
+      float a[3][kMfRb];
+      double b;
+      mf_load<false>(rs, off, ldb, 0, kl, il, cnt, xs, a, b);
+      for (; c + 8 <= cnt; c += 4) {
+        float an[3][kMfRb];
+        double bn;
+        mf_load<false>(rs, off, ldb, c + 4, kl, il, cnt, xs, an, bn);
+        mf_mma(a, b, acc);
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+#pragma unroll
+          for (int rb = 0; rb < kMfRb; ++rb) a[o][rb] = an[o][rb];
+        b = bn;
+      }
+      mf_mma(a, b, acc);
+      c += 4;
+    }
+    if (c < cnt) {
+      float a[3][kMfRb];
+      double b;
+      mf_load<true>(rs, off, ldb, c, kl, il, cnt, xs, a, b);
+      mf_mma(a, b, acc);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64 * kMfWaves) PQ3D_P2_ATTR k_hmv_phase2_mfma16(
+    const int64_t* __restrict__ tiles, const int64_t* __restrict__ tile_ptr,
+    const int64_t* __restrict__ items, const float* __restrict__ pool,
+    const double* __restrict__ v1, const double* __restrict__ v2, const double* __restrict__ t,
+    double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3, int64_t ybs) {
+  constexpr int NB = 16;
+  __shared__ float red[3][NB][64];
+  const int64_t tile = xcd_order(blockIdx.x, gridDim.x);
+  const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
+  const int w = wave_id();
+  const int g = w % kTileWaves;           // item group
+  const int rb0 = (w / kTileWaves) * kMfRb;  // first 16-row block of this wavefront
+  const int lane = threadIdx.x & 63;
+  const int kl = lane >> 4, il = lane & 15;
+  f32x4 acc[3][kMfRb];
+#pragma unroll
+  for (int o = 0; o < 3; ++o)
+#pragma unroll
+    for (int rb = 0; rb < kMfRb; ++rb) acc[o][rb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  // row blocks wholly past the tile's rows have nothing to compute
+  if (rb0 * 16 < mt)
+    mf_items(items, tile_ptr[tile] + g, tile_ptr[tile + 1], pool, v1, v2, t, mt, rb0, kl, il,
+             acc);
+  // deterministic cross-group sum: group 0 stores, groups 1.. add in order
+  // (the wavefronts of one group own disjoint rows)
+  for (int gg = 0; gg < kTileWaves; ++gg) {
+    if (g == gg) {
+#pragma unroll
+      for (int o = 0; o < 3; ++o)
+#pragma unroll
+        for (int rb = 0; rb < kMfRb; ++rb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float* p = &red[o][il][(rb0 + rb) * 16 + 4 * kl + r];
+            *p = (gg == 0) ? acc[o][rb][r] : *p + acc[o][rb][r];
+          }
+    }
+    __syncthreads();
+  }
+  double* const ys[3] = {y1, y2, y3};
+  for (int e = threadIdx.x; e < 3 * NB * 64; e += 64 * kMfWaves) {
+    const int o = e / (NB * 64), m = e % (NB * 64);
+    // consecutive threads write consecutive addresses in either output layout
+    const int row = ybs > 0 ? m % 64 : m / NB, b = ybs > 0 ? m / 64 : m % NB;
+    if (row < mt)
+      ys[o][ybs > 0 ? b * ybs + R0 + row : (R0 + row) * NB + b] = (double)red[o][b][row];
+  }
+}
+
 template <typename T, int NB>
 static void launch_matvec(const int64_t* units, int64_t nunits, const int64_t* members,
                           const int64_t* red,
@@ -866,6 +1037,43 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_me
     default: pq3d::set_error("nrhs must be 1, 2, 4, 8 or 16"); return PQ3D_EINVAL;
   }
 #undef PQ3D_MV
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+// pq3d_hmat_matvec for FP32 storage and 16 right-hand sides with the block
+// products on FP32 matrix cores: phase 1 (V^T x) as pq3d_hmat_matvec in FP64,
+// phase 2 (dense and U blocks) with weights rounded to FP32 and FP32 sums
+// per item group, added across the four groups in FP64. Same arguments;
+// dtype_bytes must be 4 and nrhs 16.
+int pq3d_hmat_matvec_mfma(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
+                          const int64_t* d_red, int64_t nred, const int64_t* d_tiles,
+                          const int64_t* d_tile_ptr, int64_t ntiles, const int64_t* d_items,
+                          const void* d_pool, int dtype_bytes, int nrhs, const double* d_v1,
+                          const double* d_v2, double* d_t, double* d_part, double* d_y1,
+                          double* d_y2, double* d_y3, int64_t y_bstride, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(dtype_bytes == 4, "the matrix-core matvec needs FP32 storage (dtype_bytes 4)");
+  PQ3D_REQUIRE(nrhs == 16, "the matrix-core matvec takes 16 right-hand sides");
+  PQ3D_REQUIRE(ntiles >= 0 && nunits >= 0 && nred >= 0, "sizes must be >= 0");
+  PQ3D_REQUIRE(y_bstride >= 0, "y_bstride must be >= 0");
+  PQ3D_REQUIRE(d_tiles && d_tile_ptr && d_items && d_pool && d_v1 && d_v2 && d_y1 && d_y2 &&
+                   d_y3,
+               "null pointer");
+  PQ3D_REQUIRE(nunits == 0 || (d_units && d_members && d_t), "null pointer");
+  PQ3D_REQUIRE(nred == 0 || (d_red && d_part), "null pointer");
+  PQ3D_REQUIRE(nunits < (1LL << 26) && nred < (1LL << 26) && ntiles < (1LL << 22),
+               "too many work items");
+  hipStream_t s = as_stream(stream);
+  const float* pool = (const float*)d_pool;
+  if (nunits > 0)
+    k_hmv_phase1<float, 16><<<(unsigned)((nunits + 3) / 4), 256, 0, s>>>(
+        d_units, nunits, d_members, pool, d_v1, d_v2, d_t, d_part);
+  if (nred > 0)
+    k_hmv_phase1b<16><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(d_red, nred, d_part, d_t);
+  if (ntiles > 0)
+    k_hmv_phase2_mfma16<<<(unsigned)ntiles, 64 * kMfWaves, 0, s>>>(
+        d_tiles, d_tile_ptr, d_items, pool, d_v1, d_v2, d_t, d_y1, d_y2, d_y3, y_bstride);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }

@@ -277,6 +277,10 @@ class HMatrix:
         self.esize = 4 if dtype == torch.float32 else 8
         if dtype not in (torch.float32, torch.float64):
             raise ValueError("dtype must be float32 or float64")
+        # "f64": every product summed in FP64 (bitwise independent of nrhs);
+        # "mfma": with FP32 storage and 16 right-hand sides, the dense and U
+        # products run on FP32 matrix cores (FP32 sums per item group)
+        self.accumulate = "f64"
         self.mu, self.lambda_ = float(mu), float(lambda_)
         nodelst = np.asarray(nodelst, dtype=np.float64)
         elelst = np.asarray(elelst, dtype=np.int64)
@@ -488,7 +492,10 @@ class HMatrix:
         if tuple(out.shape) != (3, n * nrhs) or not out.is_contiguous() or out.dtype != torch.float64:
             raise ValueError("out must be a contiguous (3, n*nrhs) float64 tensor")
         t, part, red = self._bufs(nrhs)
-        _lib.call("pq3d_hmat_matvec", _lib.ptr(self.units), self.nunits, _lib.ptr(self.members),
+        if self.accumulate not in ("f64", "mfma"):
+            raise ValueError("accumulate must be 'f64' or 'mfma'")
+        mfma = self.accumulate == "mfma" and nrhs == 16 and self.esize == 4
+        _lib.call("pq3d_hmat_matvec_mfma" if mfma else "pq3d_hmat_matvec", _lib.ptr(self.units), self.nunits, _lib.ptr(self.members),
                   _lib.ptr(red), int(red.shape[0]), _lib.ptr(self.tiles), _lib.ptr(self.tile_ptr),
                   self.ntiles,
                   _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
@@ -553,6 +560,7 @@ class HMatrix:
         self.device = dev
         self.esize = int(h["esize"])
         self.dtype = torch.float32 if self.esize == 4 else torch.float64
+        self.accumulate = "f64"
         self.n, self.nt, self.npart = int(h["n"]), int(h["nt"]), int(h["npart"])
         self.halfspace = h["halfspace"] == "1"
         self.mu, self.lambda_ = float(h["mu"]), float(h["lambda"])

@@ -128,6 +128,48 @@ def test_multi_rhs_products_match_single(gpu, pq3d_lib, case, dtype, leaf):
             assert torch.equal(yi[:, b, :], single[b]), ("instance-major", nb, b)
 
 
+@pytest.mark.parametrize("leaf", [32, 64])
+def test_mfma_matvec_16_rhs(gpu, pq3d_lib, case, leaf):
+    """FP32 storage, 16 right-hand sides, dense and U products on FP32 matrix
+    cores: within the FP32 path's stated bound (1e-3 of the largest value)
+    of the dense FP64 product, within 1e-5 of the FP64-accumulated product of
+    the same storage (FP32 rounding of the weights and FP32 sums), identical
+    on repeat, and a permutation of the right-hand sides permutes the
+    outputs bit for bit (every column takes the same sequence of
+    operations). Leaf 32 tiles use two 16-row blocks, leaf 64 four."""
+    mesh, K = case
+    H = HMatrix.from_mesh(mesh, MU, LAM, leaf=leaf, eta=2.0, eps=1e-4, dtype=torch.float32,
+                          device=gpu)
+    assert H.stats.n_lowrank > 0
+    n = H.n
+    rng = np.random.default_rng(17)
+    V1, V2 = rng.standard_normal((n, 16)), rng.standard_normal((n, 16))
+    V1[:, 3] = np.exp(-((H.xg[:, 0] / 8e3) ** 2)) * 1e-6       # a smooth deficit-like column
+    V2[:, 3] = 0.1 * V1[:, 3]
+    X1 = torch.from_numpy(np.ascontiguousarray(V1[H.perm])).to(gpu)
+    X2 = torch.from_numpy(np.ascontiguousarray(V2[H.perm])).to(gpu)
+    y64 = H.matvec(X1.view(-1), X2.view(-1), nrhs=16).view(3, n, 16).cpu().numpy()
+    H.accumulate = "mfma"
+    y = H.matvec(X1.view(-1), X2.view(-1), nrhs=16).view(3, n, 16)
+    assert torch.equal(y, H.matvec(X1.view(-1), X2.view(-1), nrhs=16).view(3, n, 16))
+    yi = H.matvec(X1.view(-1), X2.view(-1), nrhs=16, instance_major=True).view(3, 16, n)
+    assert torch.equal(yi.transpose(1, 2), y)
+    y = y.cpu().numpy()
+    for b in range(16):
+        ref = _dense(K, V1[:, b], V2[:, b])
+        got = np.empty_like(y[:, :, b])
+        got[:, H.perm] = y[:, :, b]
+        assert _relerr(got, ref) < 1e-3, b
+        assert _relerr(y[:, :, b], y64[:, :, b]) < 1e-5, b
+    p = rng.permutation(16)
+    yp = H.matvec(X1[:, p].contiguous().view(-1), X2[:, p].contiguous().view(-1),
+                  nrhs=16).view(3, n, 16).cpu().numpy()
+    assert np.array_equal(yp, y[:, :, p])
+    # other sizes and FP64 storage keep the FP64-accumulating kernels
+    y1 = H.matvec(X1[:, 0].contiguous(), X2[:, 0].contiguous()).cpu().numpy()
+    assert np.array_equal(y1, y64[:, :, 0])
+
+
 def test_halfspace_hmatrix_matches_dense_halfspace_operators(gpu, pq3d_lib):
     """Half-space H-matrix of a surface-breaking fault: all-dense storage
     equals the half-space traction block; ACA at eps 1e-7 stays within 1e-5
@@ -175,6 +217,21 @@ def test_full_size_fp32_hmatrix_sampled_rows(gpu, pq3d_lib):
         ref = torch.stack([Kr[c] @ x1 + Kr[3 + c] @ x2 for c in range(3)]).cpu().numpy()
         y = _apply(H, v1, v2)[:, rows]
         assert _relerr(y, ref) < 1e-3
+    # 16 right-hand sides on FP32 matrix cores: scaled copies of the two vectors
+    H.accumulate = "mfma"
+    v = [rng.standard_normal(n), smooth]
+    sc = 10.0 ** rng.uniform(-3, 3, 16)
+    V1 = np.stack([sc[b] * v[b % 2] for b in range(16)], axis=1)
+    V2 = np.stack([0.5 * sc[b] * v[(b + 1) % 2] for b in range(16)], axis=1)
+    X1 = torch.from_numpy(np.ascontiguousarray(V1[H.perm])).to(gpu)
+    X2 = torch.from_numpy(np.ascontiguousarray(V2[H.perm])).to(gpu)
+    Y = H.matvec(X1.view(-1), X2.view(-1), nrhs=16, instance_major=True).view(3, 16, n)
+    Y = Y[:, :, torch.from_numpy(H.iperm).to(gpu)][:, :, torch.from_numpy(rows).to(gpu)]
+    for b in range(16):
+        x1 = torch.from_numpy(V1[:, b]).to(gpu)
+        x2 = torch.from_numpy(V2[:, b]).to(gpu)
+        ref = torch.stack([Kr[c] @ x1 + Kr[3 + c] @ x2 for c in range(3)]).cpu().numpy()
+        assert _relerr(Y[:, b].cpu().numpy(), ref) < 1e-3, b
 
 
 def test_saved_hmatrix_reloads_bit_for_bit(gpu, pq3d_lib, case, tmp_path):
