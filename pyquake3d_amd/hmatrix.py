@@ -626,6 +626,7 @@ class MultiHMatrix:
         self.device = hms[0].device
         self.dtype = hms[0].dtype
         self.esize = hms[0].esize
+        self.accumulate = "f64"
         self.sizes = np.array([h.n for h in hms], dtype=np.int64)
         self.off = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
         self.n = int(self.off[-1])
