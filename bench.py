@@ -95,7 +95,7 @@ def build_hmatrix(problem, dtype):
     return H, time.perf_counter() - t0
 
 
-def run_variant(problems, H, steps, warmup, name, rounds=8):
+def run_variant(problems, H, steps, warmup, name, rounds=8, graph=False):
     """``problems`` (NB friction draws on one mesh) stepped by DeviceEnsemble:
     one NB-RHS H-matvec, ``rounds`` RK45 attempt rounds with the reference's
     accept/reject decision on the device, and the commit per iteration, with
@@ -112,7 +112,11 @@ def run_variant(problems, H, steps, warmup, name, rounds=8):
     ens.sync()
     t_sim0 = ens.time.copy()
     acc0 = ens.naccepted.copy()
-    ens.timing = True
+    if graph:
+        # every timed step replays one recorded HIP graph of the step
+        ens.capture()
+    else:
+        ens.timing = True
     ens.matvec_events = []
     w0 = time.perf_counter()
     for _ in range(steps):
@@ -128,7 +132,7 @@ def run_variant(problems, H, steps, warmup, name, rounds=8):
                sim_years=float(yrs.sum()), wall_s=wall, ms_per_step=1e3 * wall / steps,
                accepted_steps_per_draw=float(acc.mean()),
                years_per_hour=float(yrs.sum()) / (wall / 3600.0),
-               matvec_ms=sum(mv) / len(mv),
+               matvec_ms=sum(mv) / len(mv) if mv else None,
                matvec_bytes=H.matvec_bytes + 5 * 8 * H.n * (nb - 1),
                t_end_years=float((ens.time / YEAR).mean()),
                max_slipv=float(ens.max_slipv().max()))
@@ -189,6 +193,13 @@ def bench_hstep(args):
                          for k in range(1, args.nb)]
     batched, _ = run_variant(draws, H32, args.steps, args.warmup, f"batched_fp32_nb{args.nb}")
     head = batched
+    graphed = None
+    if not args.no_graph:
+        # the same ensemble, each step replaying one recorded HIP graph
+        graphed, _ = run_variant(draws, H32, args.steps, args.warmup,
+                                 f"batched_fp32_nb{args.nb}_graph", graph=True)
+        graphed["matvec_ms"] = batched["matvec_ms"]    # same kernels, timed in the eager run
+        head = graphed
     batched_mfma = None
     if args.nb == 16:
         # the same ensemble with the 16-RHS block products on FP32 matrix cores
@@ -198,6 +209,7 @@ def bench_hstep(args):
         if args.accumulate == "mfma":
             head = batched_mfma
     mf = head is batched_mfma
+    gr = head is graphed
     achieved = head["matvec_bytes"] / (head["matvec_ms"] * 1e-3) / 1e9
     res = {
         "metric": "sim-years/wall-hour at N=1e5 (fp32+batched+learned dt vs plain fp64)",
@@ -223,6 +235,7 @@ def bench_hstep(args):
                                f": {args.nb} friction draws sharing the operator, "
                                f"one {args.nb}-RHS H-matvec per step"
                                + (" (dense and U products on FP32 matrix cores)" if mf else "")
+                               + (", each step one replay of a recorded HIP graph" if gr else "")
                                + ", reference step controller per draw run on the device "
                                  "(learned step size not in this line)")},
         "speedup_vs_plain_fp64": head["years_per_hour"] / plain["years_per_hour"],
@@ -230,6 +243,7 @@ def bench_hstep(args):
         "fast_fp32": fast,
         "batched_fp32": batched,
         "batched_fp32_mfma": batched_mfma,
+        "batched_fp32_graph": graphed,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / MI355X_HBM_GBS,
                      "traffic": MEASURED_TRAFFIC.get(("f32-mfma" if mf else "f32", args.nb)),
@@ -833,6 +847,8 @@ def main():
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"],
                     help="operator storage of the matvec workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="hstep: time the headline ensemble with eager launches only")
     ap.add_argument("--accumulate", default="f64", choices=["f64", "mfma"],
                     help="16 right-hand sides over FP32 storage: 'mfma' runs the dense and U "
                          "products on FP32 matrix cores (hstep headline and matvec)")

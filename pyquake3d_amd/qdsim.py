@@ -537,7 +537,34 @@ class DeviceEnsemble(EnsembleQDsim):
         self.iterations = 0
         self.naccepted = np.zeros(nb, dtype=np.int64)
 
+    _graph = None
+
+    def capture(self):
+        """Record one step (operator product, attempt rounds, commit/advance:
+        device launches on fixed buffers, no host synchronisation) as a HIP
+        graph; later ``step()`` calls replay it. Recording runs nothing, so
+        the state is unchanged. Call after at least one eager step (the
+        matvec scratch buffers then exist), with ``timing`` off."""
+        if self.timing:
+            raise ValueError("capture needs timing off (events are not recorded into the graph)")
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._enqueue_step()
+        torch.cuda.synchronize(self.device)
+        self._graph = g
+
+    def release_graph(self):
+        self._graph = None
+
     def step(self):
+        if self._graph is not None:
+            self._graph.replay()
+        else:
+            self._enqueue_step()
+        self.iterations += 1
+
+    def _enqueue_step(self):
         s = _lib.stream_handle()
         self.compute_K()
         _lib.call("pq3d_rk45_rounds", self.ne, _lib.ptr(self.inst), _lib.ptr(self.hcur),
@@ -558,7 +585,6 @@ class DeviceEnsemble(EnsembleQDsim):
                   _lib.ptr(self.slipv1), _lib.ptr(self.slipv2), _lib.ptr(self.slipv),
                   _lib.ptr(self.rake), _lib.ptr(self.slip1), _lib.ptr(self.slip2),
                   _lib.ptr(self.vmax), self.nb, s)
-        self.iterations += 1
 
     DEVICE_FIELDS = ("hcur", "dtnext", "flag", "nrj", "status", "tdev", "nacc")
 
