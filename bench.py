@@ -4,17 +4,21 @@
 
 Workloads
   hstep     (default) H-matrix time stepping at N = 100,352 triangles on the
-            undulating 100 x 25 km fault with BP5-like heterogeneous friction
-            (leaf 64, eta 2, ACA eps 1e-4). A step is one accepted RK45 time
-            step (one six-operator H-matvec, the RK45 attempts, commit). The
-            metric is simulated years per wall-hour over K timed steps after W
-            warm-up steps, all from the same seeded initial state:
-              plain   : FP64 operator storage, one simulation
-              fast    : FP32 operator storage, one simulation
-              batched : FP32 storage, --nb friction draws (default 16) on the
-                        same mesh stepped together, one multi-RHS H-matvec
-                        per step; rate = sum over the draws
-            ``value`` is the batched variant; all three are reported.
+            undulating 100 x 25 km fault (leaf 64, eta 2, ACA eps 1e-4),
+            BP5-like heterogeneous friction with Dc x 0.7 and the normal
+            traction fixed. Every draw is integrated from its seeded initial
+            state over [0, 380] years (>= 3 whole seismic cycles) whatever
+            --steps is; the metric is simulated years per wall-hour over that
+            interval:
+              plain_fp64   : FP64 operator storage, one simulation, the
+                             reference controller (the baseline)
+              fp32_batched : FP32 operator storage, --nb friction draws
+                             (default 16) sharing one multi-RHS H-matvec per
+                             step, learned first-attempt step size; rate =
+                             sum over the draws (``value``)
+              fp64_nb16    : every draw in FP64 with the reference
+                             controller: the event-onset reference of each
+                             draw (reported per draw)
   example   The reference's BP5-QD example model (N = 9,250, half space)
             from its initial state: dense FP64 vs FP64 / FP32 H-matrices,
             event times and simulated years per wall-hour.
@@ -24,9 +28,11 @@ Workloads
   ragged    --count simulations (default 64) on meshes of uneven size as
             one block-diagonal FP32 H-matrix with the device controller;
             per-instance accuracy of every 8th against its FP64 solo run.
-  learned   Learned step size against the reference controller at equal
-            rtol (--nb draws, default 8): rejected-step fraction and wall
-            time of each over --steps steps from the same state.
+  learned   The step-size network at its configured scale: ~1e6 training
+            rows from the reference controller on the N = 20,000 mesh,
+            training on the GPU, then ten seismic cycles at N = 100,352 with
+            the reference controller and with the learned proposal:
+            rejected-step fraction and wall time at equal rtol.
   matvec    H-matvec only, --nb right-hand sides, on the hstep operator
             (--dtype f32|f64); the command the hardware-counter profiles use.
   assembly  Kernel assembly only: all N x N pairs at N = 20,000 (FP64, six
@@ -95,55 +101,145 @@ def build_hmatrix(problem, dtype):
     return H, time.perf_counter() - t0
 
 
-def run_variant(problems, H, steps, warmup, name, rounds=8, graph=False):
-    """``problems`` (NB friction draws on one mesh) stepped by DeviceEnsemble:
-    one NB-RHS H-matvec, ``rounds`` RK45 attempt rounds with the reference's
-    accept/reject decision on the device, and the commit per iteration, with
-    no host synchronisation inside the timed region. The rate is the sum of
-    the draws' simulated years per wall-hour."""
+# The config-3 workload of the headline: BP5-like friction draws on the
+# undulating fault with Dc scaled by 0.7 and the normal traction fixed
+# (the reference's Fix_Tn switch, src/QDsim.py:906-909). With the normal
+# traction coupled, every rupture on this geometry (300 m undulation over
+# 5 km) drives the normal traction through zero within the first event; the
+# rate-and-state law then overflows (V ~ sinh(tau / (a sigma))) and the
+# reference's abort rule ends the run. With it fixed, every draw completes
+# seismic cycles of ~110 years (peak slip rates 1-2 m/s) under the
+# reference controller.
+C3_DC_SCALE = 0.7
+C3_T_END_YEARS = 380.0       # every draw has had >= 4 events (3 whole cycles) by then
+EVENT_VUP, EVENT_VDOWN = 1e-2, 1e-4   # seismic onset and re-arm thresholds (m/s)
+EVENT_MERGE_YEARS = 1.0      # onsets closer than this belong to one rupture sequence
+STEP_MLP = REPO / "pyquake3d_amd" / "data" / "step_mlp_c2.npz"
+
+
+def c3_draws(nb, nx=448, nz=112, dc_scale=C3_DC_SCALE):
+    """NB friction draws (seeds 0..NB-1) on one undulating 100 x 25 km mesh
+    (448 x 112 rectangles -> N = 100,352; 200 x 50 -> the N = 20,000 mesh
+    of the assembly workload), Dc scaled by ``dc_scale``, normal traction
+    fixed."""
+    from dataclasses import replace
+
+    from pyquake3d_amd.problem import undulating_c3
+    p0 = undulating_c3(nx=nx, nz=nz)
+    draws = [p0] + [undulating_c3(nx=nx, nz=nz, friction_seed=k, mesh=p0.mesh)
+                    for k in range(1, nb)]
+    return [replace(p, dc=p.dc * dc_scale, fix_Tn=True) for p in draws]
+
+
+def merge_events(ev_s):
+    """Event onsets (s) -> onsets in years with sequences closer than
+    EVENT_MERGE_YEARS merged into their first onset."""
+    out = []
+    for t in sorted(float(x) / YEAR for x in ev_s):
+        if not out or t - out[-1] > EVENT_MERGE_YEARS:
+            out.append(t)
+    return out
+
+
+def run_cycles(problems, H, name, t_end_years, warmup, rounds=8, mlp=None, timing_every=0,
+               check_every=100, max_wall=900.0):
+    """``problems`` (friction draws on the operator's mesh) stepped by the
+    device controller from their initial state until every draw reaches
+    ``t_end_years`` (a draw that gets there first stops taking steps; the
+    shared product still runs for all). ``warmup`` iterations are run first
+    and the state restored, so the timed region starts from t = 0. Events
+    are detected on the device. Returns the result dict and the ensemble."""
+    import numpy as np
     import torch
 
-    from pyquake3d_amd.qdsim import DeviceEnsemble
+    from pyquake3d_amd.qdsim import DeviceEnsemble, StepAborted
 
-    ens = DeviceEnsemble(problems, H, device=H.device, rounds=rounds)
+    ens = DeviceEnsemble(problems, H, device=H.device, rounds=rounds, t_end=t_end_years * YEAR,
+                         vup=EVENT_VUP, vdown=EVENT_VDOWN, mlp=mlp)
+    s0 = ens.save_state()
     for _ in range(warmup):
         ens.step()
     torch.cuda.synchronize()
-    ens.sync()
-    t_sim0 = ens.time.copy()
-    acc0 = ens.naccepted.copy()
-    if graph:
-        # every timed step replays one recorded HIP graph of the step
-        ens.capture()
-    else:
+    ens.load_state(s0)
+    ens.iterations = 0
+    if timing_every:
         ens.timing = True
+        ens.timing_every = timing_every
     ens.matvec_events = []
+    it = 0
     w0 = time.perf_counter()
-    for _ in range(steps):
+    last = w0
+    while True:
         ens.step()
+        it += 1
+        if it % check_every == 0:
+            fl = ens.flag.cpu().numpy()
+            now = time.perf_counter()
+            if np.all(fl >= 2) or now - w0 > max_wall:
+                break
+            if now - last > 30.0:
+                last = now
+                tt = ens.tdev.cpu().numpy() / YEAR
+                log(f"{name}: {it} iterations, t = {tt.min():.1f}..{tt.max():.1f} yr, "
+                    f"{now - w0:.0f} s")
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
-    ens.sync()
-    mv = ens.matvec_ms()
-    yrs = (ens.time - t_sim0) / YEAR
+    try:
+        ens.sync()
+        status = "ok"
+    except StepAborted as e:
+        status = f"abort: {e}"
+    mv = ens.matvec_ms() if timing_every else []
     nb = len(problems)
-    acc = ens.naccepted - acc0
+    yrs = ens.time / YEAR
+    ev = [merge_events(e) for e in ens.events()]
+    rej = ens.rejected()
+    acc = ens.naccepted
+    planes = 2 if problems[0].fix_Tn else 3
     res = dict(name=name, dtype=str(H.dtype).replace("torch.", ""), nb=nb, rounds=rounds,
-               sim_years=float(yrs.sum()), wall_s=wall, ms_per_step=1e3 * wall / steps,
-               accepted_steps_per_draw=float(acc.mean()),
+               learned=mlp is not None, iterations=it, wall_s=wall,
+               ms_per_iteration=1e3 * wall / it, sim_years=float(yrs.sum()),
                years_per_hour=float(yrs.sum()) / (wall / 3600.0),
-               matvec_ms=sum(mv) / len(mv) if mv else None,
-               matvec_bytes=H.matvec_bytes + 5 * 8 * H.n * (nb - 1),
-               t_end_years=float((ens.time / YEAR).mean()),
-               max_slipv=float(ens.max_slipv().max()))
-    log(f"{name}: {json.dumps(res)}")
+               t_end_years=[float(x) for x in yrs], finished=ens.finished().tolist(),
+               aborted=ens.aborted().tolist(), status=status,
+               accepted_steps=[int(x) for x in acc], rejected_attempts=[int(x) for x in rej],
+               rejected_fraction=float(rej.sum() / max(rej.sum() + acc.sum(), 1)),
+               events_years=ev, events_per_draw=[len(e) for e in ev],
+               peak_slip_rate=[float(x) for x in ens.peak_slipv()],
+               matvec_ms=float(np.mean(mv)) if mv else None, matvec_samples=len(mv),
+               matvec_bytes=H.algorithmic_bytes(ens.nrhs, planes))
+    log(f"{name}: {json.dumps({k: v for k, v in res.items() if k not in ('events_years', 't_end_years', 'accepted_steps', 'rejected_attempts', 'peak_slip_rate', 'finished', 'aborted')})}")
     return res, ens
 
 
-def cpu_baseline_hstep(problem, H, gpu_res, nsteps=4):
-    """Reference-style CPU step (per-block numpy FP64 matvec + numpy RK45) on
-    the same blocks, timed over ``nsteps`` steps from the initial state; the
-    rate is expressed for the GPU run's simulated interval."""
+def event_errors(fast, ref):
+    """Per draw: paired event onsets (k-th with k-th) of a run against the
+    FP64 reference run, relative errors of the onset times and of the
+    recurrence intervals, peak slip rates."""
+    import numpy as np
+    out = []
+    for b, (ef, er) in enumerate(zip(fast["events_years"], ref["events_years"])):
+        k = min(len(ef), len(er))
+        rel = [abs(ef[i] - er[i]) / er[i] for i in range(k)]
+        rf, rr = np.diff(ef[:k]), np.diff(er[:k])
+        out.append({"draw": b, "events": len(ef), "events_fp64": len(er),
+                    "onsets_yr": [round(x, 4) for x in ef[:k]],
+                    "onsets_fp64_yr": [round(x, 4) for x in er[:k]],
+                    "onset_rel_err": [float(x) for x in rel],
+                    "max_onset_rel_err_after_first": float(max(rel[1:])) if k > 1 else None,
+                    "max_recurrence_rel_err": (float(np.max(np.abs(rf - rr) / rr))
+                                               if k > 1 else None),
+                    "peak_log10_slip_rate": float(np.log10(fast["peak_slip_rate"][b])),
+                    "peak_log10_slip_rate_fp64": float(np.log10(ref["peak_slip_rate"][b]))})
+    return out
+
+
+def cpu_baseline_hstep(problem, H, gpu_res, nsteps=3):
+    """Reference-style CPU step (per-block numpy FP64 matvec of the four A
+    operators, as the reference applies with a fixed normal traction, and the
+    numpy RK45 step) on the same blocks, timed over ``nsteps`` steps from
+    the initial state; the rate is expressed for the GPU run's simulated
+    interval and accepted steps."""
     import numpy as np
 
     from oracle.hmat_cpu import CpuHMatrix
@@ -154,105 +250,114 @@ def cpu_baseline_hstep(problem, H, gpu_res, nsteps=4):
     perm = H.perm
     init = [np.asarray(x)[perm] for x in problem.initial_state()]
     g = lambda x: np.asarray(x)[perm]
+    planes = 2 if problem.fix_Tn else 3
     ora = OracleSim(None, g(problem.a), g(problem.b), g(problem.dc), g(problem.f0), problem.V0,
                     problem.mu, problem.Cs, g(problem.Vpl), g(problem.rake), init,
-                    matvec=C.matvec)
+                    normal_loading=g(problem.normal_loading), fix_Tn=problem.fix_Tn,
+                    matvec=lambda v1, v2: C.matvec(v1, v2, planes=planes))
     dt = 1e-3
     t0 = time.perf_counter()
     for _ in range(nsteps):
         _, dt = ora.simu_forward(dt)
     per_step = (time.perf_counter() - t0) / nsteps
-    steps = gpu_res["wall_s"] / (gpu_res["ms_per_step"] / 1e3)
+    steps = gpu_res["accepted_steps"][0]
     cpu_hours = per_step * steps / 3600.0
     return {"value": gpu_res["sim_years"] / cpu_hours, "unit": "sim-years/wall-hour",
             "cores": 1, "kind": "port",
             "sample": f"{nsteps} steps of the reference-style CPU step (per-block numpy FP64 "
-                      f"matvec over the same {len(C.dense)} dense + {len(C.lowrank)} low-rank "
-                      f"blocks, numpy RK45) at {per_step:.2f} s/step, applied to the GPU run's "
-                      f"{steps:.0f} steps / {gpu_res['sim_years']:.2f} simulated years; "
-                      f"host setup {setup:.1f} s"}
+                      f"matvec of the {2 * planes} operators over the same {len(C.dense)} dense "
+                      f"+ {len(C.lowrank)} low-rank blocks, numpy RK45) at {per_step:.2f} "
+                      f"s/step, applied to the plain GPU run's {steps} accepted steps / "
+                      f"{gpu_res['sim_years']:.1f} simulated years; host setup {setup:.1f} s"}
+
+
+def load_step_mlp(dev):
+    from pyquake3d_amd.learned_dt import StepMLP
+    if not STEP_MLP.exists():
+        raise RuntimeError(f"{STEP_MLP} is missing: train it with `python bench.py --workload "
+                           f"learned` (it writes the network there)")
+    return StepMLP.load(STEP_MLP, device=dev)
 
 
 def bench_hstep(args):
+    """The headline: sim-years per wall-hour over whole seismic cycles at
+    N = 100,352, plain FP64 (one simulation, FP64 operator storage, the
+    reference controller) against the metric's variant (FP32 storage,
+    --nb draws batched into one multi-RHS product per step, learned
+    step-size proposal), each integrating [0, C3_T_END_YEARS] of every
+    draw whatever --steps is; event onsets of every draw of the variant are
+    compared with the same draw's FP64 run."""
     import torch
 
-    from pyquake3d_amd.problem import undulating_c3
-
-    problem = undulating_c3()
-    log(f"hstep: N={problem.n}")
+    draws = c3_draws(args.nb)
+    problem = draws[0]
+    t_end = args.t_end_years
+    log(f"hstep: N={problem.n}, {args.nb} draws, [0, {t_end} yr]")
+    dev = torch.device("cuda:0")
+    mlp = None if args.no_learned else load_step_mlp(dev)
     H64, b64 = build_hmatrix(problem, torch.float64)
-    plain, _ = run_variant([problem], H64, args.steps, args.warmup, "plain_fp64")
+    plain, _ = run_cycles([problem], H64, "plain_fp64", t_end, args.warmup, timing_every=20)
     plain["build_s"] = b64
+    ref64 = None
+    if not args.no_accuracy:
+        # FP64 reference of every draw: the batched FP64 product is bitwise the
+        # single-draw product, so draw b here is draw b's FP64 solo run
+        ref64, _ = run_cycles(draws, H64, f"fp64_nb{args.nb}", t_end, args.warmup)
     del H64
     torch.cuda.empty_cache()
     H32, b32 = build_hmatrix(problem, torch.float32)
-    fast, _ = run_variant([problem], H32, args.steps, args.warmup, "fast_fp32")
+    fast, _ = run_cycles(draws, H32, f"fp32_nb{args.nb}" + ("_learned" if mlp else ""), t_end,
+                         args.warmup, mlp=mlp, timing_every=20)
     fast["build_s"] = b32
-    # friction draws 0..NB-1 on the same mesh; draw 0 is the single-run problem
-    draws = [problem] + [undulating_c3(friction_seed=k, mesh=problem.mesh)
-                         for k in range(1, args.nb)]
-    batched, _ = run_variant(draws, H32, args.steps, args.warmup, f"batched_fp32_nb{args.nb}")
-    head = batched
-    graphed = None
-    if not args.no_graph:
-        # the same ensemble, each step replaying one recorded HIP graph
-        graphed, _ = run_variant(draws, H32, args.steps, args.warmup,
-                                 f"batched_fp32_nb{args.nb}_graph", graph=True)
-        graphed["matvec_ms"] = batched["matvec_ms"]    # same kernels, timed in the eager run
-        head = graphed
-    batched_mfma = None
-    if args.nb == 16:
-        # the same ensemble with the 16-RHS block products on FP32 matrix cores
-        H32.accumulate = "mfma"
-        batched_mfma, _ = run_variant(draws, H32, args.steps, args.warmup, "batched_fp32_mfma")
-        H32.accumulate = "f64"
-        if args.accumulate == "mfma":
-            head = batched_mfma
-    mf = head is batched_mfma
-    gr = head is graphed
-    achieved = head["matvec_bytes"] / (head["matvec_ms"] * 1e-3) / 1e9
+    achieved = fast["matvec_bytes"] / (fast["matvec_ms"] * 1e-3) / 1e9
+    acc = event_errors(fast, ref64) if ref64 else None
+    worst = None
+    if acc:
+        vals = [a["max_onset_rel_err_after_first"] for a in acc
+                if a["max_onset_rel_err_after_first"] is not None]
+        worst = max(vals) if vals else None
     res = {
         "metric": "sim-years/wall-hour at N=1e5 (fp32+batched+learned dt vs plain fp64)",
-        "value": head["years_per_hour"],
+        "value": fast["years_per_hour"],
         "unit": "sim-years/wall-hour",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": fast["iterations"],
         "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
+        "ms_per_step": fast["ms_per_iteration"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32 operator storage and matrix-core products (f32 sums per item group), "
-                  "f64 low-rank V products, state and RK45" if mf else
-                  "f32 operator storage, f64 state and accumulation"),
+        "dtype": "f32 operator storage, f64 state and accumulation; f32 step-size network",
         "data": "synthetic undulating fault mesh and seeded friction fields",
-        "config": {"workload": f"H-matrix RK45 time stepping, undulating fault 100x25 km, "
-                               f"N={problem.n}, leaf 64, eta 2, ACA eps 1e-4, BP5-like friction, "
-                               f"{args.steps} steps after {args.warmup} from the seeded initial "
-                               f"state (a step: one accepted RK45 step of every draw; a draw "
-                               f"that needs more than 8 attempts finishes in the next step)",
-                   "variant": (("batched_fp32_mfma" if mf else "batched_fp32") +
-                               f": {args.nb} friction draws sharing the operator, "
-                               f"one {args.nb}-RHS H-matvec per step"
-                               + (" (dense and U products on FP32 matrix cores)" if mf else "")
-                               + (", each step one replay of a recorded HIP graph" if gr else "")
-                               + ", reference step controller per draw run on the device "
-                                 "(learned step size not in this line)")},
-        "speedup_vs_plain_fp64": head["years_per_hour"] / plain["years_per_hour"],
+        "config": {"workload": f"H-matrix RK45 time stepping over whole seismic cycles, "
+                               f"undulating fault 100x25 km, N={problem.n}, leaf 64, eta 2, ACA "
+                               f"eps 1e-4, BP5-like friction draws with Dc x {C3_DC_SCALE}, "
+                               f"normal traction fixed (reference Fix_Tn), every draw "
+                               f"integrated over [0, {t_end:g}] yr (>= 3 whole cycles) "
+                               f"whatever --steps is; a step: one iteration of the device "
+                               f"controller (one product, up to 8 attempt rounds, commit)",
+                   "variant": f"fp32+batched+learned: {args.nb} friction draws sharing the "
+                              f"operator, one {args.nb}-RHS H-matvec per step, learned "
+                              f"first-attempt step size (RK45 estimator accepts or rejects, PI "
+                              f"fallback)" if mlp else
+                              f"fp32+batched: {args.nb} draws, reference controller",
+                   "baseline": "plain_fp64: one simulation (draw 0), FP64 operator storage, "
+                               "reference controller"},
+        "speedup_vs_plain_fp64": fast["years_per_hour"] / plain["years_per_hour"],
+        "event_onset_max_rel_err_after_first": worst,
         "plain_fp64": plain,
-        "fast_fp32": fast,
-        "batched_fp32": batched,
-        "batched_fp32_mfma": batched_mfma,
-        "batched_fp32_graph": graphed,
+        "fp32_batched": fast,
+        "fp64_batched_reference": ref64,
+        "event_accuracy_vs_fp64": acc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / MI355X_HBM_GBS,
-                     "traffic": MEASURED_TRAFFIC.get(("f32-mfma" if mf else "f32", args.nb)),
-                     "kernel": f"H-matvec, {args.nb} right-hand sides (k_hmv_phase1 + "
-                               f"k_hmv_phase1b + " + ("k_hmv_phase2_mfma16)" if mf else "k_hmv_phase2)"),
-                     "kernel_ms": head["matvec_ms"], "algorithmic_bytes": head["matvec_bytes"]},
+                     "traffic": MEASURED_TRAFFIC.get(("f32", args.nb, 2)),
+                     "kernel": f"H-matvec, {args.nb} right-hand sides, two row planes "
+                               f"(k_hmv_phase1 + k_hmv_phase1b + k_hmv_phase2)",
+                     "kernel_ms": fast["matvec_ms"], "algorithmic_bytes": fast["matvec_bytes"]},
     }
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline_hstep(problem, H32, fast)
+        res["cpu_baseline"] = cpu_baseline_hstep(problem, H32, plain)
     return res
 
 
@@ -336,77 +441,96 @@ def bench_matvec(args):
 
 
 # ------------------------------------------------------------- learned
-def _controller_run(ens, steps):
-    import torch
-    a0, s0, r0, t0 = ens.nattempts, ens.nsteps, int(ens.nrejects.sum()), ens.time.copy()
-    w0 = time.perf_counter()
-    for _ in range(steps):
-        ens.step()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    attempts = ens.nattempts - a0
-    rejected = int(ens.nrejects.sum()) - r0
-    accepted = (ens.nsteps - s0) * ens.nb
-    yrs = float(((ens.time - t0) / YEAR).sum())
-    return {"steps": ens.nsteps - s0, "attempt_rounds": attempts,
-            "accepted_steps": accepted, "rejected_steps": rejected,
-            "rejected_fraction": rejected / (accepted + rejected),
-            "attempt_rounds_per_step": attempts / max(ens.nsteps - s0, 1),
-            "wall_s": wall, "sim_years": yrs, "years_per_hour": yrs / (wall / 3600.0)}
-
-
 def bench_learned(args):
-    """Learned first-attempt step size against the reference controller at
-    equal rtol (1e-4) on the hstep problem: --nb friction draws, FP32
-    H-matrix at N = 100,352. The reference controller runs --record steps
-    and logs (features, log10 h_accepted/dtnext) pairs; the MLP (hand-written
-    forward/backward/Adam kernels) trains on them; then both controllers run
-    --steps steps from the same saved state. Both loops take their accept or
-    reject decision on the host after every attempt round, so the wall times
-    compare the controllers, not the drivers."""
+    """The learned step-size proposal at its configured scale.
+
+    1. Training rows from the reference controller on the device: --nb
+       friction draws of the config-3 friction on the N = 20,000 mesh of the
+       same undulating geometry (200 x 50 rectangles), FP32 H-matrix,
+       --record iterations; every accepted step logs (six features of its
+       start, log10 h_accepted / dtnext) -> about --nb x --record rows
+       through interseismic, nucleation and rupture phases.
+    2. Training on the GPU with the library's forward/backward/Adam kernels;
+       the network is written to --mlp-out.
+    3. Ten seismic cycles at N = 100,352 (the hstep draws, [0, --cycle-years]
+       yr), plain reference controller against the learned proposal (RK45
+       estimator accepts or rejects every attempt, PI fallback), both on the
+       device controller at the same rtol (1e-4) and rounds: rejected-step
+       fraction, wall time, accepted steps, event onsets of every draw."""
+    import numpy as np
     import torch
 
-    from pyquake3d_amd.learned_dt import FeatureEnsemble, StepMLP
-    from pyquake3d_amd.problem import undulating_c3
+    from pyquake3d_amd.learned_dt import StepMLP
+    from pyquake3d_amd.qdsim import DeviceEnsemble
 
-    problem = undulating_c3()
-    draws = [problem] + [undulating_c3(friction_seed=k, mesh=problem.mesh)
-                         for k in range(1, args.nb)]
-    H, _ = build_hmatrix(problem, torch.float32)
-    rec = FeatureEnsemble(draws, H, device=H.device)
-    rec.record = True
-    for _ in range(args.record):
+    dev = torch.device("cuda:0")
+    small = c3_draws(args.nb, nx=200, nz=50)
+    Hs, _ = build_hmatrix(small[0], torch.float32)
+    rec = DeviceEnsemble(small, Hs, device=dev, rounds=8, record=args.nb * args.record)
+    w0 = time.perf_counter()
+    last = w0
+    for i in range(args.record):
         rec.step()
-    X, T = rec.dataset()
-    mlp = StepMLP(device=H.device, seed=0)
+        if i % 2000 == 1999 and time.perf_counter() - last > 30:
+            last = time.perf_counter()
+            rec.sync()
+            log(f"learned: recording {i + 1}/{args.record}, t = "
+                f"{rec.time.min() / YEAR:.1f}..{rec.time.max() / YEAR:.1f} yr")
+    rec.sync()
+    rec_s = time.perf_counter() - w0
+    X, T = rec.samples()
+    rec_years = float(rec.time.mean() / YEAR)
+    rec_events = [len(merge_events(e)) for e in rec.events()]
+    del rec, Hs
+    torch.cuda.empty_cache()
+    log(f"learned: {len(T)} rows from {args.nb} draws over {rec_years:.0f} yr "
+        f"({rec_s:.0f} s), events per draw {rec_events}")
+    mlp = StepMLP(device=dev, seed=0)
     t0 = time.perf_counter()
-    hist = mlp.fit(X, T, epochs=40, batch=1024, lr=2e-3)
+    hist = mlp.fit(X, T, epochs=args.epochs, batch=4096, lr=2e-3)
+    torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
-    saved = rec.save_state()
-    rec.record = False
-    plain = _controller_run(rec, args.steps)
-    le = FeatureEnsemble(draws, H, mlp=mlp, device=H.device)
-    le.load_state(saved)
-    learned = _controller_run(le, args.steps)
-    learned.update(first_attempt_rejected=le.nfirst_rejected, pi_fallback_attempts=le.nfallback,
-                   max_accepted_error_ratio=le.max_accepted_ratio)
-    log(f"learned: plain {json.dumps(plain)} learned {json.dumps(learned)}")
+    pred = mlp.predict_log_dt(X[:200000])
+    fit = {"rows": int(len(T)), "epochs": args.epochs, "loss_first": float(hist[0]),
+           "loss_last": float(hist[-1]), "train_s": train_s,
+           "target_mean": float(T.mean()), "target_sd": float(T.std()),
+           "rms_error_log10": float(np.sqrt(np.mean((pred - T[:200000]) ** 2))),
+           "recorded_years_per_draw": rec_years, "recorded_events_per_draw": rec_events}
+    out = Path(args.mlp_out)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    mlp.save(out, rows=len(T), epochs=args.epochs, source=f"{args.nb} draws, N=20000 "
+             f"undulating mesh, Dc x {C3_DC_SCALE}, fixed normal traction, {args.record} "
+             f"iterations of the reference controller")
+    log(f"learned: trained ({json.dumps(fit)}), written to {out}")
+    draws = c3_draws(args.nb)
+    H, _ = build_hmatrix(draws[0], torch.float32)
+    plain, _ = run_cycles(draws, H, "plain_controller", args.cycle_years, args.warmup,
+                          max_wall=args.max_wall)
+    learned, _ = run_cycles(draws, H, "learned_controller", args.cycle_years, args.warmup,
+                            mlp=mlp, max_wall=args.max_wall)
+    acc = event_errors(learned, plain)
     return {
-        "metric": "wall-time ratio, plain / learned step-size controller at equal rtol "
-                  "(N=1e5, rejected-step fractions alongside)",
+        "metric": "wall-time ratio over ten seismic cycles, reference / learned step-size "
+                  "controller at equal rtol (N=1e5, rejected-step fractions alongside)",
         "value": plain["wall_s"] / learned["wall_s"], "unit": "x",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.record,
-        "ms_per_step": 1e3 * learned["wall_s"] / args.steps,
+        "n_gpus": 1, "steps": learned["iterations"], "warmup": args.warmup,
+        "ms_per_step": learned["ms_per_iteration"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 operator storage, f64 state; f32 MLP",
-        "data": "synthetic undulating fault mesh and seeded friction fields",
-        "config": {"workload": f"learned step size, undulating fault 100x25 km, N={problem.n}, "
-                               f"{args.nb} friction draws, rtol 1e-4, {args.record} recorded "
-                               f"steps ({len(T)} samples), then {args.steps} steps per "
-                               f"controller from the same state"},
-        "plain": plain, "learned": learned,
-        "training": {"samples": int(len(T)), "epochs": 40, "loss_first": float(hist[0]),
-                     "loss_last": float(hist[-1]), "train_s": train_s},
+        "dtype": "f32 operator storage, f64 state; f32 network",
+        "data": "synthetic undulating fault meshes and seeded friction fields",
+        "config": {"workload": f"learned step size: trained on {len(T)} rows of the reference "
+                               f"controller ({args.nb} draws, N=20000, {args.record} "
+                               f"iterations), evaluated over [0, {args.cycle_years:g}] yr of "
+                               f"{args.nb} draws at N={draws[0].n}, rtol 1e-4, FP32 H-matrix, "
+                               f"device controller, 8 attempt rounds per iteration"},
+        "rejected_fraction": {"reference": plain["rejected_fraction"],
+                              "learned": learned["rejected_fraction"]},
+        "accepted_steps": {"reference": int(np.sum(plain["accepted_steps"])),
+                           "learned": int(np.sum(learned["accepted_steps"]))},
+        "sim_years_per_hour": {"reference": plain["years_per_hour"],
+                               "learned": learned["years_per_hour"]},
+        "training": fit, "reference": plain, "learned": learned,
+        "event_onsets_learned_vs_reference": acc,
     }
 
 
@@ -863,8 +987,20 @@ def main():
     ap.add_argument("--vdown", type=float, default=3e-9, help="accuracy: re-arm max V (m/s)")
     ap.add_argument("--tmax", type=float, default=150.0, help="accuracy: simulated years cap")
     ap.add_argument("--dc-scale", type=float, default=1.0, help="accuracy: factor on Dc")
-    ap.add_argument("--record", type=int, default=800,
-                    help="learned: steps of the reference controller logged for training")
+    ap.add_argument("--record", type=int, default=62500,
+                    help="learned: iterations of the reference controller logged for training")
+    ap.add_argument("--epochs", type=int, default=20, help="learned: training epochs")
+    ap.add_argument("--cycle-years", type=float, default=1120.0,
+                    help="learned: simulated years of the evaluation (about ten cycles)")
+    ap.add_argument("--mlp-out", default=str(STEP_MLP), help="learned: where the network goes")
+    ap.add_argument("--max-wall", type=float, default=900.0,
+                    help="wall-time cap (s) of one cycle run")
+    ap.add_argument("--t-end-years", type=float, default=C3_T_END_YEARS,
+                    help="hstep: simulated interval of every draw")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="hstep: the batched variant with the reference controller")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="hstep: skip the FP64 reference run of every draw")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the "
@@ -873,8 +1009,8 @@ def main():
     if args.workload != "learned":
         args.nb = 16 if args.nb is None else args.nb
     if args.workload == "hstep":
-        args.steps = 3000 if args.steps is None else args.steps
-        args.warmup = 200 if args.warmup is None else args.warmup
+        # the timed region is the whole simulated interval; --steps is ignored
+        args.warmup = 20 if args.warmup is None else args.warmup
         res = bench_hstep(args)
     elif args.workload == "example":
         args.steps = 20000 if args.steps is None else args.steps
@@ -887,8 +1023,8 @@ def main():
         args.warmup = 100 if args.warmup is None else args.warmup
         res = bench_ragged(args)
     elif args.workload == "learned":
-        args.steps = 1500 if args.steps is None else args.steps
-        args.nb = 8 if args.nb is None else args.nb
+        args.warmup = 20 if args.warmup is None else args.warmup
+        args.nb = 16 if args.nb is None else args.nb
         res = bench_learned(args)
     elif args.workload == "disp":
         args.steps = 3 if args.steps is None else args.steps

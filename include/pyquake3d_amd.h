@@ -228,7 +228,9 @@ int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nbl
  * (phase-1 units and their member blocks, phase-1b reductions, tiles, items)
  * as built by
  * pyquake3d_amd.hmatrix.build_worklists; d_t: nt*nrhs and d_part:
- * npart*nrhs doubles of scratch. */
+ * npart*nrhs doubles of scratch. d_y3 may be NULL: the Bs and Bd operators
+ * are then neither read nor applied, as in the reference's step with a fixed
+ * normal traction (src/QDsim.py:906-909, Fix_Tn). */
 int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
                      const int64_t* d_red, int64_t nred,
                      const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
@@ -312,7 +314,13 @@ int pq3d_rk45_commit(int64_t n, const int32_t* d_inst, const double* d_h,
  * d_flag 0 at d_h, then the reference's decision on device (accept: flag 1,
  * d_dtnext = min(hn1, hn2, 1.5 h); reject: h = max(0.5 h, min(hn1, hn2));
  * flag 2 and d_status 1 when h < 1e-15 or after 20 rejections). No host
- * synchronisation; the operator products must be current for the step. */
+ * synchronisation; the operator products must be current for the step.
+ * d_lstate: NULL, or one learned-state record per instance (PQ3D_LS_W
+ * doubles: learned flag, last accepted max(c1, c2), last dtnext/h, dtnext
+ * base, six features). With it, an accepted step stores its max(c1, c2) and
+ * dtnext/h, and a rejected attempt of a step that began with a learned
+ * proposal takes the PI fallback h 0.9 (1/err)^0.14 prev^0.08 within
+ * [0.2 h, 0.9 h] (no reference counterpart; the reference rule otherwise). */
 int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
                      int32_t* d_nrj, double* d_dtnext, int32_t* d_status, int rounds,
                      double tol, const double* d_Tno, const double* d_Tt1o, const double* d_Tt2o,
@@ -322,12 +330,25 @@ int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_f
                      const double* d_shear_loading, const double* d_slipv,
                      const double* d_cellpar, double* d_yTno, double* d_yTt1o, double* d_yTt2o,
                      double* d_ystate, unsigned long long* d_err, double a63, int64_t n_inst,
-                     void* stream);
+                     double* d_lstate, void* stream);
 
 /* Commit the instances with d_flag 1 (as pq3d_rk45_commit, the state update
  * of src/QDsim.py:960-993), then t += h, h = dtnext, rejection count 0,
  * accepted-step count + 1, flag 0. Instances still attempting keep state, h
- * and count; they continue in the next pq3d_rk45_rounds of the same step. */
+ * and count; they continue in the next pq3d_rk45_rounds of the same step.
+ * Optional (NULL: unused, none has a reference counterpart; they replace
+ * the driver loop's own bookkeeping, src/main.py:157-223):
+ *   d_run     per-instance run record (PQ3D_RUN_W doubles): [0] end time
+ *             (once t >= it the instance finishes: flag 3, no more attempts
+ *             or commits), [1] armed, [2] event count, [3] peak slip rate,
+ *             [4] event threshold up, [5] re-arm threshold down, [6]
+ *             rejected attempts of the accepted steps, [8..] event start
+ *             times (t after the first step whose max slip rate exceeds [4]
+ *             while armed; re-armed below [5]);
+ *   d_samples with d_lstate: one training row per accepted step
+ *             (PQ3D_SAMPLE_W floats: the six features of the step's start,
+ *             log10(h / dtnext base), 1) at row nacc*n_inst + b while it is
+ *             below sample_cap. */
 int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
                       int32_t* d_nrj, const double* d_dtnext, double* d_t, int64_t* d_nacc,
                       const double* d_a, const double* d_cellpar, const double* d_yTno,
@@ -335,7 +356,13 @@ int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_
                       double* d_Tno, double* d_Tt1o, double* d_Tt2o, double* d_state,
                       double* d_slipv1, double* d_slipv2, double* d_slipv, double* d_rake,
                       double* d_slip1, double* d_slip2, unsigned long long* d_vmax,
-                      int64_t n_inst, void* stream);
+                      double* d_run, const double* d_lstate, float* d_samples,
+                      int64_t sample_cap, int64_t n_inst, void* stream);
+
+/* Widths of the controller records, in this order: run record
+ * (PQ3D_RUN_W = 64), learned-state record (PQ3D_LS_W = 16), training row
+ * (PQ3D_SAMPLE_W = 8). Returns the number written (at most n). */
+int pq3d_controller_widths(int64_t* out, int64_t n);
 
 /* ------------------------------------------------- learned step size
  * 6 -> 64 -> 64 -> 64 -> 1 tanh MLP in FP32. Parameters packed as
@@ -370,6 +397,19 @@ int pq3d_mlp_adam(float* d_prm, const float* d_grad, int64_t nblk, int64_t nsamp
 int pq3d_features(int64_t ne, int nb, const int32_t* d_inst, const double* d_slipv,
                   const double* d_psi, const double* d_b, const double* d_f0, double V0,
                   uint64_t* d_acc, void* stream);
+
+/* First-attempt step size of the device controller for the instances that
+ * begin a fresh step (d_flag 0, d_nrj 0): the six features (d_acc of
+ * pq3d_features over the current state divided by d_ncell[b] cells, and the
+ * last accepted max(c1, c2) and dtnext/h from d_lstate) and the dtnext base
+ * (d_h) go into d_lstate; with d_prm (NULL: record only) the network's
+ * prediction p (inputs normalised by d_norm[0..11], output scaled by
+ * d_norm[13] and shifted by d_norm[12]) sets d_h = base 10^min(p - d_norm[14],
+ * d_norm[15]) and marks the step learned. No reference counterpart. */
+int pq3d_learned_propose(int64_t n_inst, const uint64_t* d_acc, const int64_t* d_ncell,
+                         const uint8_t* d_flag, const int32_t* d_nrj, double* d_h,
+                         double* d_lstate, const float* d_prm, const float* d_norm,
+                         void* stream);
 
 #ifdef __cplusplus
 }

@@ -30,13 +30,15 @@ class CpuHMatrix:
             r0, m, c0, n = int(B["r0"][bi]), int(B["m"][bi]), int(B["c0"][bi]), int(B["n"][bi])
             self.lowrank.append((r0, m, c0, n, np.ascontiguousarray(U), np.ascontiguousarray(V)))
 
-    def matvec(self, v1, v2):
-        """(AdotV1, AdotV2, dsigma) in cluster order, FP64."""
+    def matvec(self, v1, v2, planes=3):
+        """(AdotV1, AdotV2, dsigma) in cluster order, FP64. ``planes=2``
+        applies only the four A operators, as the reference does with a
+        fixed normal traction (src/QDsim.py:906-909); dsigma is then zero."""
         y = np.zeros((3, self.n))
         for r0, m, c0, n, M in self.dense:
             x = np.concatenate([v1[c0:c0 + n], v2[c0:c0 + n]])
-            y[:, r0:r0 + m] += (M @ x).reshape(3, m)
+            y[:planes, r0:r0 + m] += (M[:planes * m] @ x).reshape(planes, m)
         for r0, m, c0, n, U, V in self.lowrank:
             x = np.concatenate([v1[c0:c0 + n], v2[c0:c0 + n]])
-            y[:, r0:r0 + m] += (U @ (V @ x)).reshape(3, m)
+            y[:planes, r0:r0 + m] += (U[:planes * m] @ (V @ x)).reshape(planes, m)
         return y

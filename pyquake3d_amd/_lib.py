@@ -94,9 +94,11 @@ SIGNATURES = {
                                  c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "pq3d_rk45_rounds": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_int, c_double] + [c_void_p] * 19 +
-                         [c_double, c_int64, c_void_p]),
+                         [c_double, c_int64, c_void_p, c_void_p]),
     "pq3d_rk45_advance": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                  c_void_p, c_void_p] + [c_void_p] * 17 + [c_int64, c_void_p]),
+                                  c_void_p, c_void_p] + [c_void_p] * 20 +
+                          [c_int64, c_int64, c_void_p]),
+    "pq3d_controller_widths": (c_int, [_L, c_int64]),
     "pq3d_deficit": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p]),
     "pq3d_deficit_interleave": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p,
@@ -113,6 +115,8 @@ SIGNATURES = {
                               c_float, c_float, c_float, c_int, c_void_p, c_void_p]),
     "pq3d_features": (c_int, [c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                               c_double, c_void_p, c_void_p]),
+    "pq3d_learned_propose": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 _lib = None
@@ -164,6 +168,19 @@ def check_width(name: str, table) -> None:
     contiguous = table.is_contiguous() if hasattr(table, "is_contiguous") else table.flags["C_CONTIGUOUS"]
     if not contiguous:
         raise ValueError(f"{name} table is not contiguous")
+
+
+CONTROLLER_NAMES = ("run", "lstate", "sample")
+
+
+def controller_widths() -> dict:
+    """Record widths of the device controller (from the library itself)."""
+    import numpy as np
+    out = np.zeros(len(CONTROLLER_NAMES), dtype=np.int64)
+    k = load().pq3d_controller_widths(out.ctypes.data_as(_L), len(out))
+    if k != len(out):
+        raise RuntimeError("pq3d_controller_widths returned %d values" % k)
+    return dict(zip(CONTROLLER_NAMES, out.tolist()))
 
 
 def exported(lib: ctypes.CDLL) -> dict:

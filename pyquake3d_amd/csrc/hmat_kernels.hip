@@ -570,31 +570,32 @@ __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uin
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kBufAux));
 }
 
-template <typename T, int U>
+// NP row planes are read: 3 (y1, y2, y3), or 2 when the dsigma rows (the Bs
+// and Bd operators) are not wanted.
+template <typename T, int U, int NP>
 __device__ __forceinline__ void p2_load(__amdgpu_buffer_rsrc_t r, int ldb,
                                         const uint32_t (&off)[3], int64_t c, T (&a)[U][3]) {
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int soff = (int)(c + u) * ldb;
 #pragma unroll
-    for (int o = 0; o < 3; ++o) a[u][o] = buf_load<T>(r, off[o], soff);
+    for (int o = 0; o < NP; ++o) a[u][o] = buf_load<T>(r, off[o], soff);
   }
 }
 
 // x0: the weights of the group's first column; NB per column, of which this
 // wavefront uses NW (x0 already points at its first).
-template <typename T, int NB, int NW, int U>
+template <typename T, int NB, int NW, int U, int NP>
 __device__ __forceinline__ void p2_madd(const T (&a)[U][3], const double* __restrict__ x0,
-                                        double (&acc)[3 * NW]) {
+                                        double (&acc)[NP * NW]) {
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const double a0 = (double)a[u][0], a1 = (double)a[u][1], a2 = (double)a[u][2];
     const double* x = x0 + u * NB;
 #pragma unroll
-    for (int b = 0; b < NW; ++b) {
-      acc[3 * b + 0] = fma(a0, x[b], acc[3 * b + 0]);
-      acc[3 * b + 1] = fma(a1, x[b], acc[3 * b + 1]);
-      acc[3 * b + 2] = fma(a2, x[b], acc[3 * b + 2]);
+    for (int o = 0; o < NP; ++o) {
+      const double ao = (double)a[u][o];
+#pragma unroll
+      for (int b = 0; b < NW; ++b) acc[NP * b + o] = fma(ao, x[b], acc[NP * b + o]);
     }
   }
 }
@@ -614,7 +615,7 @@ __device__ __forceinline__ void p2_madd(const T (&a)[U][3], const double* __rest
 #else
 #define PQ3D_P2_ATTR
 #endif
-template <typename T, int NB>
+template <typename T, int NB, int NP>
 __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATTR k_hmv_phase2(
     const int64_t* __restrict__ tiles, const int64_t* __restrict__ tile_ptr,
     const int64_t* __restrict__ items, const T* __restrict__ pool,
@@ -624,7 +625,8 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
   constexpr int H = p2_halves<NB>();
   constexpr int NW = NB / H;  // right-hand sides per wavefront
   static_assert(NW * H == NB && 64 * kTileWaves * H <= 1024, "bad split");
-  __shared__ double red[3 * NB][64];
+  static_assert(NP == 2 || NP == 3, "two or three row planes");
+  __shared__ double red[NP * NB][64];
   const int64_t tile = xcd_order(blockIdx.x, gridDim.x);
   const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
   const int w = wave_id();
@@ -635,9 +637,9 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
   // lanes past the tile's last row repeat that row (in bounds, never stored),
   // which keeps the item loop free of divergent branches
   const int row = live ? lane : (int)mt - 1;
-  double acc[3 * NW];
+  double acc[NP * NW];
 #pragma unroll
-  for (int q = 0; q < 3 * NW; ++q) acc[q] = 0.0;
+  for (int q = 0; q < NP * NW; ++q) acc[q] = 0.0;
   for (int64_t q = tile_ptr[tile] + g; q < tile_ptr[tile + 1]; q += kTileWaves) {
     const int64_t* it = items + q * IT_W;
     const int64_t kind = it[IT_KIND], base = it[IT_BASE], ld = it[IT_LD], plane = it[IT_PLANE];
@@ -656,12 +658,12 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
       // current group is multiplied
       if (U <= cnt) {
         T a[U][3];
-        p2_load<T, U>(rs, ldb, off, c, a);
+        p2_load<T, U, NP>(rs, ldb, off, c, a);
         for (;;) {
           const bool more = c + 2 * U <= cnt;
           T an[U][3];
-          if (more) p2_load<T, U>(rs, ldb, off, c + U, an);
-          p2_madd<T, NB, NW, U>(a, xs + c * NB, acc);
+          if (more) p2_load<T, U, NP>(rs, ldb, off, c + U, an);
+          p2_madd<T, NB, NW, U, NP>(a, xs + c * NB, acc);
           c += U;
           if (!more) break;
 #pragma unroll
@@ -673,22 +675,22 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
     } else {
       for (; c + U <= cnt; c += U) {
         T a[U][3];
-        p2_load<T, U>(rs, ldb, off, c, a);
-        p2_madd<T, NB, NW, U>(a, xs + c * NB, acc);
+        p2_load<T, U, NP>(rs, ldb, off, c, a);
+        p2_madd<T, NB, NW, U, NP>(a, xs + c * NB, acc);
       }
     }
     for (; c < cnt; ++c) {
       T a[1][3];
-      p2_load<T, 1>(rs, ldb, off, c, a);
-      p2_madd<T, NB, NW, 1>(a, xs + c * NB, acc);
+      p2_load<T, 1, NP>(rs, ldb, off, c, a);
+      p2_madd<T, NB, NW, 1, NP>(a, xs + c * NB, acc);
     }
   }
   // deterministic cross-group sum: group 0 stores, groups 1.. add in order
   for (int gg = 0; gg < kTileWaves; ++gg) {
     if (g == gg) {
 #pragma unroll
-      for (int q = 0; q < 3 * NW; ++q) {
-        double* r = &red[3 * h * NW + q][lane];
+      for (int q = 0; q < NP * NW; ++q) {
+        double* r = &red[NP * h * NW + q][lane];
         *r = (gg == 0) ? acc[q] : *r + acc[q];
       }
     }
@@ -700,9 +702,9 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
 #pragma unroll
     for (int bb = 0; bb < NW; ++bb) {
       const int b = h * NW + bb;
-      y1[r * sr + b * sb] = red[3 * b + 0][lane];
-      y2[r * sr + b * sb] = red[3 * b + 1][lane];
-      y3[r * sr + b * sb] = red[3 * b + 2][lane];
+      y1[r * sr + b * sb] = red[NP * b + 0][lane];
+      y2[r * sr + b * sb] = red[NP * b + 1][lane];
+      if constexpr (NP == 3) y3[r * sr + b * sb] = red[NP * b + 2][lane];
     }
   }
 }
@@ -890,9 +892,14 @@ static void launch_matvec(const int64_t* units, int64_t nunits, const int64_t* m
                                                                       pool, v1, v2, t, part);
   if (nred > 0)
     k_hmv_phase1b<NB><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(red, nred, part, t);
-  if (ntiles > 0)
-    k_hmv_phase2<T, NB><<<(unsigned)ntiles, 64 * kTileWaves * p2_halves<NB>(), 0, s>>>(
-        tiles, tile_ptr, items, pool, v1, v2, t, y1, y2, y3, ybs);
+  if (ntiles > 0) {
+    if (y3)
+      k_hmv_phase2<T, NB, 3><<<(unsigned)ntiles, 64 * kTileWaves * p2_halves<NB>(), 0, s>>>(
+          tiles, tile_ptr, items, pool, v1, v2, t, y1, y2, y3, ybs);
+    else
+      k_hmv_phase2<T, NB, 2><<<(unsigned)ntiles, 64 * kTileWaves * p2_halves<NB>(), 0, s>>>(
+          tiles, tile_ptr, items, pool, v1, v2, t, y1, y2, nullptr, ybs);
+  }
 }
 
 }  // namespace pq3d
@@ -994,7 +1001,9 @@ int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nbl
 // right-hand sides (all FP64, cluster order, element (row, b) of v1, v2 at
 // row*nrhs + b). Outputs: the same layout with y_bstride = 0, or element
 // (row, b) at b*y_bstride + row (instance-major) with y_bstride > 0.
-// d_t: nt*nrhs doubles, d_part: npart*nrhs doubles of scratch.
+// d_t: nt*nrhs doubles, d_part: npart*nrhs doubles of scratch. d_y3 may be
+// NULL: the dsigma rows (Bs, Bd operators) are then skipped, as the reference
+// skips them when the normal traction is fixed (src/QDsim.py:906-909).
 int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
                      const int64_t* d_red, int64_t nred,
                      const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
@@ -1006,8 +1015,7 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_me
   PQ3D_REQUIRE(ntiles >= 0 && nunits >= 0 && nred >= 0, "sizes must be >= 0");
   PQ3D_REQUIRE(y_bstride >= 0, "y_bstride must be >= 0");
   PQ3D_REQUIRE(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
-  PQ3D_REQUIRE(d_tiles && d_tile_ptr && d_items && d_pool && d_v1 && d_v2 && d_y1 && d_y2 &&
-                   d_y3,
+  PQ3D_REQUIRE(d_tiles && d_tile_ptr && d_items && d_pool && d_v1 && d_v2 && d_y1 && d_y2,
                "null pointer");
   PQ3D_REQUIRE(nunits == 0 || (d_units && d_members && d_t), "null pointer");
   PQ3D_REQUIRE(nred == 0 || (d_red && d_part), "null pointer");
@@ -1043,9 +1051,10 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_me
 
 // pq3d_hmat_matvec for FP32 storage and 16 right-hand sides with the block
 // products on FP32 matrix cores: phase 1 (V^T x) as pq3d_hmat_matvec in FP64,
-// phase 2 (dense and U blocks) with weights rounded to FP32 and FP32 sums
-// per item group, added across the four groups in FP64. Same arguments;
-// dtype_bytes must be 4 and nrhs 16.
+// phase 2 (dense and U blocks) with weights rounded to FP32, FP32 sums over
+// all of an item group's dense and U columns, and the four groups' sums added
+// in FP32 (in LDS, in group order) before the FP64 store. Same arguments;
+// dtype_bytes must be 4, nrhs 16 and d_y3 non-NULL.
 int pq3d_hmat_matvec_mfma(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
                           const int64_t* d_red, int64_t nred, const int64_t* d_tiles,
                           const int64_t* d_tile_ptr, int64_t ntiles, const int64_t* d_items,

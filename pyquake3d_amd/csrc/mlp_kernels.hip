@@ -193,6 +193,73 @@ __global__ void k_feat_clear(unsigned long long* p, int64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
+// Learned-state record of the device controller (doubles per instance; the
+// same layout as in the time-step kernels): learned flag of the current
+// step, last accepted max(c1, c2), last dtnext / h, dtnext base, features.
+enum { kLsLearned = 0, kLsRatio = 1, kLsGrowth = 2, kLsBase = 3, kLsFeat = 4, kLsW = 16 };
+// Normalisation block (floats): input mean[6], input sd[6], output mean,
+// output sd, margin, largest log10 correction.
+enum { kNmXmu = 0, kNmXsd = 6, kNmYmu = 12, kNmYsd = 13, kNmMargin = 14, kNmHi = 15, kNmW = 16 };
+
+// Step-size proposal at the start of a step, one wavefront per instance.
+// Only instances beginning a fresh step (flag 0, no rejection yet) are
+// touched: the six features (decoded from the pq3d_features slots, plus the
+// last accepted error ratio and growth) and the dtnext base are stored in the
+// learned-state record; with a network, the first attempt becomes
+// base * 10^min(prediction - margin, hi) and the record is marked learned
+// (a non-finite prediction leaves the reference's dtnext and the mark clear).
+// The forward pass is k_mlp's, operation for operation.
+__global__ void __launch_bounds__(64) k_learned_propose(
+    int64_t n_inst, const unsigned long long* __restrict__ acc, const int64_t* __restrict__ ncell,
+    const uint8_t* __restrict__ flag, const int32_t* __restrict__ nrj, double* __restrict__ hcur,
+    double* __restrict__ lstate, const float* __restrict__ prm, const float* __restrict__ norm) {
+  __shared__ float x[kIn], a1[kH], a2[kH];
+  const int64_t b = blockIdx.x;
+  const int j = threadIdx.x;
+  if (b >= n_inst || flag[b] != 0 || nrj[b] != 0) return;
+  double* ls = lstate + kLsW * b;
+  const double nc = (double)ncell[b];
+  double f[kIn];
+  f[0] = __longlong_as_double((long long)acc[4 * b + 0]) - 1000.0;
+  f[1] = 1000.0 - __longlong_as_double((long long)acc[4 * b + 1]);
+  f[2] = (double)(long long)acc[4 * b + 2] / kFeatScale / nc;
+  f[3] = (double)acc[4 * b + 3] / nc;
+  f[4] = log10(fmax(ls[kLsRatio], 1e-12));
+  f[5] = log10(ls[kLsGrowth]);
+  const double base = hcur[b];
+  __syncthreads();
+  if (j == 0) {
+#pragma unroll
+    for (int q = 0; q < kIn; ++q) ls[kLsFeat + q] = f[q];
+    ls[kLsBase] = base;
+    ls[kLsLearned] = 0.0;
+  }
+  if (!prm) return;
+  if (j < kIn) x[j] = ((float)f[j] - norm[kNmXmu + j]) / norm[kNmXsd + j];
+  __syncthreads();
+  float z = prm[oB1 + j];
+#pragma unroll
+  for (int k = 0; k < kIn; ++k) z += prm[oW1 + j * kIn + k] * x[k];
+  a1[j] = tanhf(z);
+  __syncthreads();
+  z = prm[oB2 + j];
+  for (int k = 0; k < kH; ++k) z += prm[oW2 + j * kH + k] * a1[k];
+  a2[j] = tanhf(z);
+  __syncthreads();
+  z = prm[oB3 + j];
+  for (int k = 0; k < kH; ++k) z += prm[oW3 + j * kH + k] * a2[k];
+  const float h3 = tanhf(z);
+  const float y = wave_sum(prm[oW4 + j] * h3) + prm[oB4];
+  if (j == 0) {
+    const double pred = (double)y * (double)norm[kNmYsd] + (double)norm[kNmYmu];
+    if (isfinite(pred)) {
+      const double corr = fmin(pred - (double)norm[kNmMargin], (double)norm[kNmHi]);
+      hcur[b] = base * pow(10.0, corr);
+      ls[kLsLearned] = 1.0;
+    }
+  }
+}
+
 }  // namespace pq3d
 
 using namespace pq3d;
@@ -254,6 +321,26 @@ int pq3d_features(int64_t ne, int nb, const int32_t* d_inst, const double* d_sli
     k_features<<<grid, 256, 0, s>>>(ne, nb, d_inst, d_slipv, d_psi, d_b, d_f0, V0,
                                     (unsigned long long*)d_acc);
   }
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+// Step-size proposal of the device controller for n_inst instances (see
+// k_learned_propose): d_acc from pq3d_features over the current state,
+// d_ncell the cell count of every instance, d_lstate the learned-state
+// records; d_prm NULL records features and base only, otherwise d_norm holds
+// the 16 normalisation floats.
+int pq3d_learned_propose(int64_t n_inst, const uint64_t* d_acc, const int64_t* d_ncell,
+                         const uint8_t* d_flag, const int32_t* d_nrj, double* d_h,
+                         double* d_lstate, const float* d_prm, const float* d_norm,
+                         void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n_inst >= 1 && n_inst <= kFeatMaxInst, "n_inst must be in [1, 1024]");
+  PQ3D_REQUIRE(d_acc && d_ncell && d_flag && d_nrj && d_h && d_lstate, "null pointer");
+  PQ3D_REQUIRE(!d_prm || d_norm, "a network needs its normalisation block");
+  k_learned_propose<<<(unsigned)n_inst, 64, 0, as_stream(stream)>>>(
+      n_inst, (const unsigned long long*)d_acc, d_ncell, d_flag, d_nrj, d_h, d_lstate, d_prm,
+      d_norm);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }

@@ -65,6 +65,18 @@ __device__ inline void qd_rhs(double Tno, double Tt1o, double Tt2o, double state
 // Instances whose error/max slots are first reduced in LDS within a block.
 constexpr int kLdsInst = 64;
 
+// Run record of one instance (doubles): end time, armed flag, event count,
+// peak slip rate, event thresholds up and down, rejected attempts of the
+// accepted steps so far, (unused), then event start times.
+enum { kRunEnd = 0, kRunArmed = 1, kRunCount = 2, kRunPeak = 3, kRunVup = 4, kRunVdown = 5,
+       kRunRejected = 6, kRunEv0 = 8, kRunW = 64, kRunEvents = kRunW - kRunEv0 };
+// Learned-state record of one instance (doubles): learned flag of the
+// current step, last accepted max(c1, c2), last dtnext / h, dtnext base of
+// the current step, its six features.
+enum { kLsLearned = 0, kLsRatio = 1, kLsGrowth = 2, kLsBase = 3, kLsFeat = 4, kLsW = 16 };
+// Training sample row (floats): six features, target, weight.
+constexpr int kSampleW = 8;
+
 // Monotone encoding of |x| for atomic max (NaN sorts above +inf).
 __device__ inline unsigned long long abs_bits(double x) {
   return (unsigned long long)__double_as_longlong(fabs(x));
@@ -328,10 +340,16 @@ __device__ inline double py_min(double a, double b) { return b < a ? b : a; }
 // h = max(0.5 h, min(hn1, hn2)), and flag 2 with status 1 when h < 1e-15 or
 // after more than 20 rejections. hn_i = 0.9 h (tol / R_i)^0.2,
 // R1 = err_psi + 1e-10, R2 = max(err_tau1, err_tau2, err_sigma) + 1e-10.
+// With a learned-state record (lstate != null, kLsW doubles per instance):
+// an accepted step stores max(c1, c2) and dtnext / h there; a rejected
+// attempt of a step that began with a learned proposal (lstate[kLsLearned])
+// takes the PI fallback h (1/err)^0.14 prev^0.08 * 0.9 within [0.2 h, 0.9 h]
+// (prev = the last accepted max(c1, c2), at least 1e-6) unless err is not
+// finite, in which case the reference rule applies.
 __global__ void k_rk45_control(int64_t n_inst, unsigned long long* __restrict__ err,
                                double tol, double* __restrict__ hcur, uint8_t* __restrict__ flag,
                                int32_t* __restrict__ nrj, double* __restrict__ dtnext,
-                               int32_t* __restrict__ status) {
+                               int32_t* __restrict__ status, double* __restrict__ lstate) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n_inst || flag[b]) return;
   const double e0 = __longlong_as_double((long long)err[4 * b + 0]);
@@ -347,14 +365,28 @@ __global__ void k_rk45_control(int64_t n_inst, unsigned long long* __restrict__ 
   const double h = hcur[b];
   const double hn1 = h * 0.9 * pow(tol / R1, 0.2);
   const double hn2 = h * 0.9 * pow(tol / R2, 0.2);
+  double* ls = lstate ? lstate + kLsW * b : nullptr;
   if (py_max(c1, c2) < 1.0 && !isnan(c1) && !isnan(c2)) {
-    dtnext[b] = py_min(py_min(hn1, hn2), 1.5 * h);
+    const double dn = py_min(py_min(hn1, hn2), 1.5 * h);
+    dtnext[b] = dn;
     flag[b] = 1;
+    if (ls) {
+      ls[kLsRatio] = py_max(c1, c2);
+      ls[kLsGrowth] = dn / h;
+    }
     return;
   }
   const int32_t r = nrj[b] + 1;
   nrj[b] = r;
-  const double hn = py_max(0.5 * h, py_min(hn1, hn2));
+  const double e = py_max(c1, c2);
+  double hn;
+  if (ls && ls[kLsLearned] != 0.0 && isfinite(e)) {
+    const double prev = fmax(ls[kLsRatio], 1e-6);
+    const double hpi = h * 0.9 * pow(1.0 / e, 0.14) * pow(prev, 0.08);
+    hn = fmin(fmax(0.2 * h, hpi), 0.9 * h);
+  } else {
+    hn = py_max(0.5 * h, py_min(hn1, hn2));
+  }
   hcur[b] = hn;
   if (hn < 1e-15 || r > 20) {
     flag[b] = 2;
@@ -370,18 +402,60 @@ __global__ void k_clear_accepted(int64_t n_inst, const uint8_t* __restrict__ fla
 }
 
 // After the commit: accepted instances advance t += h, take h = dtnext for
-// their next step and start it with a clear rejection count.
+// their next step and start it with a clear rejection count. With a run
+// record (run != null, kRunW doubles per instance): the instance finishes
+// (flag 3, left out of later attempts and commits) once t >= its end time;
+// an event starts when the step's max slip rate rises above the up
+// threshold while armed (armed again below the down threshold), its start
+// time is kept while there is room, and the peak slip rate is tracked.
+// With a learned-state record and a sample buffer, every accepted step
+// writes one training row (the six features of the step's start, then
+// log10(h / dtnext_base)) at row nacc * n_inst + b while it fits.
 __global__ void k_rk45_advance(int64_t n_inst, uint8_t* __restrict__ flag,
                                double* __restrict__ hcur, const double* __restrict__ dtnext,
                                int32_t* __restrict__ nrj, double* __restrict__ t,
-                               int64_t* __restrict__ nacc) {
+                               int64_t* __restrict__ nacc,
+                               const unsigned long long* __restrict__ vmax,
+                               double* __restrict__ run, const double* __restrict__ lstate,
+                               float* __restrict__ samples, int64_t sample_cap) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n_inst || flag[b] != 1) return;
-  t[b] = t[b] + hcur[b];
+  const double h = hcur[b];
+  const double tn = t[b] + h;
+  t[b] = tn;
+  const int64_t k = nacc[b];
+  if (samples && lstate) {
+    const int64_t row = k * n_inst + b;
+    if (row < sample_cap) {
+      const double* ls = lstate + kLsW * b;
+      float* o = samples + kSampleW * row;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) o[q] = (float)ls[kLsFeat + q];
+      o[6] = (float)log10(h / ls[kLsBase]);
+      o[7] = 1.0f;
+    }
+  }
+  const int32_t rej = nrj[b];
   hcur[b] = dtnext[b];
   nrj[b] = 0;
-  nacc[b] = nacc[b] + 1;
-  flag[b] = 0;
+  nacc[b] = k + 1;
+  uint8_t f = 0;
+  if (run) {
+    double* r = run + kRunW * b;
+    r[kRunRejected] += (double)rej;
+    const double v = __longlong_as_double((long long)vmax[b]);
+    if (v > r[kRunPeak] || isnan(v)) r[kRunPeak] = v;
+    if (r[kRunArmed] != 0.0 && v > r[kRunVup]) {
+      const int64_t ne = (int64_t)r[kRunCount];
+      if (ne < kRunEvents) r[kRunEv0 + ne] = tn;
+      r[kRunCount] = (double)(ne + 1);
+      r[kRunArmed] = 0.0;
+    } else if (v < r[kRunVdown]) {
+      r[kRunArmed] = 1.0;
+    }
+    if (tn >= r[kRunEnd]) f = 3;
+  }
+  flag[b] = f;
 }
 
 }  // namespace pq3d
@@ -485,8 +559,10 @@ int pq3d_add_loading(int64_t n, double* d_dsig, const double* d_load, int fix_tn
 // flag 0 at its h, then the reference's accept/reject decision), with no host
 // synchronisation. The operator products must be current for the step.
 // Per-instance arrays (n_inst): d_h (attempt step size), d_flag (0 attempting,
-// 1 accepted, 2 aborted), d_nrj (rejections this step), d_dtnext, d_status
-// (1 = aborted: dt < 1e-15 or more than 20 rejections).
+// 1 accepted, 2 aborted, 3 finished), d_nrj (rejections this step), d_dtnext,
+// d_status (1 = aborted: dt < 1e-15 or more than 20 rejections). d_lstate:
+// NULL, or the learned-state records (PQ3D_LS_W doubles per instance, see
+// k_rk45_control).
 int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
                      int32_t* d_nrj, double* d_dtnext, int32_t* d_status, int rounds,
                      double tol, const double* d_Tno, const double* d_Tt1o, const double* d_Tt2o,
@@ -496,7 +572,7 @@ int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_f
                      const double* d_shear_loading, const double* d_slipv,
                      const double* d_cellpar, double* d_yTno, double* d_yTt1o, double* d_yTt2o,
                      double* d_ystate, unsigned long long* d_err, double a63, int64_t n_inst,
-                     void* stream) {
+                     double* d_lstate, void* stream) {
   clear_error();
   PQ3D_REQUIRE(n >= 0 && n_inst >= 1 && rounds >= 1, "bad sizes");
   PQ3D_REQUIRE(tol > 0.0, "tol must be > 0");
@@ -510,7 +586,7 @@ int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_f
         d_AdotV2, d_dsigmadt, d_shear_loading, d_slipv, d_cellpar, d_yTno, d_yTt1o, d_yTt2o,
         d_ystate, d_err, a63, n_inst, d_flag);
     k_rk45_control<<<blocks_for(n_inst, 64), 64, 0, s>>>(n_inst, d_err, tol, d_h, d_flag, d_nrj,
-                                                          d_dtnext, d_status);
+                                                          d_dtnext, d_status, d_lstate);
   }
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
@@ -520,7 +596,11 @@ int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_f
 // t += h, h = dtnext, rejection count 0, accepted-step count + 1, flag 0.
 // Instances still attempting keep their state, h and count, and continue in
 // the next call of pq3d_rk45_rounds (after the operator products are
-// recomputed from the same, unchanged state).
+// recomputed from the same, unchanged state). Optional (NULL: unused):
+// d_run, per-instance run records (PQ3D_RUN_W doubles: end time, event
+// detection, peak slip rate; flag 3 once the end time is reached);
+// d_lstate with d_samples, one training row (PQ3D_SAMPLE_W floats) per
+// accepted step at row nacc*n_inst + b, kept while row < sample_cap.
 int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
                       int32_t* d_nrj, const double* d_dtnext, double* d_t, int64_t* d_nacc,
                       const double* d_a, const double* d_cellpar, const double* d_yTno,
@@ -528,9 +608,11 @@ int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_
                       double* d_Tno, double* d_Tt1o, double* d_Tt2o, double* d_state,
                       double* d_slipv1, double* d_slipv2, double* d_slipv, double* d_rake,
                       double* d_slip1, double* d_slip2, unsigned long long* d_vmax,
-                      int64_t n_inst, void* stream) {
+                      double* d_run, const double* d_lstate, float* d_samples,
+                      int64_t sample_cap, int64_t n_inst, void* stream) {
   clear_error();
-  PQ3D_REQUIRE(n >= 0 && n_inst >= 1, "bad sizes");
+  PQ3D_REQUIRE(n >= 0 && n_inst >= 1 && sample_cap >= 0, "bad sizes");
+  PQ3D_REQUIRE(!d_samples || d_lstate, "training rows need the learned-state records");
   if (n == 0) return PQ3D_OK;
   hipStream_t s = as_stream(stream);
   k_clear_accepted<<<blocks_for(n_inst, 64), 64, 0, s>>>(n_inst, d_flag, d_vmax);
@@ -539,9 +621,19 @@ int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_
       d_Tt1o, d_Tt2o, d_state, d_slipv1, d_slipv2, d_slipv, d_rake, d_slip1, d_slip2, d_vmax,
       n_inst);
   k_rk45_advance<<<blocks_for(n_inst, 64), 64, 0, s>>>(n_inst, d_flag, d_h, d_dtnext, d_nrj, d_t,
-                                                        d_nacc);
+                                                        d_nacc, d_vmax, d_run, d_lstate,
+                                                        d_samples, sample_cap);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
+}
+
+// Record widths of the device controller (run record, learned-state record,
+// training row), in this order. Returns the number written.
+int pq3d_controller_widths(int64_t* out, int64_t n) {
+  const int64_t v[3] = {kRunW, kLsW, kSampleW};
+  int64_t k = 0;
+  for (; k < n && k < 3; ++k) out[k] = v[k];
+  return (int)k;
 }
 
 }  // extern "C"

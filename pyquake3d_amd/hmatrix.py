@@ -472,15 +472,19 @@ class HMatrix:
         return self._scratch[nrhs]
 
     # ----------------------------------------------------------------- apply
-    def matvec(self, v1, v2, out=None, stream=None, nrhs=1, instance_major=False):
+    def matvec(self, v1, v2, out=None, stream=None, nrhs=1, instance_major=False, planes=3):
         """(AdotV1, AdotV2, dsigma_K) = six-operator product, cluster order,
         FP64. With nrhs > 1, v1/v2 rows are interleaved by instance (element
         (cell, b) at cell*nrhs + b) and out has shape (3, n*nrhs), in the
         same layout or, with ``instance_major``, element (cell, b) at
-        b*n + cell."""
+        b*n + cell. ``planes=2`` skips the Bs and Bd operators (the
+        reference's step with a fixed normal traction, src/QDsim.py:906-909):
+        out[2] is then left as it was."""
         n = self.n
         if nrhs not in NRHS:
             raise ValueError(f"nrhs must be one of {NRHS}")
+        if planes not in (2, 3):
+            raise ValueError("planes must be 2 or 3")
         if out is None:
             out = torch.empty((3, n * nrhs), dtype=torch.float64, device=self.device)
         if not (v1.dtype == torch.float64 and v2.dtype == torch.float64):
@@ -495,14 +499,32 @@ class HMatrix:
         if self.accumulate not in ("f64", "mfma"):
             raise ValueError("accumulate must be 'f64' or 'mfma'")
         mfma = self.accumulate == "mfma" and nrhs == 16 and self.esize == 4
-        _lib.call("pq3d_hmat_matvec_mfma" if mfma else "pq3d_hmat_matvec", _lib.ptr(self.units), self.nunits, _lib.ptr(self.members),
+        y3 = _lib.ptr(out[2]) if (planes == 3 or mfma) else None
+        _lib.call("pq3d_hmat_matvec_mfma" if mfma else "pq3d_hmat_matvec", _lib.ptr(self.units),
+                  self.nunits, _lib.ptr(self.members),
                   _lib.ptr(red), int(red.shape[0]), _lib.ptr(self.tiles), _lib.ptr(self.tile_ptr),
                   self.ntiles,
                   _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
                   _lib.ptr(v2), _lib.ptr(t), _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]),
-                  _lib.ptr(out[2]), n if (instance_major and nrhs > 1) else 0,
+                  y3, n if (instance_major and nrhs > 1) else 0,
                   _lib.stream_handle(stream))
         return out
+
+    def algorithmic_bytes(self, nrhs=1, planes=3):
+        """Bytes one product must move at least: every stored operator
+        element it applies, once (the third row plane of the dense and U
+        storage only with ``planes=3``), plus two FP64 input and ``planes``
+        FP64 output vectors per right-hand side."""
+        B = self.blocks
+        m, nn = np.asarray(B["m"]), np.asarray(B["n"])
+        di = np.asarray(B["dense_idx"])
+        li = np.asarray(B["lr_idx"])
+        k = np.asarray(B["ranks"])
+        dense = int((6 * m[di] * nn[di]).sum())
+        u = int((3 * m[li] * k).sum())
+        v = int((2 * nn[li] * k).sum())
+        ops = v + (dense + u) * planes // 3
+        return ops * self.esize + (2 + planes) * 8 * self.n * nrhs
 
     def to_cluster(self, x: torch.Tensor) -> torch.Tensor:
         """Original element order -> cluster order."""
@@ -688,3 +710,9 @@ class MultiHMatrix:
 
     _bufs = HMatrix._bufs
     matvec = HMatrix.matvec
+
+    def algorithmic_bytes(self, nrhs=1, planes=3):
+        """``HMatrix.algorithmic_bytes`` summed over the instances (one
+        right-hand side each)."""
+        vec = (2 + planes) * 8
+        return sum(h.algorithmic_bytes(1, planes) - vec * h.n for h in self.parts) + vec * self.n

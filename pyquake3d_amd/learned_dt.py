@@ -95,10 +95,11 @@ class StepMLP:
             seed=0, log=None):
         feats = np.asarray(feats, dtype=np.float32)
         target = np.asarray(target, dtype=np.float32)
-        self.x_mu = feats.mean(0)
-        self.x_sd = feats.std(0) + 1e-6
-        self.y_mu = float(target.mean())
-        self.y_sd = float(target.std() + 1e-6)
+        # normalisation in FP32 values: the device proposal kernel reads them as FP32
+        self.x_mu = feats.mean(0).astype(np.float32)
+        self.x_sd = (feats.std(0) + 1e-6).astype(np.float32)
+        self.y_mu = float(np.float32(target.mean()))
+        self.y_sd = float(np.float32(target.std() + 1e-6))
         Xn = torch.from_numpy((feats - self.x_mu) / self.x_sd).to(self.device)
         Tn = torch.from_numpy((target - self.y_mu) / self.y_sd).to(self.device)
         n = Xn.shape[0]
@@ -134,8 +135,27 @@ class StepMLP:
 
     def load_state_dict(self, d):
         self.prm = torch.from_numpy(np.asarray(d["prm"], np.float32)).to(self.device)
-        self.x_mu, self.x_sd = np.asarray(d["x_mu"]), np.asarray(d["x_sd"])
-        self.y_mu, self.y_sd = float(d["y"][0]), float(d["y"][1])
+        self.x_mu = np.asarray(d["x_mu"], np.float32)
+        self.x_sd = np.asarray(d["x_sd"], np.float32)
+        self.y_mu = float(np.float32(d["y"][0]))
+        self.y_sd = float(np.float32(d["y"][1]))
+
+    def save(self, path, **meta) -> None:
+        """Parameters and normalisation as one .npz (no pickles); ``meta``
+        strings are stored alongside (e.g. how the network was trained)."""
+        d = self.state_dict()
+        np.savez(path, **d, **{"meta_" + k: np.array(str(v)) for k, v in meta.items()})
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        with np.load(path, allow_pickle=False) as z:
+            d = {k: np.array(z[k]) for k in z.files}
+        if d["prm"].shape != (int(_lib.load().pq3d_mlp_num_params()),):
+            raise ValueError(f"{path}: parameter count does not match the network")
+        m = cls(device=device)
+        m.load_state_dict(d)
+        m.meta = {k[5:]: str(v) for k, v in d.items() if k.startswith("meta_")}
+        return m
 
 
 class FeatureEnsemble(EnsembleQDsim):

@@ -55,9 +55,10 @@ class DenseOperator:
         self._x = torch.empty(2 * n, dtype=dtype, device=self.device)
         self.matvec_bytes = self.M.numel() * self.M.element_size() + 5 * 8 * n
 
-    def matvec(self, v1, v2, out=None, stream=None, nrhs=1, instance_major=False):
+    def matvec(self, v1, v2, out=None, stream=None, nrhs=1, instance_major=False, planes=3):
         """nrhs > 1: v1, v2 rows interleaved by instance (cell*nrhs + b); out
-        in the same layout, or instance-major (b*n + cell)."""
+        in the same layout, or instance-major (b*n + cell). ``planes`` is
+        accepted for the H-matrix's interface; all three rows are formed."""
         n = self.n
         if out is None:
             out = torch.empty((3, n * nrhs), dtype=torch.float64, device=self.device)
@@ -147,15 +148,19 @@ class QDsim:
         _lib.call("pq3d_deficit", self.n, _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
                   _lib.ptr(self.pl1), _lib.ptr(self.pl2), _lib.ptr(self.x[0]),
                   _lib.ptr(self.x[1]), s)
-        if self.timing:
+        # with a fixed normal traction the Bs, Bd products are not needed
+        # (src/QDsim.py:906-909)
+        planes = 2 if self.fix_Tn else 3
+        self._kcalls = getattr(self, "_kcalls", 0) + 1
+        if self.timing and self._kcalls % max(int(getattr(self, "timing_every", 1)), 1) == 0:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            self.op.matvec(self.x[0], self.x[1], out=self.K)
+            self.op.matvec(self.x[0], self.x[1], out=self.K, planes=planes)
             e1.record()
             self.matvec_events.append((e0, e1))
         else:
-            self.op.matvec(self.x[0], self.x[1], out=self.K)
+            self.op.matvec(self.x[0], self.x[1], out=self.K, planes=planes)
         _lib.call("pq3d_add_loading", self.n, _lib.ptr(self.K[2]), _lib.ptr(self.normal_loading),
                   int(self.fix_Tn), s)
 
@@ -402,15 +407,19 @@ class EnsembleQDsim:
             _lib.call("pq3d_deficit", self.ne, _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
                       _lib.ptr(self.pl1), _lib.ptr(self.pl2), _lib.ptr(self.x[0]),
                       _lib.ptr(self.x[1]), s)
-        if self.timing:
+        planes = 2 if self.fix_Tn else 3
+        self._kcalls = getattr(self, "_kcalls", 0) + 1
+        if self.timing and self._kcalls % max(int(getattr(self, "timing_every", 1)), 1) == 0:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs, instance_major=True)
+            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs, instance_major=True,
+                           planes=planes)
             e1.record()
             self.matvec_events.append((e0, e1))
         else:
-            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs, instance_major=True)
+            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs, instance_major=True,
+                           planes=planes)
         _lib.call("pq3d_add_loading", self.ne, _lib.ptr(self.K[2]),
                   _lib.ptr(self.normal_loading), int(self.fix_Tn), s)
 
@@ -519,10 +528,30 @@ class DeviceEnsemble(EnsembleQDsim):
     (identical values), so the accepted step is the one the reference loop
     would take. ``sync()`` copies time, next dt and step counts to the host
     and raises StepAborted for an instance whose dt fell below 1e-15 s or
-    that was rejected more than 20 times."""
+    that was rejected more than 20 times.
+
+    Run control on the device (no host round trip per step):
+      ``t_end``         simulated time (s) at which every instance stops
+                        (it finishes once its time reaches t_end; the
+                        others carry on);
+      ``vup``/``vdown`` event detection: an event starts after the first
+                        accepted step whose max slip rate exceeds vup while
+                        armed, re-armed below vdown; ``events()`` returns the
+                        start times, ``peak_slipv()`` the largest slip rate;
+      ``mlp``           a trained ``learned_dt.StepMLP``: each fresh step's
+                        first attempt is the network's proposal (the
+                        estimator still accepts or rejects every attempt, a
+                        rejection falls back to a PI controller);
+      ``record``        log one training row per accepted step (features of
+                        the step's start, log10 of the accepted h over the
+                        reference controller's dtnext) into a device buffer
+                        of ``record`` rows (``samples()``).
+    """
 
     def __init__(self, problems, op, reltol: float = 1e-4, device="cuda", tableau: str = "ref",
-                 rounds: int = 8, dt0: float = 1e-3):
+                 rounds: int = 8, dt0: float = 1e-3, t_end: float = math.inf,
+                 vup: float = 1e-2, vdown: float = 1e-4, mlp=None, record: int = 0,
+                 margin: float = 0.0, hi: float = 0.0):
         super().__init__(problems, op, reltol=reltol, device=device, tableau=tableau)
         dev = self.device
         nb = self.nb
@@ -536,28 +565,68 @@ class DeviceEnsemble(EnsembleQDsim):
         self.nacc = torch.zeros(nb, dtype=torch.int64, device=dev)
         self.iterations = 0
         self.naccepted = np.zeros(nb, dtype=np.int64)
+        W = _lib.controller_widths()
+        self._W = W
+        run = np.zeros((nb, W["run"]))
+        run[:, 0] = float(t_end)
+        run[:, 1] = 0.0                  # armed once max V falls below vdown
+        run[:, 4], run[:, 5] = float(vup), float(vdown)
+        self.run = torch.from_numpy(run).to(dev)
+        self.mlp = mlp
+        self.record_cap = int(record)
+        self.learned = mlp is not None or self.record_cap > 0
+        if self.learned:
+            ls = np.zeros((nb, W["lstate"]))
+            ls[:, 1] = 0.5               # last accepted max(c1, c2) before the first step
+            ls[:, 2] = 1.0               # last dtnext / h before the first step
+            self.lstate = torch.from_numpy(ls).to(dev)
+            self.facc = torch.zeros(4 * nb, dtype=torch.int64, device=dev)
+            self.ncell = torch.from_numpy(np.diff(self.off).astype(np.int64)).to(dev)
+            self.V0 = float(problems[0].V0)
+            if len({float(p.V0) for p in problems}) != 1:
+                raise ValueError("the learned proposal needs one V0 for all instances")
+        else:
+            self.lstate = None
+        self.samples_buf = (torch.zeros((self.record_cap, W["sample"]), dtype=torch.float32,
+                                        device=dev) if self.record_cap > 0 else None)
+        if mlp is not None:
+            norm = np.concatenate([mlp.x_mu, mlp.x_sd, [mlp.y_mu, mlp.y_sd, margin, hi]])
+            self.norm = torch.from_numpy(norm.astype(np.float32)).to(dev)
+        else:
+            self.norm = None
 
     _graph = None
+    _graph_key = None
+
+    def _graph_state(self):
+        return (getattr(self.op, "accumulate", None), self.mlp is not None, self.rounds)
 
     def capture(self):
         """Record one step (operator product, attempt rounds, commit/advance:
         device launches on fixed buffers, no host synchronisation) as a HIP
         graph; later ``step()`` calls replay it. Recording runs nothing, so
-        the state is unchanged. Call after at least one eager step (the
-        matvec scratch buffers then exist), with ``timing`` off."""
+        the state is unchanged. The operator's scratch buffers are created
+        first; a later change of the operator's accumulation mode (or of
+        ``rounds``) drops the graph."""
         if self.timing:
             raise ValueError("capture needs timing off (events are not recorded into the graph)")
+        if hasattr(self.op, "_bufs"):
+            self.op._bufs(self.nrhs)
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self._enqueue_step()
         torch.cuda.synchronize(self.device)
         self._graph = g
+        self._graph_key = self._graph_state()
 
     def release_graph(self):
         self._graph = None
+        self._graph_key = None
 
     def step(self):
+        if self._graph is not None and self._graph_key != self._graph_state():
+            self.release_graph()
         if self._graph is not None:
             self._graph.replay()
         else:
@@ -566,6 +635,14 @@ class DeviceEnsemble(EnsembleQDsim):
 
     def _enqueue_step(self):
         s = _lib.stream_handle()
+        if self.learned:
+            _lib.call("pq3d_features", self.ne, self.nb, _lib.ptr(self.inst),
+                      _lib.ptr(self.slipv), _lib.ptr(self.state), _lib.ptr(self.b),
+                      _lib.ptr(self.f0), self.V0, _lib.ptr(self.facc), s)
+            _lib.call("pq3d_learned_propose", self.nb, _lib.ptr(self.facc), _lib.ptr(self.ncell),
+                      _lib.ptr(self.flag), _lib.ptr(self.nrj), _lib.ptr(self.hcur),
+                      _lib.ptr(self.lstate), _lib.ptr(self.mlp.prm) if self.mlp else None,
+                      _lib.ptr(self.norm), s)
         self.compute_K()
         _lib.call("pq3d_rk45_rounds", self.ne, _lib.ptr(self.inst), _lib.ptr(self.hcur),
                   _lib.ptr(self.flag), _lib.ptr(self.nrj), _lib.ptr(self.dtnext),
@@ -575,7 +652,8 @@ class DeviceEnsemble(EnsembleQDsim):
                   _lib.ptr(self.K[0]), _lib.ptr(self.K[1]), _lib.ptr(self.K[2]),
                   _lib.ptr(self.shear_loading), _lib.ptr(self.slipv), _lib.ptr(self.cellpar),
                   _lib.ptr(self.y[0]), _lib.ptr(self.y[1]), _lib.ptr(self.y[2]),
-                  _lib.ptr(self.y[3]), _lib.ptr(self.err), self.a63, self.nb, s)
+                  _lib.ptr(self.y[3]), _lib.ptr(self.err), self.a63, self.nb,
+                  _lib.ptr(self.lstate), s)
         _lib.call("pq3d_rk45_advance", self.ne, _lib.ptr(self.inst), _lib.ptr(self.hcur),
                   _lib.ptr(self.flag), _lib.ptr(self.nrj), _lib.ptr(self.dtnext),
                   _lib.ptr(self.tdev), _lib.ptr(self.nacc), _lib.ptr(self.a),
@@ -584,20 +662,26 @@ class DeviceEnsemble(EnsembleQDsim):
                   _lib.ptr(self.Tt1o), _lib.ptr(self.Tt2o), _lib.ptr(self.state),
                   _lib.ptr(self.slipv1), _lib.ptr(self.slipv2), _lib.ptr(self.slipv),
                   _lib.ptr(self.rake), _lib.ptr(self.slip1), _lib.ptr(self.slip2),
-                  _lib.ptr(self.vmax), self.nb, s)
+                  _lib.ptr(self.vmax), _lib.ptr(self.run), _lib.ptr(self.lstate),
+                  _lib.ptr(self.samples_buf), self.record_cap, self.nb, s)
 
-    DEVICE_FIELDS = ("hcur", "dtnext", "flag", "nrj", "status", "tdev", "nacc")
+    DEVICE_FIELDS = ("hcur", "dtnext", "flag", "nrj", "status", "tdev", "nacc", "run")
 
     def save_state(self) -> dict:
         self.sync()
         d = super().save_state()
         d.update({k: getattr(self, k).clone() for k in self.DEVICE_FIELDS})
+        if self.lstate is not None:
+            d["lstate"] = self.lstate.clone()
         return d
 
     def load_state(self, d: dict) -> None:
         super().load_state(d)
         for k in self.DEVICE_FIELDS:
-            getattr(self, k).copy_(d[k])
+            if k in d:
+                getattr(self, k).copy_(d[k])
+        if self.lstate is not None and "lstate" in d:
+            self.lstate.copy_(d["lstate"])
         self.sync()
 
     def sync(self):
@@ -612,3 +696,42 @@ class DeviceEnsemble(EnsembleQDsim):
             b = int(np.argmax(st != 0))
             raise StepAborted(f"instance {b}: dt is too small or more than 20 rejections")
         return self.time
+
+    def finished(self) -> np.ndarray:
+        """Which instances have reached t_end (flag 3)."""
+        return self.flag.cpu().numpy() == 3
+
+    def aborted(self) -> np.ndarray:
+        return self.status.cpu().numpy() != 0
+
+    RUN_EV0 = 8
+
+    def events(self) -> list:
+        """Event start times (s) of every instance (at most the record's
+        capacity; ``event_counts()`` has the full counts)."""
+        r = self.run.cpu().numpy()
+        cnt = r[:, 2].astype(np.int64)
+        e0 = self.RUN_EV0
+        cap = r.shape[1] - e0
+        return [r[b, e0:e0 + min(int(cnt[b]), cap)].copy() for b in range(self.nb)]
+
+    def rejected(self) -> np.ndarray:
+        """Rejected attempts of every instance's accepted steps so far."""
+        return self.run[:, 6].cpu().numpy().astype(np.int64)
+
+    def event_counts(self) -> np.ndarray:
+        return self.run[:, 2].cpu().numpy().astype(np.int64)
+
+    def peak_slipv(self) -> np.ndarray:
+        return self.run[:, 3].cpu().numpy().copy()
+
+    def samples(self):
+        """(features, targets) of the recorded training rows so far."""
+        if self.samples_buf is None:
+            raise ValueError("construct with record > 0 to log training rows")
+        acc = self.nacc.cpu().numpy()
+        rows = np.arange(self.record_cap)
+        k, b = rows // self.nb, rows % self.nb
+        valid = k < acc[b]
+        S = self.samples_buf.cpu().numpy()[valid]
+        return S[:, :6].astype(np.float64), S[:, 6].astype(np.float64)

@@ -18,7 +18,8 @@ class IdentityOp:
         self.perm = np.arange(n)
         self.K = K
 
-    def matvec(self, v1, v2, out=None, nrhs=1, instance_major=False):
+    def matvec(self, v1, v2, out=None, nrhs=1, instance_major=False, planes=3):
+        assert planes in (2, 3)
         K = self.K
         n = K["A1s"].shape[0]
         v1, v2 = v1.numpy().reshape(n, nrhs), v2.numpy().reshape(n, nrhs)
@@ -112,7 +113,7 @@ class DeviceStepChecker(StepChecker):
 
     def pq3d_rk45_rounds(self, n, inst, h, flag, nrj, dtn, status, rounds, tol, Tno, Tt1o,
                          Tt2o, st, a, b, dc, f0, A1, A2, ds, sl, V, cp, y0, y1, y2, y3, err, a63,
-                         ninst, stream):
+                         ninst, lstate, stream):
         self._cells(n, Tno, Tt1o, Tt2o, st, a, b, dc, f0, A1, A2, ds, sl, V, y0, y1, y2, y3)
         ii = self.T(inst)
         assert ii.numel() >= n and int(ii.max()) < ninst and int(ii.min()) >= 0
@@ -143,12 +144,21 @@ class DeviceStepChecker(StepChecker):
                     S[g] = 1
                 self.attempts += 1
 
+    vmax_script = None    # per-instance max slip rate of the committed step
+
     def pq3d_rk45_advance(self, n, inst, h, flag, nrj, dtn, t, nacc, a, cp, y0, y1, y2, y3, Tno,
-                          Tt1o, Tt2o, st, s1, s2, V, rake, sl1, sl2, vmax, ninst, stream):
+                          Tt1o, Tt2o, st, s1, s2, V, rake, sl1, sl2, vmax, run, lstate, samples,
+                          sample_cap, ninst, stream):
         self._cells(n, a, y0, y1, y2, y3, Tno, Tt1o, Tt2o, st, s1, s2, V, rake, sl1, sl2)
         for q in (h, flag, nrj, dtn, t, nacc, vmax):
             assert self.T(q).numel() >= ninst
+        assert sample_cap >= 0 and (samples is None or lstate is not None)
         H, F, R, D, Tt, N = (self.T(q) for q in (h, flag, nrj, dtn, t, nacc))
+        RUN = None
+        if run is not None:
+            RUN = self.T(run).view(ninst, -1)
+            assert RUN.shape[1] == _lib.controller_widths()["run"]
+        R_before = R.clone()
         for g in range(ninst):
             if F[g] == 1:
                 Tt[g] += H[g]
@@ -156,6 +166,22 @@ class DeviceStepChecker(StepChecker):
                 R[g] = 0
                 N[g] += 1
                 F[g] = 0
+                if RUN is not None:
+                    # run record as k_rk45_advance keeps it
+                    v = float(self.vmax_script[g]) if self.vmax_script is not None else 0.0
+                    r = RUN[g]
+                    r[6] += float(R_before[g])
+                    r[3] = max(float(r[3]), v)
+                    if r[1] != 0 and v > r[4]:
+                        c = int(r[2])
+                        if c < RUN.shape[1] - 8:
+                            r[8 + c] = Tt[g]
+                        r[2] = c + 1
+                        r[1] = 0
+                    elif v < r[5]:
+                        r[1] = 1
+                    if Tt[g] >= r[0]:
+                        F[g] = 3
 
 
 def test_stepper_argument_lists_and_controller(monkeypatch, oracle):
@@ -246,3 +272,38 @@ def test_device_ensemble_argument_lists(monkeypatch, oracle):
         for _ in range(20):
             ens.step()
         ens.sync()
+
+
+def test_device_ensemble_end_time_and_events(monkeypatch, oracle):
+    """Run control of the device controller, restated on the CPU: an
+    instance stops at t_end while the other carries on, and event start
+    times follow the up/down thresholds of the committed max slip rate."""
+    from oracle.qdsim_oracle import dense_operators
+    from pyquake3d_amd.mesh import get_eleVec
+    from pyquake3d_amd.qdsim import DeviceEnsemble
+    p = planar_c1(nx=6, nz=3)
+    ev, xg = get_eleVec(p.mesh.nodelst, p.mesh.elelst)
+    K = dense_operators(oracle, xg, ev, p.mesh.triangles(), p.mu, p.lambda_)
+    ora = OracleSim(K, p.a, p.b, p.dc, p.f0, p.V0, p.mu, p.Cs, p.Vpl, p.rake, p.initial_state())
+    chk = DeviceStepChecker(ora)
+    chk.attempts = 0
+    monkeypatch.setattr(_lib, "call", chk.call)
+    monkeypatch.setattr(_lib, "ptr", chk.ptr)
+    monkeypatch.setattr(_lib, "stream_handle", lambda s=None: None)
+    ok = [1e-12, 1e-12, 1e-12, 1e-12]          # every attempt accepted, dt grows 1.5x
+    chk._errs = [[ok], [ok]]
+    ens = DeviceEnsemble([p, p], IdentityOp(p.n, K), device="cpu", rounds=1, dt0=1.0,
+                         t_end=4.0, vup=1e-2, vdown=1e-4)
+    seq = [1e-6, 5e-2, 1e-1, 1e-5, 2e-2, 1e-6]   # instance 1's max slip rate per step
+    for v in seq:
+        chk.vmax_script = [1e-9, v]
+        ens.step()
+    ens.sync()
+    # instance 0: t = 1, 2.5, 4.75 -> finished after three steps
+    assert ens.finished().tolist() == [True, True]
+    assert list(ens.naccepted) == [3, 3]
+    # instance 1 (same steps, finished at 4.75): up-crossing at step 2 only
+    ev1 = ens.events()[1]
+    assert ev1.tolist() == [2.5]
+    assert ens.peak_slipv()[1] == pytest.approx(1e-1)
+    assert ens.event_counts().tolist() == [0, 1]
