@@ -66,7 +66,7 @@ YEAR = 3.15576e7
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of the matvec workload,
 # FETCH scaled by 2.0 and WRITE by 1.0: the factors the same counters read
 # for a 2 GiB device-to-device copy (profiles/hmatvec_traffic_N100352.md).
-MEASURED_TRAFFIC = {("f32", 1): 6_276_266_539, ("f32", 16): 7_350_784_928}
+MEASURED_TRAFFIC = {("f32", 1, 3): 6_276_266_539, ("f32", 16, 3): 7_350_784_928}
 
 # FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
 # (unit strike and dip slip, both projected), counted operator by operator
@@ -115,6 +115,14 @@ C3_T_END_YEARS = 380.0       # every draw has had >= 4 events (3 whole cycles) b
 EVENT_VUP, EVENT_VDOWN = 1e-2, 1e-4   # seismic onset and re-arm thresholds (m/s)
 EVENT_MERGE_YEARS = 1.0      # onsets closer than this belong to one rupture sequence
 STEP_MLP = REPO / "pyquake3d_amd" / "data" / "step_mlp_c2.npz"
+# first attempt = dtnext 10^min(prediction - margin, 0): a negative margin
+# leans the proposal towards the reference controller's dtnext. On the
+# device a rejected attempt costs one cell-local attempt round while every
+# accepted step costs an operator product, so leaning towards larger steps
+# pays: over [0, 380] yr of the 16 draws (plain FP32 matrix-core sums)
+# margin 0 took 57,300 iterations, -0.03 56,100, -0.06 54,500; the reference
+# controller 57,300.
+LEARNED_MARGIN = -0.06
 
 
 def c3_draws(nb, nx=448, nz=112, dc_scale=C3_DC_SCALE):
@@ -142,7 +150,7 @@ def merge_events(ev_s):
 
 
 def run_cycles(problems, H, name, t_end_years, warmup, rounds=8, mlp=None, timing_every=0,
-               check_every=100, max_wall=900.0):
+               check_every=100, max_wall=900.0, margin=0.0, hi=0.0, dt0=1e-3):
     """``problems`` (friction draws on the operator's mesh) stepped by the
     device controller from their initial state until every draw reaches
     ``t_end_years`` (a draw that gets there first stops taking steps; the
@@ -155,7 +163,8 @@ def run_cycles(problems, H, name, t_end_years, warmup, rounds=8, mlp=None, timin
     from pyquake3d_amd.qdsim import DeviceEnsemble, StepAborted
 
     ens = DeviceEnsemble(problems, H, device=H.device, rounds=rounds, t_end=t_end_years * YEAR,
-                         vup=EVENT_VUP, vdown=EVENT_VDOWN, mlp=mlp)
+                         vup=EVENT_VUP, vdown=EVENT_VDOWN, mlp=mlp, margin=margin, hi=hi,
+                         dt0=dt0)
     s0 = ens.save_state()
     for _ in range(warmup):
         ens.step()
@@ -298,16 +307,26 @@ def bench_hstep(args):
     H64, b64 = build_hmatrix(problem, torch.float64)
     plain, _ = run_cycles([problem], H64, "plain_fp64", t_end, args.warmup, timing_every=20)
     plain["build_s"] = b64
-    ref64 = None
+    ref64 = floor = None
     if not args.no_accuracy:
         # FP64 reference of every draw: the batched FP64 product is bitwise the
         # single-draw product, so draw b here is draw b's FP64 solo run
         ref64, _ = run_cycles(draws, H64, f"fp64_nb{args.nb}", t_end, args.warmup)
+        if args.noise_floor:
+            # the same FP64 runs started from dt = 2e-3 s instead of 1e-3 s:
+            # how far the reference method's own event onsets move when only
+            # its step sequence changes
+            alt, _ = run_cycles(draws, H64, f"fp64_nb{args.nb}_dt0x2", t_end, args.warmup,
+                                dt0=2e-3)
+            floor = event_errors(alt, ref64)
     del H64
     torch.cuda.empty_cache()
     H32, b32 = build_hmatrix(problem, torch.float32)
+    H32.accumulate = args.accumulate
     fast, _ = run_cycles(draws, H32, f"fp32_nb{args.nb}" + ("_learned" if mlp else ""), t_end,
-                         args.warmup, mlp=mlp, timing_every=20)
+                         args.warmup, mlp=mlp, timing_every=20, margin=LEARNED_MARGIN)
+    fast["accumulate"] = args.accumulate
+    fast["learned_margin"] = LEARNED_MARGIN if mlp else None
     fast["build_s"] = b32
     achieved = fast["matvec_bytes"] / (fast["matvec_ms"] * 1e-3) / 1e9
     acc = event_errors(fast, ref64) if ref64 else None
@@ -327,7 +346,12 @@ def bench_hstep(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 operator storage, f64 state and accumulation; f32 step-size network",
+        "dtype": ("f32 operator storage and f32 matrix-core block products (compensated f32 "
+                  "sums across items, f64 across item groups), f64 state and RK45; f32 "
+                  "step-size network" if args.accumulate == "mfma_comp" else
+                  "f32 operator storage and f32 matrix-core block products, f64 state and RK45; "
+                  "f32 step-size network" if args.accumulate == "mfma" else
+                  "f32 operator storage, f64 state and accumulation; f32 step-size network"),
         "data": "synthetic undulating fault mesh and seeded friction fields",
         "config": {"workload": f"H-matrix RK45 time stepping over whole seismic cycles, "
                                f"undulating fault 100x25 km, N={problem.n}, leaf 64, eta 2, ACA "
@@ -349,11 +373,15 @@ def bench_hstep(args):
         "fp32_batched": fast,
         "fp64_batched_reference": ref64,
         "event_accuracy_vs_fp64": acc,
+        "fp64_noise_floor_dt0x2": floor,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / MI355X_HBM_GBS,
                      "traffic": MEASURED_TRAFFIC.get(("f32", args.nb, 2)),
-                     "kernel": f"H-matvec, {args.nb} right-hand sides, two row planes "
-                               f"(k_hmv_phase1 + k_hmv_phase1b + k_hmv_phase2)",
+                     "kernel": (f"H-matvec, {args.nb} right-hand sides, two row planes "
+                                f"(k_inputs_f32 + k_hmv_phase1_mf16 + k_hmv_phase1b + "
+                                f"k_hmv_phase2_mf16)" if args.accumulate.startswith("mfma")
+                                else f"H-matvec, {args.nb} right-hand sides, two row planes "
+                                     f"(k_hmv_phase1 + k_hmv_phase1b + k_hmv_phase2)"),
                      "kernel_ms": fast["matvec_ms"], "algorithmic_bytes": fast["matvec_bytes"]},
     }
     if not args.no_cpu_baseline:
@@ -396,26 +424,40 @@ def bench_matvec(args):
     v1 = torch.randn(n * nb, dtype=torch.float64, device=dev, generator=gen)
     v2 = torch.randn(n * nb, dtype=torch.float64, device=dev, generator=gen)
     out = torch.empty((3, n * nb), dtype=torch.float64, device=dev)
+    planes = args.planes
     for _ in range(args.warmup):
-        H.matvec(v1, v2, out=out, nrhs=nb)
+        H.matvec(v1, v2, out=out, nrhs=nb, planes=planes)
     torch.cuda.synchronize()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(args.steps)]
     w0 = time.perf_counter()
     for e0, e1 in evs:
         e0.record()
-        H.matvec(v1, v2, out=out, nrhs=nb)
+        H.matvec(v1, v2, out=out, nrhs=nb, planes=planes)
         e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / len(evs)
+    mf_err = None
+    if H.accumulate.startswith("mfma") and nb == 16 and args.dtype == "f32":
+        # the matrix-core product against the FP64-accumulated product of the
+        # same FP32 storage: max |difference| / max |FP64-accumulated value|
+        # per output row plane and right-hand side, worst case reported
+        ref = torch.empty_like(out)
+        H.accumulate = "f64"
+        H.matvec(v1, v2, out=ref, nrhs=nb, planes=planes)
+        H.accumulate = args.accumulate
+        H.matvec(v1, v2, out=out, nrhs=nb, planes=planes)
+        d = (out[:planes] - ref[:planes]).abs().view(planes, n, nb).amax(dim=1)
+        r = ref[:planes].abs().view(planes, n, nb).amax(dim=1)
+        mf_err = float((d / r).max())
     B = H.blocks
     lr, k = B["lr_idx"], B["ranks"]
     v_bytes = int((2 * B["n"][lr] * k).sum()) * H.esize
-    u_bytes = int((3 * B["m"][lr] * k).sum()) * H.esize
-    d_bytes = int(H.stats.dense_entries) * H.esize
-    vec_bytes = 5 * 8 * n * nb
-    algo = H.matvec_bytes + 5 * 8 * n * (nb - 1)
+    u_bytes = int((planes * B["m"][lr] * k).sum()) * H.esize
+    d_bytes = int(H.stats.dense_entries) * H.esize * planes // 3
+    vec_bytes = (2 + planes) * 8 * n * nb
+    algo = H.algorithmic_bytes(nb, planes)
     achieved = algo / (ms * 1e-3) / 1e9
     return {
         "metric": "H-matvec effective HBM bandwidth",
@@ -424,19 +466,20 @@ def bench_matvec(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": (f"{args.dtype} operator storage, f32 matrix-core products for the dense and U "
                   f"blocks, f64 V products and vectors"
-                  if (H.accumulate == "mfma" and nb == 16 and args.dtype == "f32") else
+                  if (H.accumulate.startswith("mfma") and nb == 16 and args.dtype == "f32") else
                   f"{args.dtype} operator storage, f64 vectors and accumulation"),
         "data": "synthetic undulating fault mesh, seeded random slip-rate vectors",
         "config": {"workload": f"H-matvec, undulating fault 100x25 km, N={n}, leaf 64, eta 2, "
-                               f"ACA eps 1e-4, {nb} right-hand sides"},
+                               f"ACA eps 1e-4, {nb} right-hand sides, {planes} row planes"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((args.dtype, nb)),
+                     "traffic": MEASURED_TRAFFIC.get((args.dtype, nb, planes)),
                      "kernel_ms": ms, "algorithmic_bytes": algo},
         "bytes": {"phase1_lowrank_V": v_bytes, "phase2_lowrank_U": u_bytes,
                   "phase2_dense": d_bytes, "vectors": vec_bytes, "total": algo},
         "copy_2GiB_GBs": copy_gbs,
         "build_s": build_s,
+        "mfma_rel_err_vs_f64_accumulation": mf_err,
     }
 
 
@@ -971,9 +1014,11 @@ def main():
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"],
                     help="operator storage of the matvec workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--planes", type=int, default=3, choices=[2, 3],
+                    help="matvec: row planes (2: the Bs, Bd operators skipped)")
     ap.add_argument("--no-graph", action="store_true",
                     help="hstep: time the headline ensemble with eager launches only")
-    ap.add_argument("--accumulate", default="f64", choices=["f64", "mfma"],
+    ap.add_argument("--accumulate", default=None, choices=["f64", "mfma", "mfma_comp"],
                     help="16 right-hand sides over FP32 storage: 'mfma' runs the dense and U "
                          "products on FP32 matrix cores (hstep headline and matvec)")
     ap.add_argument("--halfspace", action="store_true",
@@ -1001,6 +1046,9 @@ def main():
                     help="hstep: the batched variant with the reference controller")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="hstep: skip the FP64 reference run of every draw")
+    ap.add_argument("--noise-floor", action="store_true",
+                    help="hstep: also rerun the FP64 draws from dt = 2e-3 s (event-onset "
+                         "spread of the reference method itself)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the "
@@ -1008,6 +1056,10 @@ def main():
         sys.exit(2)
     if args.workload != "learned":
         args.nb = 16 if args.nb is None else args.nb
+    if args.accumulate is None:
+        # the headline's batched products run on the matrix cores; the other
+        # workloads default to FP64 sums
+        args.accumulate = "mfma" if args.workload == "hstep" else "f64"
     if args.workload == "hstep":
         # the timed region is the whole simulated interval; --steps is ignored
         args.warmup = 20 if args.warmup is None else args.warmup

@@ -239,18 +239,23 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_me
                      double* d_y1, double* d_y2, double* d_y3, int64_t y_bstride,
                      void* stream);
 
-/* pq3d_hmat_matvec with the dense and U-block products on FP32 matrix cores
- * (same replaced reference path, src/Hmatrix.py:1445-1538): FP32 storage
- * (dtype_bytes 4) and nrhs 16 only. Phase 1 (V^T x) is the FP64 one of
- * pq3d_hmat_matvec; phase 2 rounds the weights to FP32 and sums in FP32
- * within each of a tile's four item groups, the groups added in FP32 in
- * a fixed order; outputs FP64. */
+/* pq3d_hmat_matvec with every block product on FP32 matrix cores (same
+ * replaced reference path, src/Hmatrix.py:1445-1538): FP32 storage
+ * (dtype_bytes 4), nrhs 16 and tiles of at least 4 rows only. The inputs are
+ * rounded to FP32 into d_x32 (2 n 16 floats of scratch, n = rows); d_t and
+ * d_part serve as FP32 arrays (nt*16 and npart*16 floats). Phase 1 sums each
+ * phase-1 unit's columns (at most 512) in FP32, split blocks' partials in
+ * FP64; phase 2 sums each item in FP32 on the matrix cores and adds the
+ * items of an item group in compensated FP32 (hi + lo pairs, TwoSum) when
+ * compensated != 0, else in plain FP32; the four item groups in FP64;
+ * outputs FP64. d_y3 may be NULL (two planes). */
 int pq3d_hmat_matvec_mfma(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
                           const int64_t* d_red, int64_t nred, const int64_t* d_tiles,
                           const int64_t* d_tile_ptr, int64_t ntiles, const int64_t* d_items,
                           const void* d_pool, int dtype_bytes, int nrhs, const double* d_v1,
                           const double* d_v2, double* d_t, double* d_part, double* d_y1,
-                          double* d_y2, double* d_y3, int64_t y_bstride, void* stream);
+                          double* d_y2, double* d_y3, int64_t y_bstride, float* d_x32,
+                          int64_t n, int compensated, void* stream);
 
 /* ---------------------------------------------------------- time step
  * Cells may belong to n_inst independent instances (d_inst[i], or NULL for

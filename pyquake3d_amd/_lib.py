@@ -9,7 +9,12 @@ import ctypes
 from ctypes import POINTER, c_bool, c_double, c_float, c_int, c_int64, c_size_t, c_void_p
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "libpq3d.so"
+import os
+
+# PQ3D_LIB names another build of the same library (e.g. one compiled with
+# different tuning macros); the default is the in-tree build
+LIB_PATH = Path(os.environ.get("PQ3D_LIB") or
+                Path(__file__).resolve().parent / "lib" / "libpq3d.so")
 
 _D = POINTER(c_double)
 _F = POINTER(c_float)
@@ -81,7 +86,8 @@ SIGNATURES = {
     "pq3d_hmat_matvec_mfma": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64,
                                       c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int,
                                       c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                      c_void_p, c_void_p, c_int64, c_void_p]),
+                                      c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int,
+                                      c_void_p]),
     # time step
     "pq3d_rk45_attempt": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

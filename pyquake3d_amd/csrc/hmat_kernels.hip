@@ -512,10 +512,10 @@ __global__ void __launch_bounds__(256) k_hmv_phase1(const int64_t* __restrict__ 
 // unit order (deterministic). Element e = l*NB + b of unit q sits at
 // part[(p0 + q*k)*NB + e]; one wavefront per rank chunk [l0, l0 + lc) of a
 // block (about 64 elements); loads are issued eight units at a time.
-template <int NB>
+template <int NB, typename E = double>
 __global__ void __launch_bounds__(256) k_hmv_phase1b(const int64_t* __restrict__ red,
-                                                     int64_t nred, const double* __restrict__ part,
-                                                     double* __restrict__ t) {
+                                                     int64_t nred, const E* __restrict__ part,
+                                                     E* __restrict__ t) {
   const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + wave_id();
   const int lane = threadIdx.x & 63;
   if (r >= nred) return;
@@ -524,18 +524,18 @@ __global__ void __launch_bounds__(256) k_hmv_phase1b(const int64_t* __restrict__
   const int64_t l0 = it[RD_L0], lc = it[RD_LC];
   const int64_t qs = k * NB;
   for (int64_t e = l0 * NB + lane; e < (l0 + lc) * NB; e += 64) {
-    const double* pp = part + p0 * NB + e;
+    const E* pp = part + p0 * NB + e;
     double s = 0.0;
     int64_t q = 0;
     for (; q + 8 <= nu; q += 8) {
-      double v[8];
+      E v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) v[u] = pp[(q + u) * qs];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) s += v[u];
+      for (int u = 0; u < 8; ++u) s += (double)v[u];
     }
-    for (; q < nu; ++q) s += pp[q * qs];
-    t[to * NB + e] = s;
+    for (; q < nu; ++q) s += (double)pp[q * qs];
+    t[to * NB + e] = (E)s;
   }
 }
 
@@ -709,174 +709,287 @@ __global__ void __launch_bounds__(64 * kTileWaves * p2_halves<NB>()) PQ3D_P2_ATT
   }
 }
 
-// Phase 2 on FP32 matrix cores, 16 right-hand sides, FP32 operator storage:
-// the same tiles, items and item groups as k_hmv_phase2<float, 16>, but each
-// item is a (rows x columns) x (columns x 16) product on
-// v_mfma_f32_16x16x4f32 with the weights rounded to FP32 and FP32
-// accumulation inside an item group (over all of its items); the four
-// groups' sums are added in LDS in group order (FP32) and written as FP64.
-// A tile's 64 rows are four 16-row blocks per output plane, held by one
-// wavefront per item group (kMfRb = 4: 12 accumulator tiles of 16 x 16) or
-// split over 2 or 4 wavefronts.
-//   A (16 x 4):  lane l holds row (l & 15), column (l >> 4) of the step
-//   B (4 x 16):  lane l holds column (l >> 4), right-hand side (l & 15)
-//   D (16 x 16): lane l, register r holds row 4 (l >> 4) + r,
-//                right-hand side (l & 15)
-// Measured on MI355X at N = 100,352 (16-RHS product, phase 1 included):
-// kMfRb 4 1.79-1.80 ms, 2 2.15 ms, 1 2.71 ms; the FP64-accumulating
-// k_hmv_phase2 1.71-1.74 ms.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Phase 2 on FP32 matrix cores for 16 right-hand sides, FP32 storage,
+// NP = 2 or 3 row planes, with contiguous operator loads.
+//
+// v_mfma_f32_16x16x4f32 computes D (16 x 16) += A (16 x 4) B (4 x 16):
+//   A: lane l holds A[l & 15][l >> 4]; B: lane l holds B[l >> 4][l & 15];
+//   D: lane l, register r holds D[4 (l >> 4) + r][l & 15].
+// Lane l loads the four consecutive rows s .. s+3 (s = 4 (l & 15)) of column
+// c + (l >> 4) of a plane with one 16-byte load, so one load instruction
+// reads four whole 64-row columns. Element q of that load is the A operand
+// of product q, whose 16 A rows are therefore the tile rows 4 i + q: four
+// products per plane and 4-column step cover the 64 rows. A tile with
+// mt < 64 rows clamps the row group start to mt - 4, so no load leaves the
+// tile's rows; rows duplicated by the clamp are stored once. B is the step's
+// weights (the FP32 copies of v1, v2, or the FP32 t of phase 1; one 256-byte
+// load per wavefront). Each item is summed from zero on the matrix cores and then added into
+// a compensated FP32 pair (hi, lo) per output value (TwoSum: the exact
+// rounding error of every hi + item addition is kept in lo), so across items the
+// sum carries about twice the FP32 precision; the four item groups' hi + lo
+// are added in FP64 in group order. Tiles need at least 4 rows
+// (host-checked).
+constexpr int kMf2Waves = kTileWaves;
 
-// 16-row blocks per wavefront: a tile's 64 rows are split over 4 / kMfRb
-// wavefronts per item group (fewer accumulators per wave, more waves).
-#ifndef PQ3D_MF_RB
-#define PQ3D_MF_RB 4
-#endif
-constexpr int kMfRb = PQ3D_MF_RB;
-constexpr int kMfWaves = kTileWaves * (4 / kMfRb);
-static_assert(kMfRb == 1 || kMfRb == 2 || kMfRb == 4, "bad row split");
+__device__ __forceinline__ f32x4 buf_load4(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
+  return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kBufAux));
+}
 
-// One 4-column step of an item: off[o][rb] is the lane's byte offset of
-// plane o, row block rb, column kl of the step (per lane, voffset); the
-// step's first column c goes in soffset, the same for the whole wavefront.
-// In the masked last step a column past the item's end reads the item's last
-// column (in bounds), and its A and B values are zeroed.
-template <bool MASK>
-__device__ __forceinline__ void mf_load(__amdgpu_buffer_rsrc_t rs,
-                                        const uint32_t (&off)[3][kMfRb], int ldb, int64_t c,
-                                        int kl, int il, int64_t cnt,
-                                        const double* __restrict__ xs, float (&a)[3][kMfRb],
-                                        double& b) {
+template <int NP>
+struct Mf2Step {
+  f32x4 a[NP];
+  float b;
+};
+
+template <int NP, bool MASK>
+__device__ __forceinline__ void mf2_load(__amdgpu_buffer_rsrc_t rs, const uint32_t (&off)[NP],
+                                         int ldb, int64_t c, int kl, int il, int64_t cnt,
+                                         const float* __restrict__ xs, Mf2Step<NP>& st) {
   const bool in = !MASK || c + kl < cnt;
   const int64_t col = in ? c + kl : cnt - 1;
+  // masked columns read the item's last column (in bounds) and are zeroed
   const uint32_t back = MASK ? (uint32_t)((c + kl - col) * ldb) : 0u;
   const int soff = (int)c * ldb;
 #pragma unroll
-  for (int o = 0; o < 3; ++o)
-#pragma unroll
-    for (int rb = 0; rb < kMfRb; ++rb) {
-      const float v = buf_load<float>(rs, off[o][rb] - back, soff);
-      a[o][rb] = in ? v : 0.0f;
-    }
-  const double x = xs[col * 16 + il];
-  b = in ? x : 0.0;
+  for (int o = 0; o < NP; ++o) {
+    const f32x4 v = buf_load4(rs, off[o] - back, soff);
+    st.a[o] = in ? v : f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  const float x = xs[col * 16 + il];
+  st.b = in ? x : 0.0f;
 }
 
-__device__ __forceinline__ void mf_mma(const float (&a)[3][kMfRb], double b,
-                                       f32x4 (&acc)[3][kMfRb]) {
-  const float bf = (float)b;
+template <int NP>
+__device__ __forceinline__ void mf2_mma(const Mf2Step<NP>& st, f32x4 (&acc)[NP][4]) {
 #pragma unroll
-  for (int o = 0; o < 3; ++o)
+  for (int o = 0; o < NP; ++o)
 #pragma unroll
-    for (int rb = 0; rb < kMfRb; ++rb)
-      acc[o][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[o][rb], bf, acc[o][rb], 0, 0, 0);
+    for (int q = 0; q < 4; ++q)
+      acc[o][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(st.a[o][q], st.b, acc[o][q], 0, 0, 0);
 }
 
-// The item group's items of one tile, for the wavefront's row blocks
-// rb0 .. rb0 + kMfRb - 1 (rows past the tile's last repeat it: in bounds,
-// their sums are not stored). Each step's loads are issued one step ahead of
-// its products.
-__device__ __forceinline__ void mf_items(const int64_t* __restrict__ items, int64_t q0,
-                                         int64_t q1, const float* __restrict__ pool,
-                                         const double* __restrict__ v1,
-                                         const double* __restrict__ v2,
-                                         const double* __restrict__ t, int64_t mt, int rb0,
-                                         int kl, int il, f32x4 (&acc)[3][kMfRb]) {
+// COMP: items added into compensated (hi, lo) FP32 pairs; otherwise every
+// item group is summed in plain FP32 on the matrix cores (fewer registers,
+// more waves per SIMD).
+template <int NP, bool COMP>
+__global__ void __launch_bounds__(64 * kMf2Waves) k_hmv_phase2_mf16(
+    const int64_t* __restrict__ tiles, const int64_t* __restrict__ tile_ptr,
+    const int64_t* __restrict__ items, const float* __restrict__ pool,
+    const float* __restrict__ x1, const float* __restrict__ x2, const float* __restrict__ t,
+    double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3, int64_t ybs) {
   constexpr int NB = 16;
-  for (int64_t q = q0; q < q1; q += kTileWaves) {
-    const int64_t* it = items + q * IT_W;
+  __shared__ double red[NP][NB][64];
+  const int64_t tile = xcd_order(blockIdx.x, gridDim.x);
+  const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
+  const int g = wave_id();
+  const int lane = threadIdx.x & 63;
+  const int kl = lane >> 4, il = lane & 15;
+  // the lane's row group: rows s .. s+3 of the tile (clamped into the tile)
+  const int s0 = 4 * il < (int)mt - 4 ? 4 * il : (int)mt - 4;
+  f32x4 hi[NP][4], lo[NP][4];
+#pragma unroll
+  for (int o = 0; o < NP; ++o)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      hi[o][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+      lo[o][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  f32x4 accd[NP][4];
+  f32x4 (&acc)[NP][4] = COMP ? accd : hi;
+  for (int64_t qi = tile_ptr[tile] + g; qi < tile_ptr[tile + 1]; qi += kMf2Waves) {
+    const int64_t* it = items + qi * IT_W;
     const int64_t kind = it[IT_KIND], base = it[IT_BASE], ld = it[IT_LD], plane = it[IT_PLANE];
     const int64_t cnt = it[IT_COUNT], xo = it[IT_XOFF];
-    // the item spans (cnt - 1) ld + 2 plane + mt elements from pool + base
+    // the item spans (cnt - 1) ld + (NP - 1) plane + mt elements from pool + base
     // (host-checked: under 2^31 bytes)
-    const __amdgpu_buffer_rsrc_t rs =
-        item_rsrc(pool + base, (uint64_t)((cnt - 1) * ld + 2 * plane + mt) * sizeof(float));
+    const __amdgpu_buffer_rsrc_t rs = item_rsrc(
+        pool + base, (uint64_t)((cnt - 1) * ld + (NP - 1) * plane + mt) * sizeof(float));
     const int ldb = (int)(ld * (int64_t)sizeof(float));
-    uint32_t off[3][kMfRb];
+    uint32_t off[NP];
 #pragma unroll
-    for (int rb = 0; rb < kMfRb; ++rb) {
-      const int64_t r0 = (int64_t)(rb0 + rb) * 16 + il;
-      const int64_t r = r0 < mt ? r0 : mt - 1;
+    for (int o = 0; o < NP; ++o)
+      off[o] = (uint32_t)((o * plane + s0) * sizeof(float)) + (uint32_t)(kl * ldb);
+    const float* xs = (kind == 0 ? x1 : kind == 1 ? t : x2) + xo * NB;
+    if constexpr (COMP) {
 #pragma unroll
-      for (int o = 0; o < 3; ++o)
-        off[o][rb] = (uint32_t)((o * plane + r) * sizeof(float)) + (uint32_t)(kl * ldb);
+      for (int o = 0; o < NP; ++o)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[o][q] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
-    const double* xs = (kind == 0 ? v1 : kind == 1 ? t : v2) + xo * NB;
     int64_t c = 0;
     if (cnt >= 4) {
-      float a[3][kMfRb];
-      double b;
-      mf_load<false>(rs, off, ldb, 0, kl, il, cnt, xs, a, b);
+      Mf2Step<NP> cur;
+      mf2_load<NP, false>(rs, off, ldb, 0, kl, il, cnt, xs, cur);
       for (; c + 8 <= cnt; c += 4) {
-        float an[3][kMfRb];
-        double bn;
-        mf_load<false>(rs, off, ldb, c + 4, kl, il, cnt, xs, an, bn);
-        mf_mma(a, b, acc);
-#pragma unroll
-        for (int o = 0; o < 3; ++o)
-#pragma unroll
-          for (int rb = 0; rb < kMfRb; ++rb) a[o][rb] = an[o][rb];
-        b = bn;
+        Mf2Step<NP> nxt;
+        mf2_load<NP, false>(rs, off, ldb, c + 4, kl, il, cnt, xs, nxt);
+        mf2_mma<NP>(cur, acc);
+        cur = nxt;
       }
-      mf_mma(a, b, acc);
+      mf2_mma<NP>(cur, acc);
       c += 4;
     }
     if (c < cnt) {
-      float a[3][kMfRb];
-      double b;
-      mf_load<true>(rs, off, ldb, c, kl, il, cnt, xs, a, b);
-      mf_mma(a, b, acc);
+      Mf2Step<NP> last;
+      mf2_load<NP, true>(rs, off, ldb, c, kl, il, cnt, xs, last);
+      mf2_mma<NP>(last, acc);
     }
+    // compensated accumulation across items (TwoSum: the exact error of
+    // hi + x goes into lo, whatever their magnitudes)
+    if constexpr (COMP)
+#pragma unroll
+    for (int o = 0; o < NP; ++o)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float a = hi[o][q][r], x = acc[o][q][r];
+          const float sum = a + x;
+          const float bb = sum - a;
+          const float err = (a - (sum - bb)) + (x - bb);
+          hi[o][q][r] = sum;
+          lo[o][q][r] += err;
+        }
   }
-}
-
-__global__ void __launch_bounds__(64 * kMfWaves) PQ3D_P2_ATTR k_hmv_phase2_mfma16(
-    const int64_t* __restrict__ tiles, const int64_t* __restrict__ tile_ptr,
-    const int64_t* __restrict__ items, const float* __restrict__ pool,
-    const double* __restrict__ v1, const double* __restrict__ v2, const double* __restrict__ t,
-    double* __restrict__ y1, double* __restrict__ y2, double* __restrict__ y3, int64_t ybs) {
-  constexpr int NB = 16;
-  __shared__ float red[3][NB][64];
-  const int64_t tile = xcd_order(blockIdx.x, gridDim.x);
-  const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
-  const int w = wave_id();
-  const int g = w % kTileWaves;           // item group
-  const int rb0 = (w / kTileWaves) * kMfRb;  // first 16-row block of this wavefront
-  const int lane = threadIdx.x & 63;
-  const int kl = lane >> 4, il = lane & 15;
-  f32x4 acc[3][kMfRb];
-#pragma unroll
-  for (int o = 0; o < 3; ++o)
-#pragma unroll
-    for (int rb = 0; rb < kMfRb; ++rb) acc[o][rb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  // row blocks wholly past the tile's rows have nothing to compute
-  if (rb0 * 16 < mt)
-    mf_items(items, tile_ptr[tile] + g, tile_ptr[tile + 1], pool, v1, v2, t, mt, rb0, kl, il,
-             acc);
-  // deterministic cross-group sum: group 0 stores, groups 1.. add in order
-  // (the wavefronts of one group own disjoint rows)
-  for (int gg = 0; gg < kTileWaves; ++gg) {
+  // deterministic cross-group sum in FP64: group 0 stores, groups 1.. add in
+  // order; each row is taken from the lane group that owns it (rows the
+  // clamp duplicated are skipped)
+  for (int gg = 0; gg < kMf2Waves; ++gg) {
     if (g == gg) {
 #pragma unroll
-      for (int o = 0; o < 3; ++o)
+      for (int r = 0; r < 4; ++r) {
+        const int i = 4 * kl + r;
+        const int si = 4 * i < (int)mt - 4 ? 4 * i : (int)mt - 4;
 #pragma unroll
-        for (int rb = 0; rb < kMfRb; ++rb)
+        for (int q = 0; q < 4; ++q) {
+          const int row = si + q;
+          if (row >= 4 * i && row < mt) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            float* p = &red[o][il][(rb0 + rb) * 16 + 4 * kl + r];
-            *p = (gg == 0) ? acc[o][rb][r] : *p + acc[o][rb][r];
+            for (int o = 0; o < NP; ++o) {
+              const double v = (double)hi[o][q][r] + (double)lo[o][q][r];
+              double* p = &red[o][il][row];
+              *p = (gg == 0) ? v : *p + v;
+            }
           }
+        }
+      }
     }
     __syncthreads();
   }
   double* const ys[3] = {y1, y2, y3};
-  for (int e = threadIdx.x; e < 3 * NB * 64; e += 64 * kMfWaves) {
+  for (int e = threadIdx.x; e < NP * NB * 64; e += 64 * kMf2Waves) {
     const int o = e / (NB * 64), m = e % (NB * 64);
     // consecutive threads write consecutive addresses in either output layout
     const int row = ybs > 0 ? m % 64 : m / NB, b = ybs > 0 ? m / 64 : m % NB;
     if (row < mt)
-      ys[o][ybs > 0 ? b * ybs + R0 + row : (R0 + row) * NB + b] = (double)red[o][b][row];
+      ys[o][ybs > 0 ? b * ybs + R0 + row : (R0 + row) * NB + b] = red[o][b][row];
+  }
+}
+
+// FP32 copies of the two interleaved input vectors (m values each): the
+// matrix-core products take their weights in FP32.
+__global__ void __launch_bounds__(256) k_inputs_f32(int64_t m, const double* __restrict__ v1,
+                                                    const double* __restrict__ v2,
+                                                    float* __restrict__ x1, float* __restrict__ x2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  x1[i] = (float)v1[i];
+  x2[i] = (float)v2[i];
+}
+
+// Phase 1 on FP32 matrix cores for 16 right-hand sides, FP32 storage:
+// t[l][b] = sum over the unit's columns c of V^T[c][l] x_b[c], one wavefront
+// per unit, its member blocks (and rank chunks of 64) one after another.
+// A = V^T (ranks x 4 columns), B = the 4 columns' weights (rounded to FP32),
+// D = ranks x 16 right-hand sides. Lane l loads ranks s .. s+3 of column
+// c + (l >> 4) with one 16-byte load (the four columns of a step are
+// contiguous in V^T, so a load instruction reads 4 k consecutive values);
+// element q is the A operand of product q, whose rows are ranks 4 i + q. The
+// row group start is clamped to k - 4 (to 0 when k < 4: ranks past the
+// chunk then read other data and are not stored). The weights are the FP32
+// copies x1 = v1, x2 = v2 (interleaved, 16 per cell); each unit's sum (at
+// most 512 columns) is formed in FP32 and stored as FP32 rows of t or of the
+// partial buffer.
+__global__ void __launch_bounds__(256) k_hmv_phase1_mf16(const int64_t* __restrict__ units,
+                                                         int64_t nunits,
+                                                         const int64_t* __restrict__ members,
+                                                         const float* __restrict__ pool,
+                                                         const float* __restrict__ x1,
+                                                         const float* __restrict__ x2,
+                                                         float* __restrict__ t,
+                                                         float* __restrict__ part) {
+  constexpr int NB = 16;
+  const int64_t u = xcd_order(blockIdx.x, gridDim.x) * (blockDim.x / 64) + wave_id();
+  if (u >= nunits) return;
+  const int lane = threadIdx.x & 63;
+  const int kl = lane >> 4, il = lane & 15;
+  const int64_t* it = units + u * UN_W;
+  const int64_t mem0 = it[UN_MEM], nmem = it[UN_NMEM], cb = it[UN_CB], cc = it[UN_CC];
+  const int64_t n = it[UN_N], c0 = it[UN_C0];
+  float* dst = it[UN_DIRECT] ? t : part;
+  for (int64_t mi = 0; mi < nmem; ++mi) {
+    const int64_t* mb = members + (mem0 + mi) * MB_W;
+    const int64_t vt = mb[MB_VT], k = mb[MB_K], out = mb[MB_OUT];
+    // the member's V^T: 2n x k row-major at pool + vt; the unit reads its
+    // columns [cb, cb + cc)
+    const __amdgpu_buffer_rsrc_t rs =
+        item_rsrc(pool + vt + cb * k, (uint64_t)(cc * k) * sizeof(float));
+    const int ldb = (int)(k * (int64_t)sizeof(float));
+    for (int64_t l0 = 0; l0 < k; l0 += 64) {
+      const int kc = (int)(k - l0 < 64 ? k - l0 : 64);
+      const int s0 = (int)l0 + (kc >= 4 ? (4 * il < kc - 4 ? 4 * il : kc - 4) : 0);
+      const uint32_t voff = (uint32_t)(s0 * sizeof(float)) + (uint32_t)(kl * ldb);
+      f32x4 acc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+      // four 4-column steps per iteration: all their loads are issued before
+      // the products
+      int64_t c = 0;
+      for (; c + 16 <= cc; c += 16) {
+        f32x4 a[4];
+        float x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int64_t g = cb + c + 4 * j + kl;
+          x[j] = g < n ? x1[(c0 + g) * NB + il] : x2[(c0 + g - n) * NB + il];
+          a[j] = buf_load4(rs, voff, (int)((c + 4 * j) * ldb));
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float bf = x[j];
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j][q], bf, acc[q], 0, 0, 0);
+        }
+      }
+      for (; c < cc; c += 4) {
+        const int64_t col = c + kl;  // column of the unit (0 .. cc-1)
+        const bool in = col < cc;
+        const int64_t cl = in ? col : cc - 1;
+        const int64_t g = cb + cl;   // stacked column of the block
+        const float x = g < n ? x1[(c0 + g) * NB + il] : x2[(c0 + g - n) * NB + il];
+        const f32x4 a = buf_load4(rs, voff - (uint32_t)((col - cl) * ldb), (int)(c * ldb));
+        const float bf = in ? x : 0.0f;
+        const f32x4 av = in ? a : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bf, acc[q], 0, 0, 0);
+      }
+      // lane l, register r holds ranks s_i + q (i = 4 kl + r) of right-hand
+      // side il; ranks the clamp duplicated are stored once
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 4 * kl + r;
+        const int si = kc >= 4 ? (4 * i < kc - 4 ? 4 * i : kc - 4) : 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rk = si + q;
+          if (rk >= 4 * i && rk < kc) dst[(out + l0 + rk) * NB + il] = acc[q][r];
+        }
+      }
+    }
   }
 }
 
@@ -1049,40 +1162,65 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_me
   return PQ3D_OK;
 }
 
-// pq3d_hmat_matvec for FP32 storage and 16 right-hand sides with the block
-// products on FP32 matrix cores: phase 1 (V^T x) as pq3d_hmat_matvec in FP64,
-// phase 2 (dense and U blocks) with weights rounded to FP32, FP32 sums over
-// all of an item group's dense and U columns, and the four groups' sums added
-// in FP32 (in LDS, in group order) before the FP64 store. Same arguments;
-// dtype_bytes must be 4, nrhs 16 and d_y3 non-NULL.
+// pq3d_hmat_matvec for FP32 storage and 16 right-hand sides with every block
+// product on FP32 matrix cores. The inputs are first copied to FP32 (d_x32:
+// 2 n 16 floats of scratch, n = rows); phase 1 (V^T x, k_hmv_phase1_mf16)
+// sums each unit's columns in FP32 into FP32 rows of t (d_t and d_part are
+// used as FP32 arrays; partials of split blocks added in FP64 by phase 1b);
+// phase 2 (k_hmv_phase2_mf16) takes those FP32 weights, sums each
+// item in FP32 on the matrix cores, accumulates the items of an item group in
+// a compensated FP32 pair (compensated != 0) or in plain FP32, and adds the
+// four groups in FP64. Same arguments;
+// dtype_bytes must be 4 and nrhs 16; d_y3 may be NULL (two row planes);
+// every tile must have at least 4 rows.
 int pq3d_hmat_matvec_mfma(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
                           const int64_t* d_red, int64_t nred, const int64_t* d_tiles,
                           const int64_t* d_tile_ptr, int64_t ntiles, const int64_t* d_items,
                           const void* d_pool, int dtype_bytes, int nrhs, const double* d_v1,
                           const double* d_v2, double* d_t, double* d_part, double* d_y1,
-                          double* d_y2, double* d_y3, int64_t y_bstride, void* stream) {
+                          double* d_y2, double* d_y3, int64_t y_bstride, float* d_x32,
+                          int64_t n, int compensated, void* stream) {
   clear_error();
   PQ3D_REQUIRE(dtype_bytes == 4, "the matrix-core matvec needs FP32 storage (dtype_bytes 4)");
   PQ3D_REQUIRE(nrhs == 16, "the matrix-core matvec takes 16 right-hand sides");
-  PQ3D_REQUIRE(ntiles >= 0 && nunits >= 0 && nred >= 0, "sizes must be >= 0");
+  PQ3D_REQUIRE(ntiles >= 0 && nunits >= 0 && nred >= 0 && n >= 0, "sizes must be >= 0");
   PQ3D_REQUIRE(y_bstride >= 0, "y_bstride must be >= 0");
   PQ3D_REQUIRE(d_tiles && d_tile_ptr && d_items && d_pool && d_v1 && d_v2 && d_y1 && d_y2 &&
-                   d_y3,
+                   d_x32,
                "null pointer");
   PQ3D_REQUIRE(nunits == 0 || (d_units && d_members && d_t), "null pointer");
   PQ3D_REQUIRE(nred == 0 || (d_red && d_part), "null pointer");
-  PQ3D_REQUIRE(nunits < (1LL << 26) && nred < (1LL << 26) && ntiles < (1LL << 22),
+  PQ3D_REQUIRE(nunits < (1LL << 26) && nred < (1LL << 26) && ntiles < (1LL << 22) &&
+                   n < (1LL << 26),
                "too many work items");
   hipStream_t s = as_stream(stream);
   const float* pool = (const float*)d_pool;
+  const int64_t m = n * 16;
+  float* x1 = d_x32;
+  float* x2 = d_x32 + m;
+  float* t32 = (float*)d_t;
+  float* part32 = (float*)d_part;
+  if (m > 0)
+    k_inputs_f32<<<blocks_for(m, 256), 256, 0, s>>>(m, d_v1, d_v2, x1, x2);
   if (nunits > 0)
-    k_hmv_phase1<float, 16><<<(unsigned)((nunits + 3) / 4), 256, 0, s>>>(
-        d_units, nunits, d_members, pool, d_v1, d_v2, d_t, d_part);
+    k_hmv_phase1_mf16<<<(unsigned)((nunits + 3) / 4), 256, 0, s>>>(
+        d_units, nunits, d_members, pool, x1, x2, t32, part32);
   if (nred > 0)
-    k_hmv_phase1b<16><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(d_red, nred, d_part, d_t);
-  if (ntiles > 0)
-    k_hmv_phase2_mfma16<<<(unsigned)ntiles, 64 * kMfWaves, 0, s>>>(
-        d_tiles, d_tile_ptr, d_items, pool, d_v1, d_v2, d_t, d_y1, d_y2, d_y3, y_bstride);
+    k_hmv_phase1b<16, float><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(d_red, nred, part32,
+                                                                        t32);
+  if (ntiles > 0) {
+#define PQ3D_P2MF(NPV, CV, Y3)                                                               \
+  k_hmv_phase2_mf16<NPV, CV><<<(unsigned)ntiles, 64 * kMf2Waves, 0, s>>>(                   \
+      d_tiles, d_tile_ptr, d_items, pool, x1, x2, t32, d_y1, d_y2, Y3, y_bstride)
+    if (d_y3) {
+      if (compensated) PQ3D_P2MF(3, true, d_y3);
+      else PQ3D_P2MF(3, false, d_y3);
+    } else {
+      if (compensated) PQ3D_P2MF(2, true, nullptr);
+      else PQ3D_P2MF(2, false, nullptr);
+    }
+#undef PQ3D_P2MF
+  }
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }

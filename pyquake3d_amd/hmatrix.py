@@ -35,6 +35,11 @@ DENSE_CHUNK = 32   # columns per dense work item
 LR_CHUNK = 16      # rank columns per low-rank work item
 UNIT_COLS = 512    # stacked V columns per phase-1 unit
 NRHS = (1, 2, 4, 8, 16)
+# "f64": every product summed in FP64 (bitwise independent of nrhs); with FP32
+# storage and 16 right-hand sides, "mfma" runs every block product on FP32
+# matrix cores (FP32 sums per phase-1 unit and per phase-2 item group) and
+# "mfma_comp" adds the items of a group in compensated FP32 (hi + lo)
+ACCUMULATE = ("f64", "mfma", "mfma_comp")
 
 
 def _sync(dev):
@@ -182,6 +187,33 @@ def pack_units(groups):
             np.ascontiguousarray(np.array(members, dtype=np.int64).reshape(-1, 4)))
 
 
+def xcd_chunks(g):
+    """Workgroup ranges of the eight XCDs for a grid of g workgroups, as the
+    kernels' xcd_order maps them: XCD x takes a contiguous range of q + 1
+    (x < r) or q workgroups, q = g // 8, r = g % 8."""
+    q, r = divmod(int(g), 8)
+    sizes = [q + 1 if x < r else q for x in range(8)]
+    return np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+
+
+def xcd_balanced_order(cost, per_group):
+    """Order of work items (``per_group`` per workgroup) that keeps every
+    XCD's contiguous range of work but puts the costliest items of each
+    range first: workgroups are dispatched in order, so the cheap ones fill
+    the tail instead of a long one running alone. The order changes no
+    result (every item writes its own rows)."""
+    cost = np.asarray(cost, dtype=np.float64)
+    w = len(cost)
+    g = -(-w // per_group)
+    b = xcd_chunks(g) * per_group
+    out = []
+    for x in range(8):
+        lo, hi = min(int(b[x]), w), min(int(b[x + 1]), w)
+        idx = np.arange(lo, hi)
+        out.append(idx[np.argsort(-cost[lo:hi], kind="stable")])
+    return np.concatenate(out).astype(np.int64)
+
+
 def check_item_extents(items, tiles, tile_ptr, esize) -> None:
     """Raise unless every phase-2 item spans less than 2^31 bytes: the kernel
     addresses an item through one buffer resource with 32-bit byte offsets
@@ -257,9 +289,20 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
         toff += k
     units, members = pack_units(groups)
     it = np.array(items, dtype=np.int64).reshape(-1, 7)
+    # tiles in XCD-balanced order (see xcd_balanced_order): cost = columns
+    cost = np.bincount(it[:, 0], weights=it[:, 5], minlength=ntiles)
+    order = xcd_balanced_order(cost, 1)
+    rank = np.empty(ntiles, dtype=np.int64)
+    rank[order] = np.arange(ntiles)
+    tiles = np.ascontiguousarray(tiles[order])
+    it[:, 0] = rank[it[:, 0]]
     it = it[np.argsort(it[:, 0], kind="stable")]
     counts = np.bincount(it[:, 0], minlength=ntiles)
     tile_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    if len(units):
+        # phase-1 units likewise (4 per workgroup): cost = columns x ranks
+        uo = xcd_balanced_order(units[:, 3] * units[:, 7], 4)
+        units = np.ascontiguousarray(units[uo])
     return dict(items=np.ascontiguousarray(it[:, 1:]), tile_ptr=tile_ptr,
                 tiles=tiles,
                 units=units, members=members,
@@ -277,10 +320,7 @@ class HMatrix:
         self.esize = 4 if dtype == torch.float32 else 8
         if dtype not in (torch.float32, torch.float64):
             raise ValueError("dtype must be float32 or float64")
-        # "f64": every product summed in FP64 (bitwise independent of nrhs);
-        # "mfma": with FP32 storage and 16 right-hand sides, the dense and U
-        # products run on FP32 matrix cores (FP32 sums per item group)
-        self.accumulate = "f64"
+        self.accumulate = "f64"      # one of ACCUMULATE
         self.mu, self.lambda_ = float(mu), float(lambda_)
         nodelst = np.asarray(nodelst, dtype=np.float64)
         elelst = np.asarray(elelst, dtype=np.int64)
@@ -450,6 +490,7 @@ class HMatrix:
         self.reduce = torch.from_numpy(W["reduce"]).to(dev)
         self.nred = W["reduce"].shape[0]
         self.nt, self.npart = W["nt"], W["npart"]
+        self.min_tile_rows = int(W["tiles"][:, 1].min()) if len(W["tiles"]) else 0
         self._scratch = {}
         _lib.check_width("item", self.items)
         _lib.check_width("unit", self.units)
@@ -469,6 +510,8 @@ class HMatrix:
                 torch.zeros(max(self.nt, 1) * nrhs, dtype=torch.float64, device=self.device),
                 torch.zeros(max(self.npart, 1) * nrhs, dtype=torch.float64, device=self.device),
                 red_d)
+            if nrhs == 16 and self.esize == 4:
+                self._x32(nrhs)      # created here so that graph capture finds it
         return self._scratch[nrhs]
 
     # ----------------------------------------------------------------- apply
@@ -496,19 +539,31 @@ class HMatrix:
         if tuple(out.shape) != (3, n * nrhs) or not out.is_contiguous() or out.dtype != torch.float64:
             raise ValueError("out must be a contiguous (3, n*nrhs) float64 tensor")
         t, part, red = self._bufs(nrhs)
-        if self.accumulate not in ("f64", "mfma"):
-            raise ValueError("accumulate must be 'f64' or 'mfma'")
-        mfma = self.accumulate == "mfma" and nrhs == 16 and self.esize == 4
-        y3 = _lib.ptr(out[2]) if (planes == 3 or mfma) else None
-        _lib.call("pq3d_hmat_matvec_mfma" if mfma else "pq3d_hmat_matvec", _lib.ptr(self.units),
-                  self.nunits, _lib.ptr(self.members),
-                  _lib.ptr(red), int(red.shape[0]), _lib.ptr(self.tiles), _lib.ptr(self.tile_ptr),
-                  self.ntiles,
-                  _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
-                  _lib.ptr(v2), _lib.ptr(t), _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]),
-                  y3, n if (instance_major and nrhs > 1) else 0,
-                  _lib.stream_handle(stream))
+        if self.accumulate not in ACCUMULATE:
+            raise ValueError(f"accumulate must be one of {ACCUMULATE}")
+        mfma = self.accumulate.startswith("mfma") and nrhs == 16 and self.esize == 4
+        if mfma and self.min_tile_rows < 4:
+            raise ValueError("the matrix-core product needs tiles of at least 4 rows")
+        y3 = _lib.ptr(out[2]) if planes == 3 else None
+        args = [_lib.ptr(self.units), self.nunits, _lib.ptr(self.members), _lib.ptr(red),
+                int(red.shape[0]), _lib.ptr(self.tiles), _lib.ptr(self.tile_ptr), self.ntiles,
+                _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
+                _lib.ptr(v2), _lib.ptr(t), _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]),
+                y3, n if (instance_major and nrhs > 1) else 0]
+        if mfma:
+            _lib.call("pq3d_hmat_matvec_mfma", *args, _lib.ptr(self._x32(nrhs)), n,
+                      int(self.accumulate == "mfma_comp"), _lib.stream_handle(stream))
+        else:
+            _lib.call("pq3d_hmat_matvec", *args, _lib.stream_handle(stream))
         return out
+
+    def _x32(self, nrhs):
+        """FP32 scratch of the matrix-core product: the two inputs rounded."""
+        buf = getattr(self, "_x32_buf", None)
+        if buf is None or buf.numel() < 2 * self.n * nrhs:
+            buf = torch.empty(2 * self.n * nrhs, dtype=torch.float32, device=self.device)
+            self._x32_buf = buf
+        return buf
 
     def algorithmic_bytes(self, nrhs=1, planes=3):
         """Bytes one product must move at least: every stored operator
@@ -591,6 +646,7 @@ class HMatrix:
         for k in ("units", "members", "reduce", "tiles", "tile_ptr", "items"):
             setattr(self, k, t[k].to(dev))
         self.nunits, self.nred, self.ntiles = len(t["units"]), len(t["reduce"]), len(t["tiles"])
+        self.min_tile_rows = int(t["tiles"][:, 1].min()) if self.ntiles else 0
         for name in ("unit", "member", "tile", "item"):
             _lib.check_width(name, getattr(self, name + "s"))
         check_item_extents(t["items"].numpy(), t["tiles"].numpy(), t["tile_ptr"].numpy(),
@@ -702,6 +758,7 @@ class MultiHMatrix:
         self.items = cat(items, 6)
         self.tile_ptr = torch.from_numpy(np.concatenate(ptr).astype(np.int64)).to(dev)
         self.nt, self.npart = int(t_off[-1]), int(p_off[-1])
+        self.min_tile_rows = int(self.tiles[:, 1].min()) if self.ntiles else 0
         self._scratch = {}
         for name, tab in (("item", self.items), ("unit", self.units), ("member", self.members),
                           ("tile", self.tiles)):
@@ -710,6 +767,7 @@ class MultiHMatrix:
 
     _bufs = HMatrix._bufs
     matvec = HMatrix.matvec
+    _x32 = HMatrix._x32
 
     def algorithmic_bytes(self, nrhs=1, planes=3):
         """``HMatrix.algorithmic_bytes`` summed over the instances (one

@@ -43,6 +43,18 @@ def test_two_plane_product_equals_first_rows(gpu, hm, dtype, nrhs):
     assert bool((out[2] == 7.0).all())
     b3, b2 = H.algorithmic_bytes(nrhs, 3), H.algorithmic_bytes(nrhs, 2)
     assert b2 < b3 and b3 == H.matvec_bytes + 5 * 8 * n * (nrhs - 1)
+    if dtype == torch.float32 and nrhs == 16:
+        # the matrix-core product: the two-plane kernel computes the first two
+        # planes exactly as the three-plane one
+        H.accumulate = "mfma"
+        try:
+            full = H.matvec(v1, v2, nrhs=nrhs, instance_major=True)
+            out.fill_(7.0)
+            H.matvec(v1, v2, out=out, nrhs=nrhs, instance_major=True, planes=2)
+        finally:
+            H.accumulate = "f64"
+        assert torch.equal(out[:2], full[:2])
+        assert bool((out[2] == 7.0).all())
 
 
 def test_fix_tn_steps_do_not_depend_on_the_skipped_rows(gpu, hm):

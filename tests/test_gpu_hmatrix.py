@@ -128,8 +128,9 @@ def test_multi_rhs_products_match_single(gpu, pq3d_lib, case, dtype, leaf):
             assert torch.equal(yi[:, b, :], single[b]), ("instance-major", nb, b)
 
 
+@pytest.mark.parametrize("mode", ["mfma", "mfma_comp"])
 @pytest.mark.parametrize("leaf", [32, 64])
-def test_mfma_matvec_16_rhs(gpu, pq3d_lib, case, leaf):
+def test_mfma_matvec_16_rhs(gpu, pq3d_lib, case, leaf, mode):
     """FP32 storage, 16 right-hand sides, dense and U products on FP32 matrix
     cores: within the FP32 path's stated bound (1e-3 of the largest value)
     of the dense FP64 product, within 1e-5 of the FP64-accumulated product of
@@ -149,7 +150,7 @@ def test_mfma_matvec_16_rhs(gpu, pq3d_lib, case, leaf):
     X1 = torch.from_numpy(np.ascontiguousarray(V1[H.perm])).to(gpu)
     X2 = torch.from_numpy(np.ascontiguousarray(V2[H.perm])).to(gpu)
     y64 = H.matvec(X1.view(-1), X2.view(-1), nrhs=16).view(3, n, 16).cpu().numpy()
-    H.accumulate = "mfma"
+    H.accumulate = mode
     y = H.matvec(X1.view(-1), X2.view(-1), nrhs=16).view(3, n, 16)
     assert torch.equal(y, H.matvec(X1.view(-1), X2.view(-1), nrhs=16).view(3, n, 16))
     yi = H.matvec(X1.view(-1), X2.view(-1), nrhs=16, instance_major=True).view(3, 16, n)
