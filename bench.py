@@ -62,11 +62,13 @@ MI355X_HBM_GBS = 8000.0
 MI355X_FP64_TFLOPS = 78.6
 YEAR = 3.15576e7
 
-# HBM bytes per H-matvec (all three phases) at N = 100,352 from two
-# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of the matvec workload,
+# HBM bytes per H-matvec from two rocprofv3 --pmc passes (FETCH_SIZE, then
+# WRITE_SIZE) of the matvec workload, summed over the product's kernels,
 # FETCH scaled by 2.0 and WRITE by 1.0: the factors the same counters read
-# for a 2 GiB device-to-device copy (profiles/hmatvec_traffic_N100352.md).
-MEASURED_TRAFFIC = {("f32", 1, 3): 6_276_266_539, ("f32", 16, 3): 7_350_784_928}
+# for a 2 GiB device-to-device copy (2 GiB read as 1,048,588 KiB fetched,
+# 2 GiB written as 2,097,152 KiB). Key: (storage, right-hand sides, row
+# planes, accumulation); N = 100,352 (profiles/hmatvec_traffic_N100352.md).
+MEASURED_TRAFFIC = {("f32", 16, 2, "mfma"): 5_349_731_072}
 
 # FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
 # (unit strike and dip slip, both projected), counted operator by operator
@@ -376,7 +378,7 @@ def bench_hstep(args):
         "fp64_noise_floor_dt0x2": floor,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / MI355X_HBM_GBS,
-                     "traffic": MEASURED_TRAFFIC.get(("f32", args.nb, 2)),
+                     "traffic": MEASURED_TRAFFIC.get(("f32", args.nb, 2, args.accumulate)),
                      "kernel": (f"H-matvec, {args.nb} right-hand sides, two row planes "
                                 f"(k_inputs_f32 + k_hmv_phase1_mf16 + k_hmv_phase1b + "
                                 f"k_hmv_phase2_mf16)" if args.accumulate.startswith("mfma")
@@ -473,7 +475,7 @@ def bench_matvec(args):
                                f"ACA eps 1e-4, {nb} right-hand sides, {planes} row planes"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((args.dtype, nb, planes)),
+                     "traffic": MEASURED_TRAFFIC.get((args.dtype, nb, planes, args.accumulate)),
                      "kernel_ms": ms, "algorithmic_bytes": algo},
         "bytes": {"phase1_lowrank_V": v_bytes, "phase2_lowrank_U": u_bytes,
                   "phase2_dense": d_bytes, "vectors": vec_bytes, "total": algo},
