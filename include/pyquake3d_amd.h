@@ -209,7 +209,7 @@ int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
 /* Copy converged factors into the pool (the storage half of ACA_worker,
  * src/Hmatrix.py:1014-1066): U (3m x k) at d_tab[b][4] in one panel per leaf
  * cluster of the block's rows -- the leaf with row offset ro and ls rows in
- * the block holds element (o*m + i, l) at 3*k*ro + l*3*ls + o*ls + (i - ro);
+ * the block holds element (o*m + i, l) at 3*k*ro + o*k*ls + l*ls + (i - ro);
  * V transposed (2n x k, row-major) at d_tab[b][5]; k = d_tab[b][6].
  * d_tiles: leaf clusters as (first row, rows) int64 pairs; d_tile_of_row:
  * leaf index of every cluster-order row. */

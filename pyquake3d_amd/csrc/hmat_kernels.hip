@@ -280,8 +280,9 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
 // Copy the converged factors into the storage pool (type T). U goes into one
 // panel per leaf cluster of the block's rows: the panel of leaf (ro, ls) (row
 // offset in the block, rows) starts at 3*k*ro and holds element (o*m + i, l)
-// at l*3*ls + o*ls + (i - ro), so each row tile of the product reads one
-// contiguous range. V is stored transposed (2n x k, row-major).
+// at o*k*ls + l*ls + (i - ro) (plane-major), so each row tile of the product
+// reads one contiguous range and a product without the third plane skips a
+// whole sub-panel. V is stored transposed (2n x k, row-major).
 template <typename T>
 __global__ void __launch_bounds__(256) k_aca_pack(const int64_t* __restrict__ tab,
                                                   const int64_t* __restrict__ wsoff,
@@ -304,7 +305,7 @@ __global__ void __launch_bounds__(256) k_aca_pack(const int64_t* __restrict__ ta
     const int64_t l = p / m3, r = p - l * m3, o = r / m, i = r - o * m;
     const int64_t* tl = tiles + TL_W * tile_of_row[r0 + i];
     const int64_t ro = tl[0] - r0, ls = tl[1];
-    pu[3 * k * ro + l * 3 * ls + o * ls + (i - ro)] = (T)U[p];
+    pu[3 * k * ro + o * k * ls + l * ls + (i - ro)] = (T)U[p];
   }
   const int64_t C = 2 * n;
   for (int64_t p = threadIdx.x; p < nvv; p += blockDim.x) {

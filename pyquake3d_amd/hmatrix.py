@@ -113,15 +113,17 @@ def leaf_segments(tiles, tile_of_row, r0, m):
 
 def pack_u(U, seg):
     """U (3m x k, rows o*m + i) in pool order: one panel per leaf segment
-    (ro, ls) at offset 3*k*ro, element (o*m + i, l) at l*3*ls + o*ls + i - ro.
-    A row tile then reads one contiguous panel, sharing no cache line with the
-    tiles of the block's other leaves."""
+    (ro, ls) at offset 3*k*ro, plane-major inside the panel: element
+    (o*m + i, l) at o*k*ls + l*ls + i - ro. A row tile then reads one
+    contiguous panel, sharing no cache line with the tiles of the block's
+    other leaves, and the rank columns of one row plane are contiguous (a
+    product that skips the third plane skips a whole sub-panel)."""
     m3, k = U.shape
     m = m3 // 3
     U3 = U.reshape(3, m, k)
     out = np.empty(m3 * k, dtype=U.dtype)
     for ro, ls in seg:
-        out[3 * k * ro:3 * k * (ro + ls)] = U3[:, ro:ro + ls, :].transpose(2, 0, 1).ravel()
+        out[3 * k * ro:3 * k * (ro + ls)] = U3[:, ro:ro + ls, :].transpose(0, 2, 1).ravel()
     return out
 
 
@@ -129,7 +131,7 @@ def unpack_u(flat, m, k, seg):
     """Inverse of ``pack_u``: (3m x k)."""
     U = np.empty((3, m, k), dtype=flat.dtype)
     for ro, ls in seg:
-        U[:, ro:ro + ls, :] = flat[3 * k * ro:3 * k * (ro + ls)].reshape(k, 3, ls).transpose(1, 2, 0)
+        U[:, ro:ro + ls, :] = flat[3 * k * ro:3 * k * (ro + ls)].reshape(3, k, ls).transpose(0, 2, 1)
     return U.reshape(3 * m, k)
 
 
@@ -282,10 +284,10 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
             poff += nu * k
         for q in range(tile_of_row[br0], tile_of_row[br0 + bm - 1] + 1):
             ro, ls = int(lstart[q]) - br0, int(lsize[q])
-            pb = ub + 3 * k * ro   # this leaf's panel of U
+            pb = ub + 3 * k * ro   # this leaf's panel of U (plane-major)
             for l0 in range(0, k, LR_CHUNK):
                 cnt = min(LR_CHUNK, k - l0)
-                items.append((q, 1, pb + l0 * 3 * ls, 3 * ls, ls, cnt, toff + l0))
+                items.append((q, 1, pb + l0 * ls, ls, k * ls, cnt, toff + l0))
         toff += k
     units, members = pack_units(groups)
     it = np.array(items, dtype=np.int64).reshape(-1, 7)
@@ -590,7 +592,7 @@ class HMatrix:
 
     # ------------------------------------------------------------- storage
     FORMAT = "pq3d-hmatrix"
-    VERSION = 1
+    VERSION = 2      # 2: plane-major U panels
     _BLOCK_KEYS = ("r0", "m", "c0", "n", "dense_idx", "d_base", "lr_idx", "ranks", "l_ub", "l_vb")
 
     def save(self, path, **meta) -> None:

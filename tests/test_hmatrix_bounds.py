@@ -154,8 +154,8 @@ class Checker:
             i = np.arange(m)
             assert (ro >= 0).all() and (i >= ro).all() and (i < ro + ls).all()
             assert (ro + ls <= m).all()
-            dst = (3 * k * ro[None, None, :] + np.arange(k)[:, None, None] * 3 * ls[None, None, :]
-                   + np.arange(3)[None, :, None] * ls[None, None, :] + (i - ro)[None, None, :])
+            dst = (3 * k * ro[None, None, :] + np.arange(3)[None, :, None] * k * ls[None, None, :]
+                   + np.arange(k)[:, None, None] * ls[None, None, :] + (i - ro)[None, None, :])
             assert np.array_equal(np.sort(dst.ravel()), np.arange(3 * m * k))
 
     def pq3d_hmat_matvec(self, units, nun, members, red, nred, tiles, tile_ptr, ntiles, items,

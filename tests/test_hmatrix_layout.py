@@ -198,17 +198,19 @@ def test_block_diagonal_union_reproduces_each_product(oracle):
 
 def test_leaf_panels_of_u():
     """U of a block spanning several leaves: every leaf's rows of all three
-    planes and all ranks form one contiguous panel, in leaf order, and the
-    layout of a single-leaf block is plain column-major 3m x k."""
+    planes and all ranks form one contiguous panel, in leaf order, plane-major
+    inside (each plane a column-major ls x k sub-panel), and a single-leaf
+    block is three column-major m x k planes one after another."""
     U = np.arange(3 * 10 * 4, dtype=np.float64).reshape(30, 4)   # m = 10, k = 4
     seg = np.array([[0, 3], [3, 5], [8, 2]])
     flat = pack_u(U, seg)
     for ro, ls in seg:
-        panel = flat[3 * 4 * ro:3 * 4 * (ro + ls)].reshape(4, 3, ls)
+        panel = flat[3 * 4 * ro:3 * 4 * (ro + ls)].reshape(3, 4, ls)
         for o in range(3):
-            assert np.array_equal(panel[:, o, :].T, U[o * 10 + ro:o * 10 + ro + ls, :])
+            assert np.array_equal(panel[o].T, U[o * 10 + ro:o * 10 + ro + ls, :])
     assert np.array_equal(unpack_u(flat, 10, 4, seg), U)
-    assert np.array_equal(pack_u(U, np.array([[0, 10]])), U.T.ravel())
+    assert np.array_equal(pack_u(U, np.array([[0, 10]])),
+                          U.reshape(3, 10, 4).transpose(0, 2, 1).ravel())
 
 
 def test_xcd_order_is_a_bijection():
