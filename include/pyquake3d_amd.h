@@ -239,6 +239,22 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_me
                      double* d_y1, double* d_y2, double* d_y3, int64_t y_bstride,
                      void* stream);
 
+/* pq3d_hmat_matvec without scratch arguments and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The rank-space vector t and the partial
+ * rows (nt*nrhs and npart*nrhs doubles, sizes from the work lists) live in
+ * the library's static device scratch of 8 Mi doubles (64 MiB per GPU, held
+ * until the process ends); a product needing more returns PQ3D_EINVAL.
+ * Calls that overlap on one GPU share that scratch and race. */
+int pq3d_hmat_matvec_noalloc(const int64_t* d_units, int64_t nunits,
+                             const int64_t* d_members, const int64_t* d_red, int64_t nred,
+                             const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
+                             const int64_t* d_items, const void* d_pool, int dtype_bytes,
+                             int nrhs, const double* d_v1, const double* d_v2, int64_t nt,
+                             int64_t npart, double* d_y1, double* d_y2, double* d_y3,
+                             int64_t y_bstride, void* stream);
+
 /* pq3d_hmat_matvec with every block product on FP32 matrix cores (same
  * replaced reference path, src/Hmatrix.py:1445-1538): FP32 storage
  * (dtype_bytes 4), nrhs 16 and tiles of at least 4 rows only. The inputs are

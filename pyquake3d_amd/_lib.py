@@ -83,6 +83,10 @@ SIGNATURES = {
                                  c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_int64, c_void_p]),
+    "pq3d_hmat_matvec_noalloc": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64,
+                                         c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int,
+                                         c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p,
+                                         c_void_p, c_void_p, c_int64, c_void_p]),
     "pq3d_hmat_matvec_mfma": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64,
                                       c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int,
                                       c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

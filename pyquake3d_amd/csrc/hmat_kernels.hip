@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "pq3d_common.h"
 #include "tde_device.h"
@@ -1016,6 +1017,36 @@ static void launch_matvec(const int64_t* units, int64_t nunits, const int64_t* m
   }
 }
 
+// Scratch of the no-allocation matvec (pq3d_hmat_matvec_noalloc): the
+// rank-space vector t and the phase-1 partial rows live in this file-scope
+// device array instead of caller buffers. Every GPU that loads the library
+// holds its own copy (64 MiB) until the process ends. Host code never uses
+// the array's name as an address: the per-device address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr int64_t kScratchDoubles = int64_t(8) << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kScratchMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kScratchMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device).
+static int scratch_of_current_device(double** out) {
+  int dev = -1;
+  PQ3D_HIP_CHECK(hipGetDevice(&dev));
+  PQ3D_REQUIRE(dev >= 0 && dev < kScratchMaxDevices, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PQ3D_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PQ3D_REQUIRE(p != nullptr, "no device address for the matvec scratch");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PQ3D_OK;
+}
+
 }  // namespace pq3d
 
 using namespace pq3d;
@@ -1161,6 +1192,37 @@ int pq3d_hmat_matvec(const int64_t* d_units, int64_t nunits, const int64_t* d_me
 #undef PQ3D_MV
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
+}
+
+// pq3d_hmat_matvec without caller scratch and without any allocation: t
+// (nt*nrhs doubles) and the partial rows (npart*nrhs doubles) go into the
+// library's static device scratch, at most kScratchDoubles doubles in all;
+// larger products are rejected. No element of either needs clearing (phase 1
+// or 1b writes every one before phase 2 reads it). Calls that overlap on one
+// GPU share the scratch and race: serialise them (one stream, or events).
+int pq3d_hmat_matvec_noalloc(const int64_t* d_units, int64_t nunits,
+                             const int64_t* d_members, const int64_t* d_red, int64_t nred,
+                             const int64_t* d_tiles, const int64_t* d_tile_ptr, int64_t ntiles,
+                             const int64_t* d_items, const void* d_pool, int dtype_bytes,
+                             int nrhs, const double* d_v1, const double* d_v2, int64_t nt,
+                             int64_t npart, double* d_y1, double* d_y2, double* d_y3,
+                             int64_t y_bstride, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(nt >= 0 && npart >= 0, "nt and npart must be >= 0");
+  PQ3D_REQUIRE(nrhs == 1 || nrhs == 2 || nrhs == 4 || nrhs == 8 || nrhs == 16,
+               "nrhs must be 1, 2, 4, 8 or 16");
+  PQ3D_REQUIRE(nt <= kScratchDoubles && npart <= kScratchDoubles &&
+                   (nt + npart) * nrhs <= kScratchDoubles,
+               "the product needs (nt + npart) * nrhs = %lld doubles of scratch; the "
+               "no-allocation matvec holds at most %lld (use pq3d_hmat_matvec with caller "
+               "buffers)",
+               (long long)((nt + npart) * nrhs), (long long)kScratchDoubles);
+  double* base = nullptr;
+  const int rc = scratch_of_current_device(&base);
+  if (rc != PQ3D_OK) return rc;
+  return pq3d_hmat_matvec(d_units, nunits, d_members, d_red, nred, d_tiles, d_tile_ptr, ntiles,
+                          d_items, d_pool, dtype_bytes, nrhs, d_v1, d_v2, base,
+                          base + nt * nrhs, d_y1, d_y2, d_y3, y_bstride, stream);
 }
 
 // pq3d_hmat_matvec for FP32 storage and 16 right-hand sides with every block

@@ -35,3 +35,17 @@ def test_binding_covers_header():
 def test_library_is_gfx950_code_object(pq3d_lib):
     data = _lib.LIB_PATH.read_bytes()
     assert b"gfx950" in data
+
+
+def test_noalloc_matvec_size_limit_is_checked_before_any_device_call(pq3d_lib):
+    """A product needing more than the static scratch (8 Mi doubles) is
+    rejected with a clear message before the library touches a device (so
+    the check runs here, without a GPU)."""
+    import pytest
+    args = [None, 1, None, None, 0, None, None, 1, None, None, 4, 16, None, None,
+            (8 << 20) // 16, 1, None, None, None, 0, None]
+    with pytest.raises(RuntimeError, match="holds at most 8388608"):
+        _lib.call("pq3d_hmat_matvec_noalloc", *args)
+    args[14], args[15] = -1, 0
+    with pytest.raises(RuntimeError, match="must be >= 0"):
+        _lib.call("pq3d_hmat_matvec_noalloc", *args)
