@@ -49,3 +49,7 @@ def test_noalloc_matvec_size_limit_is_checked_before_any_device_call(pq3d_lib):
     args[14], args[15] = -1, 0
     with pytest.raises(RuntimeError, match="must be >= 0"):
         _lib.call("pq3d_hmat_matvec_noalloc", *args)
+
+
+def test_controller_record_widths(pq3d_lib):
+    assert _lib.controller_widths() == {"run": 64, "lstate": 16, "sample": 8}
