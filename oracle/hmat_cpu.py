@@ -6,6 +6,11 @@ loop over leaf blocks with numpy FP64 products (dense ``Mf @ x`` or
 ``U @ (V @ x)``) scattered into y, over the same blocks and factors the GPU
 build produced (copied to host FP64), followed by the oracle RK45 step
 (oracle/qdsim_oracle.py).
+
+Pinned by tests/test_oracle_hmat_cpu.py: on pools laid out as the GPU build
+lays them out, an all-dense partition equals the dense FP64 oracle product
+to 1e-12, SVD-truncated far-field blocks stay within their truncation, and
+the two-plane product gives the first two rows.
 """
 from __future__ import annotations
 

@@ -20,7 +20,8 @@ reference's dictionary layout (U_ACA_A1s (m x k), V_ACA_A1s (k x n), ...),
 where the strike (S) and dip (D) factors share rows of one stacked
 factorisation. blocks_process_MVM accepts either a list of processed Blocks
 (numpy FP64 products, as the reference) or an ``HMatrix`` (device product).
-The half-space option is not provided and raises.
+``halfspace_jud`` selects the half-space Green's functions (free surface at
+z = 0), as the reference's flag does, for the evaluators and ACA_worker.
 """
 from __future__ import annotations
 
