@@ -164,22 +164,63 @@ def test_invalid_ranges_raise(gpu, pq3d_lib):
         dm.traction_block([0, 1, 2], [0, 1], out=out)
 
 
-def test_assembly_n20000_sampled_columns(gpu, pq3d_lib, oracle):
+def _pair_errors(got, ref, row_scale):
+    """Per (receiver, source) pair: max over the six operators of |got - ref|
+    divided by max(max over the six of |ref|, 1e-6 * the receiver's scale)."""
+    d = np.abs(got - ref).max(axis=0)
+    r = np.maximum(np.abs(ref).max(axis=0), 1e-6 * row_scale[:, None])
+    return d / r
+
+
+def test_assembly_n20000_sampled_columns_and_near_field(gpu, pq3d_lib, oracle):
     """Full N x N assembly of the undulating 100 x 25 km fault, N = 20,000
-    (6 operators, 19.2 GB FP64 in HBM), checked against the C oracle on all
-    receivers for a sample of source columns (self terms included):
-    1e-10 per receiver row."""
+    (6 operators, 19.2 GB FP64 in HBM), against the C oracle (bitwise equal
+    to the reference C++ on the golden vectors) on (a) all receivers for 1 %
+    of the source columns (self terms included) and (b) every near-field
+    pair: the non-admissible blocks of the leaf-64, eta-2 partition
+    (6.2 M pairs). Per pair: max over the six traction operators of the
+    difference <= 1e-9 x max(largest of the pair's six values, 1e-6 x the
+    receiver's scale), the scale being the median over the sampled columns
+    of the receiver's largest operator value. (Stresses are held to 1e-10
+    by the golden tests above; the projection onto the receiver frame can
+    cancel, so a pair's tractions can be smaller than its stresses and carry
+    their rounding: the worst pair measured 1.35e-10.)"""
+    from pyquake3d_amd.cluster import build_block_partition, build_cluster_tree
     mesh = undulating_fault(length=100e3, depth=25e3, nx=200, nz=50, amplitude=300.0,
                             wavelength=5e3, seed=0)
     dm = tde.DeviceMesh.from_mesh(mesh, MU, LAM, device=gpu)
     N = mesh.n
     out = dm.traction_block()
-    rng = np.random.default_rng(20)
-    cols = np.sort(np.concatenate([rng.choice(N, 14, replace=False), [0, N - 1]]))
-    got = out[:, :, torch.from_numpy(cols).to(gpu)].cpu().numpy()
-    del out
-    torch.cuda.empty_cache()
     ev, xg = get_eleVec(mesh.nodelst, mesh.elelst)
-    ref = oracle.traction_block(xg, ev, mesh.triangles(), np.arange(N), cols, MU, LAM)
-    scale = np.abs(ref).max(axis=(0, 2))[None, :, None]
-    assert (np.abs(got - ref) / scale).max() < TOL
+    tri = mesh.triangles()
+    rng = np.random.default_rng(20)
+    cols = np.sort(np.unique(np.concatenate([rng.choice(N, N // 100, replace=False),
+                                              [0, N - 1]])))
+    got = out[:, :, torch.from_numpy(cols).to(gpu)].cpu().numpy()
+    ref = oracle.traction_block(xg, ev, tri, np.arange(N), cols, MU, LAM)
+    row_scale = np.median(np.abs(ref).max(axis=0), axis=1)
+    e_cols = _pair_errors(got, ref, row_scale)
+    print(f"sampled columns: worst pair {e_cols.max():.3e}")
+    assert np.isfinite(got).all() and e_cols.max() < 1e-9, e_cols.max()
+    # the self terms are in the sample
+    assert (e_cols[cols, np.arange(len(cols))] < 1e-9).all()
+    # near field, one row leaf at a time
+    t = build_cluster_tree(xg, leaf=64)
+    P = build_block_partition(t, eta=2.0)
+    near = {}
+    for a, b in zip(P.rows[~P.adm], P.cols[~P.adm]):
+        near.setdefault(int(a), []).append(int(b))
+    pairs, worst = 0, 0.0
+    for a, bs in near.items():
+        rows = t.perm[t.start[a]:t.start[a] + t.size[a]]
+        cc = np.concatenate([t.perm[t.start[b]:t.start[b] + t.size[b]] for b in bs])
+        r_t = torch.from_numpy(rows).to(gpu)
+        c_t = torch.from_numpy(cc).to(gpu)
+        g = out[:, r_t][:, :, c_t].cpu().numpy()
+        rf = oracle.traction_block(xg, ev, tri, rows, cc, MU, LAM)
+        e = _pair_errors(g, rf, row_scale[rows])
+        worst = max(worst, float(e.max()))
+        pairs += e.size
+    print(f"near field: {pairs} pairs, worst {worst:.3e}")
+    assert pairs > 5_000_000
+    assert worst < 1e-9, worst
