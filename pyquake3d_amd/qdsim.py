@@ -32,49 +32,59 @@ class StepAborted(RuntimeError):
 
 
 class DenseOperator:
-    """FP64 dense six-operator matrix (the reference's plain method):
-    [[A1s, A1d], [A2s, A2d], [Bs, Bd]] (3N x 2N), applied with one GEMV."""
+    """The reference's plain method: the six traction operators stored densely
+    in FP64 (the dense-block products of blocks_process_MVM,
+    src/Hmatrix.py:1445-1538, applied as in src/QDsim.py:906-917), on this
+    library's block matvec. It is an ``HMatrix`` with admissibility eta = 0,
+    so every leaf-cluster pair is a near-field block assembled densely on the
+    device and no block is compressed; like any HMatrix it works in cluster
+    order (``perm``)."""
 
     def __init__(self, mesh, mu, lambda_, device="cuda", dtype=torch.float64, halfspace=False,
-                 jud_ele_order=False):
-        from .tde import DeviceMesh
-        dm = DeviceMesh.from_mesh(mesh, mu, lambda_, device=device, halfspace=halfspace,
-                                  jud_ele_order=jud_ele_order)
-        K = dm.traction_block()
-        n = dm.n
-        self.n = n
-        self.device = dm.device
-        self.M = torch.empty((3 * n, 2 * n), dtype=dtype, device=self.device)
-        for c in range(3):
-            self.M[c * n:(c + 1) * n, :n] = K[c]
-            self.M[c * n:(c + 1) * n, n:] = K[3 + c]
-        del K
-        self.perm = np.arange(n)
-        self.xg = dm.xg
-        self.eleVec = dm.eleVec
-        self._x = torch.empty(2 * n, dtype=dtype, device=self.device)
-        self.matvec_bytes = self.M.numel() * self.M.element_size() + 5 * 8 * n
+                 jud_ele_order=False, leaf=64):
+        from .hmatrix import HMatrix
+        self.mesh, self.mu, self.lambda_ = mesh, float(mu), float(lambda_)
+        self.halfspace, self.jud_ele_order = bool(halfspace), bool(jud_ele_order)
+        self.H = HMatrix.from_mesh(mesh, mu, lambda_, leaf=leaf, eta=0.0, dtype=dtype,
+                                   device=device, halfspace=halfspace,
+                                   jud_ele_order=jud_ele_order)
+        if self.H.stats.n_lowrank:
+            raise RuntimeError("an eta = 0 partition has no admissible block")
+        self.n = self.H.n
+        self.perm = self.H.perm
+        self.device = self.H.device
+        self.dtype = dtype
+        self.matvec_bytes = self.H.matvec_bytes
+
+    @property
+    def accumulate(self):
+        return self.H.accumulate
+
+    def _bufs(self, nrhs):
+        return self.H._bufs(nrhs)
 
     def matvec(self, v1, v2, out=None, stream=None, nrhs=1, instance_major=False, planes=3):
-        """nrhs > 1: v1, v2 rows interleaved by instance (cell*nrhs + b); out
-        in the same layout, or instance-major (b*n + cell). ``planes`` is
-        accepted for the H-matrix's interface; all three rows are formed."""
+        """As ``HMatrix.matvec`` (cluster order)."""
+        return self.H.matvec(v1, v2, out=out, stream=stream, nrhs=nrhs,
+                             instance_major=instance_major, planes=planes)
+
+    def algorithmic_bytes(self, nrhs=1, planes=3):
+        return self.H.algorithmic_bytes(nrhs, planes)
+
+    def matrix(self) -> torch.Tensor:
+        """[[A1s, A1d], [A2s, A2d], [Bs, Bd]] as one (3N, 2N) FP64 device
+        matrix in the original element order, assembled by the traction-block
+        kernel (for checks against host restatements of the step)."""
+        from .tde import DeviceMesh
+        dm = DeviceMesh.from_mesh(self.mesh, self.mu, self.lambda_, device=self.device,
+                                  halfspace=self.halfspace, jud_ele_order=self.jud_ele_order)
+        K = dm.traction_block()
         n = self.n
-        if out is None:
-            out = torch.empty((3, n * nrhs), dtype=torch.float64, device=self.device)
-        if nrhs == 1:
-            self._x[:n] = v1
-            self._x[n:] = v2
-            y = torch.mv(self.M, self._x.to(self.M.dtype))
-            out.view(-1).copy_(y)
-        else:
-            X = torch.cat([v1.view(n, nrhs), v2.view(n, nrhs)]).to(self.M.dtype)
-            Y = (self.M @ X).view(3, n, nrhs)
-            if instance_major:
-                out.view(3, nrhs, n).copy_(Y.transpose(1, 2))
-            else:
-                out.view(3, n, nrhs).copy_(Y)
-        return out
+        M = torch.empty((3 * n, 2 * n), dtype=torch.float64, device=self.device)
+        for c in range(3):
+            M[c * n:(c + 1) * n, :n] = K[c]
+            M[c * n:(c + 1) * n, n:] = K[3 + c]
+        return M
 
 
 class QDsim:

@@ -26,7 +26,7 @@ def test_bp5_example_steps_match_oracle(gpu, pq3d_lib):
     p, hs = load_problem(ARCHIVE)
     assert hs and p.n == 9250
     op = DenseOperator(p.mesh, p.mu, p.lambda_, device=gpu, halfspace=hs)
-    M = op.M.cpu().numpy()
+    M = op.matrix().cpu().numpy()
     n = p.n
 
     def mv(v1, v2):
