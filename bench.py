@@ -549,10 +549,11 @@ def bench_learned(args):
     log(f"learned: trained ({json.dumps(fit)}), written to {out}")
     draws = c3_draws(args.nb)
     H, _ = build_hmatrix(draws[0], torch.float32)
+    H.accumulate = args.accumulate
     plain, _ = run_cycles(draws, H, "plain_controller", args.cycle_years, args.warmup,
                           max_wall=args.max_wall)
     learned, _ = run_cycles(draws, H, "learned_controller", args.cycle_years, args.warmup,
-                            mlp=mlp, max_wall=args.max_wall)
+                            mlp=mlp, max_wall=args.max_wall, margin=LEARNED_MARGIN)
     acc = event_errors(learned, plain)
     return {
         "metric": "wall-time ratio over ten seismic cycles, reference / learned step-size "
@@ -561,13 +562,14 @@ def bench_learned(args):
         "n_gpus": 1, "steps": learned["iterations"], "warmup": args.warmup,
         "ms_per_step": learned["ms_per_iteration"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 operator storage, f64 state; f32 network",
+        "dtype": f"f32 operator storage ({args.accumulate} products), f64 state; f32 network",
         "data": "synthetic undulating fault meshes and seeded friction fields",
         "config": {"workload": f"learned step size: trained on {len(T)} rows of the reference "
                                f"controller ({args.nb} draws, N=20000, {args.record} "
                                f"iterations), evaluated over [0, {args.cycle_years:g}] yr of "
                                f"{args.nb} draws at N={draws[0].n}, rtol 1e-4, FP32 H-matrix, "
-                               f"device controller, 8 attempt rounds per iteration"},
+                               f"device controller, 8 attempt rounds per iteration, proposal "
+                               f"margin {LEARNED_MARGIN}"},
         "rejected_fraction": {"reference": plain["rejected_fraction"],
                               "learned": learned["rejected_fraction"]},
         "accepted_steps": {"reference": int(np.sum(plain["accepted_steps"])),
@@ -1061,7 +1063,7 @@ def main():
     if args.accumulate is None:
         # the headline's batched products run on the matrix cores; the other
         # workloads default to FP64 sums
-        args.accumulate = "mfma" if args.workload == "hstep" else "f64"
+        args.accumulate = "mfma" if args.workload in ("hstep", "learned") else "f64"
     if args.workload == "hstep":
         # the timed region is the whole simulated interval; --steps is ignored
         args.warmup = 20 if args.warmup is None else args.warmup

@@ -130,6 +130,14 @@ __device__ inline ArgMax block_argmax(ArgMax a, double* sv, int64_t* si) {
   return r;
 }
 
+// Sum over the 64 lanes of a wavefront (every lane gets the sum; the
+// butterfly order is fixed, so the result is deterministic).
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __device__ inline double block_sum(double v, double* sv) {
   const int t = threadIdx.x;
   sv[t] = v;
@@ -160,6 +168,7 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
     uint8_t* __restrict__ flags, int64_t* __restrict__ out_rank) {
   __shared__ double sv[kAcaThreads];
   __shared__ int64_t si[kAcaThreads];
+  __shared__ double sx[kAcaThreads];
   const int tid = threadIdx.x;
   const int64_t bid = blockIdx.x;
   const int64_t* b = tab + bid * LB_W;
@@ -258,14 +267,27 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
     for (int64_t c = tid; c < C; c += kAcaThreads) pv += vk[c] * vk[c];
     const double nu2 = block_sum(pu, sv);
     const double nv2 = block_sum(pv, sv);
+    // cross terms sum_l (U_l . u_k)(V_l . v_k) of the Frobenius estimate:
+    // each wavefront forms whole dot products for l = wave, wave + 4, ...
+    // (wave-level reductions, no workgroup barrier), one slot per l in LDS;
+    // the slots of a chunk of kAcaThreads ranks are then added in order of l
+    // by every thread. A rank step needs two barriers per chunk instead of
+    // two workgroup reductions per previous rank.
     double cross = 0.0;
-    for (int l = 0; l < k; ++l) {
-      double a = 0.0, c2 = 0.0;
-      for (int64_t r = tid; r < R; r += kAcaThreads) a += U[(int64_t)l * R + r] * uk[r];
-      for (int64_t c = tid; c < C; c += kAcaThreads) c2 += V[(int64_t)l * C + c] * vk[c];
-      const double ua = block_sum(a, sv);
-      const double vb = block_sum(c2, sv);
-      cross += ua * vb;
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int l0 = 0; l0 < k; l0 += kAcaThreads) {
+      const int lend = (k - l0 < kAcaThreads) ? k : l0 + kAcaThreads;
+      for (int l = l0 + wave; l < lend; l += kAcaThreads / 64) {
+        double a = 0.0, c2 = 0.0;
+        for (int64_t r = lane; r < R; r += 64) a += U[(int64_t)l * R + r] * uk[r];
+        for (int64_t c = lane; c < C; c += 64) c2 += V[(int64_t)l * C + c] * vk[c];
+        a = wave_sum_d(a);
+        c2 = wave_sum_d(c2);
+        if (lane == 0) sx[l - l0] = a * c2;
+      }
+      __syncthreads();
+      for (int l = 0; l < lend - l0; ++l) cross += sx[l];
+      __syncthreads();
     }
     normS2 += 2.0 * cross + nu2 * nv2;
     ++k;
