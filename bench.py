@@ -78,6 +78,14 @@ MEASURED_TRAFFIC = {("f32", 16, 2, "mfma"): 5_349_731_072}
 # + 2 x (129 global rotation + 20 Hooke + 30 projection) + 24 (summing the
 # three angular dislocations); divisions and square roots count as one.
 FLOPS_PER_PAIR = 1571.0
+# What the GPU executes per pair of the N = 20,000 assembly, from the
+# SQ_INSTS_VALU and SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 counters of one
+# launch (x 64 lanes / 4e8 pairs; an FMA counts 2): full space 3,309 VALU
+# instructions and 3,830 FP64 operations per pair (the correctly rounded
+# divisions and square roots expand into ~10 instructions each), half space
+# 23,919 and 24,526 (profiles/tde_traction_block_N20000.md).
+VALU_INSTS_PER_PAIR = {False: 3309.0, True: 23919.0}
+FP64_OPS_PER_PAIR = {False: 3830.0, True: 24526.0}
 # displacement of one (point, triangle) pair, counted per operation in the
 # device source (+ - * / sqrt log atan atan2 each 1): full space 570 (three
 # angular dislocations 3 x 127, their frame rotations 3 x 25, point transform
@@ -880,25 +888,25 @@ def bench_assembly(args):
     pairs = float(N) * N
     sample = out[:, :64, :64].cpu().numpy()
     assert np.isfinite(sample).all()
+    # VALU issue bound: one wave64 instruction occupies a SIMD (16 lanes) for
+    # 4 cycles; 1,024 SIMDs at 2.4 GHz
+    issue_ms = pairs * VALU_INSTS_PER_PAIR[hs] / 64.0 * 4.0 / 1024 / 2.4e9 * 1e3
+    executed = pairs * FP64_OPS_PER_PAIR[hs] / (kern_ms * 1e-3) / 1e12
+    roof = {"bound": "fp64-valu", "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+            "traffic": None, "kernel_ms": kern_ms,
+            "executed_fp64_tflops": executed, "executed_frac": executed / MI355X_FP64_TFLOPS,
+            "valu_issue_bound_ms": issue_ms, "valu_issue_frac": issue_ms / kern_ms}
     if hs:
-        res = {
-            "metric": "TDE evaluations/s (N x N assembly, FP64, half space)",
-            "value": 2.0 * pairs * args.steps / wall, "unit": "evaluations/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": wall * 1e3 / args.steps,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-            "data": "synthetic undulating fault mesh, seed 0",
-            "config": {"workload": "kernel assembly, undulating fault 100x25 km, N=20000, unit "
-                                   "Ss and Ds, six traction operators, FP64, half space "
-                                   "(free surface z=0)"},
-            "roofline": {"bound": "fp64-valu", "achieved": None, "peak": MI355X_FP64_TFLOPS,
-                         "unit": "TFLOP/s", "frac": None, "traffic": None,
-                         "kernel_ms": kern_ms},
-        }
-        return res
-    achieved = pairs * FLOPS_PER_PAIR / (kern_ms * 1e-3) / 1e12
+        # no operation-by-operation count of the half-space pair: the
+        # hardware-counted FP64 operations stand in for the algorithmic ones
+        roof.update(achieved=executed, frac=executed / MI355X_FP64_TFLOPS,
+                    achieved_basis="hardware-counted FP64 operations per pair")
+    else:
+        achieved = pairs * FLOPS_PER_PAIR / (kern_ms * 1e-3) / 1e12
+        roof.update(achieved=achieved, frac=achieved / MI355X_FP64_TFLOPS,
+                    achieved_basis="algorithmic FP64 operations per pair (division = 1)")
     res = {
-        "metric": "TDE evaluations/s (N x N assembly, FP64)",
+        "metric": "TDE evaluations/s (N x N assembly, FP64" + (", half space)" if hs else ")"),
         "value": 2.0 * pairs * args.steps / wall,
         "unit": "evaluations/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -906,11 +914,12 @@ def bench_assembly(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic undulating fault mesh, seed 0",
         "config": {"workload": "kernel assembly, undulating fault 100x25 km, N=20000, unit Ss "
-                               "and Ds, six traction operators, FP64"},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": MI355X_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_TFLOPS, "traffic": None,
-                     "kernel_ms": kern_ms},
+                               "and Ds, six traction operators, FP64"
+                               + (", half space (free surface z=0)" if hs else "")},
+        "roofline": roof,
     }
+    if hs:
+        return res
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline_assembly(mesh, dm)
     return res

@@ -73,30 +73,40 @@ __device__ __forceinline__ void pair_traction(const double* r, const double* __r
 
 // ------------------------------------------------------ near-field assembly
 // One workgroup per dense block; lane <-> receiver row (column-major writes).
+// The source records of kDaCols columns at a time are staged in LDS (row
+// stride kSrcRec + 1 doubles) and read from there by the pair evaluation, so
+// they take no registers: the full-space kernel runs two waves per SIMD.
+constexpr int kDaCols = 64;
 template <typename T, bool HS>
-__global__ void __launch_bounds__(256) k_dense_assemble(const int64_t* __restrict__ tab,
-                                                        const double* __restrict__ recv,
-                                                        const double* __restrict__ rec,
-                                                        const double* __restrict__ hrec,
-                                                        double mu, double lam, double nu,
-                                                        T* __restrict__ pool) {
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, 256),
+                               amdgpu_waves_per_eu(HS ? 1 : 2)))
+k_dense_assemble(const int64_t* __restrict__ tab, const double* __restrict__ recv,
+                 const double* __restrict__ rec, const double* __restrict__ hrec, double mu,
+                 double lam, double nu, T* __restrict__ pool) {
+  __shared__ double srec[kDaCols * (kSrcRec + 1)];
   const int64_t* b = tab + (int64_t)blockIdx.x * DB_W;
   const int64_t r0 = b[DB_R0], m = b[DB_M], c0 = b[DB_C0], n = b[DB_N], base = b[DB_BASE];
   const int64_t mn = m * n;
-  for (int64_t p = threadIdx.x; p < mn; p += blockDim.x) {
-    const int64_t i = p % m, j = p / m;
-    double r[kSrcRec];
-    load_rec_g(rec + kSrcRec * (c0 + j), r);
-    const double* q = recv + kRecvW * (r0 + i);
-    double ts[3], td[3];
-    pair_traction<HS>(r, hrec, c0 + j, q, mu, lam, nu, ts, td);
-    T* o = pool + base + j * m + i;
-    o[0 * mn] = (T)ts[0];
-    o[1 * mn] = (T)ts[1];
-    o[2 * mn] = (T)ts[2];
-    o[3 * mn] = (T)td[0];
-    o[4 * mn] = (T)td[1];
-    o[5 * mn] = (T)td[2];
+  for (int64_t jc = 0; jc < n; jc += kDaCols) {
+    const int64_t cw = (n - jc < kDaCols) ? n - jc : kDaCols;
+    __syncthreads();  // the previous chunk's readers are done
+    for (int64_t e = threadIdx.x; e < cw * kSrcRec; e += blockDim.x)
+      srec[(e / kSrcRec) * (kSrcRec + 1) + e % kSrcRec] = rec[kSrcRec * (c0 + jc) + e];
+    __syncthreads();
+    const int64_t mc = m * cw;
+    for (int64_t p = threadIdx.x; p < mc; p += blockDim.x) {
+      const int64_t i = p % m, jl = p / m, j = jc + jl;
+      const double* q = recv + kRecvW * (r0 + i);
+      double ts[3], td[3];
+      pair_traction<HS>(srec + jl * (kSrcRec + 1), hrec, c0 + j, q, mu, lam, nu, ts, td);
+      T* o = pool + base + j * m + i;
+      o[0 * mn] = (T)ts[0];
+      o[1 * mn] = (T)ts[1];
+      o[2 * mn] = (T)ts[2];
+      o[3 * mn] = (T)td[0];
+      o[4 * mn] = (T)td[1];
+      o[5 * mn] = (T)td[2];
+    }
   }
 }
 

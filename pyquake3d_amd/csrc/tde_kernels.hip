@@ -104,18 +104,26 @@ constexpr int kTbRows = 8;    // receivers per thread
 // thread walks kTbRows receivers whose 12 doubles (point + frame) are uniform
 // across the workgroup. Output: out[op * plane + i * si + j * sj], op order
 // A1s, A2s, Bs, A1d, A2d, Bd. rows/cols may be null (identity + offset).
+// Each lane's 32-double source record sits in LDS (row stride kSrcRec + 1
+// doubles, so the 64 lanes of a read hit distinct banks) instead of 64
+// VGPRs: the full-space kernel then fits 256 registers and runs two waves
+// per SIMD (52.3 -> 44.9 ms for N = 20,000). The half-space kernel stays at
+// one wave per SIMD: capped at 256 registers it spills ~300 VGPRs.
 template <typename OutT, bool HS>
-__global__ void __launch_bounds__(256) k_traction_block(
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, kTbCols),
+                               amdgpu_waves_per_eu(HS ? 1 : 2))) k_traction_block(
     const double* __restrict__ recv, const double* __restrict__ rec,
     const double* __restrict__ hrec,
     const int64_t* __restrict__ rows, int64_t row0, int64_t m, const int64_t* __restrict__ cols,
     int64_t col0, int64_t n, double mu, double lam, double nu, OutT* __restrict__ out, int64_t si,
     int64_t sj, int64_t plane) {
+  __shared__ double recs[kTbCols * (kSrcRec + 1)];
   const int64_t j = (int64_t)blockIdx.x * kTbCols + threadIdx.x;
   const int64_t ib = (int64_t)blockIdx.y * kTbRows;
   if (j >= n) return;
   const int64_t src = cols ? cols[j] : col0 + j;
-  double r[kSrcRec];
+  // only this lane reads its slot: no barrier
+  double* r = recs + threadIdx.x * (kSrcRec + 1);
   load_rec(rec + (int64_t)kSrcRec * src, r);
   const int64_t iend = (ib + kTbRows < m) ? ib + kTbRows : m;
   for (int64_t i = ib; i < iend; ++i) {
