@@ -206,6 +206,19 @@ int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
                   double lambda_, double eps, int kmax, double* d_wU, double* d_wV,
                   uint8_t* d_flags, int64_t* d_rank, void* stream);
 
+/* The same ACA for blocks too large for one workgroup (same arguments and
+ * results as pq3d_hmat_aca; the factors and ranks are bitwise equal): every
+ * rank step is a few launches over all the blocks, with the row and column
+ * residuals spread over workgroups of 256 columns / receivers. Returns after
+ * every block has finished (the host checks every 8 steps). Scratch from the
+ * caller: d_istate nblocks*8 + 1 int64, d_dstate nblocks*(kmax + 1) doubles.
+ * nblocks < 65536. */
+int pq3d_hmat_aca_wide(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
+                       const double* d_recv, const double* d_rec, const double* d_hrec,
+                       double mu, double lambda_, double eps, int kmax, double* d_wU,
+                       double* d_wV, uint8_t* d_flags, int64_t* d_rank, int64_t* d_istate,
+                       double* d_dstate, void* stream);
+
 /* Copy converged factors into the pool (the storage half of ACA_worker,
  * src/Hmatrix.py:1014-1066): U (3m x k) at d_tab[b][4] in one panel per leaf
  * cluster of the block's rows -- the leaf with row offset ro and ls rows in

@@ -77,6 +77,9 @@ SIGNATURES = {
     "pq3d_hmat_aca": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                               c_double, c_double, c_double, c_int, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p]),
+    "pq3d_hmat_aca_wide": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                   c_double, c_double, c_double, c_int, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pq3d_hmat_aca_pack": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_int, c_void_p]),
     "pq3d_hmat_matvec": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p,

@@ -18,7 +18,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <mutex>
+#include <vector>
 
 #include "pq3d_common.h"
 #include "tde_device.h"
@@ -308,6 +310,265 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
     rstar = nr.i;
   }
   if (tid == 0) out_rank[bid] = converged ? rank : -1;
+}
+
+// ------------------------------------------------------------ wide ACA
+// The same partially pivoted ACA as k_aca for blocks too large for one
+// workgroup (at N = 100,352 the 106 blocks with m + n >= 4,096 took 0.98 s
+// of a 1.27 s build on a few CUs while the rest of the GPU idled). Each rank
+// step is five launches over all wide blocks at once: the row residual and
+// the column residual spread over workgroups of 256 columns / receivers, the
+// cross terms one wavefront per previous rank, and two one-workgroup-per-block
+// steps (pivot, norms) that reduce in exactly k_aca's order, so the factors
+// and ranks equal k_aca's bit for bit. Block state (int64, kAwW per block):
+enum { AW_RSTAR = 0, AW_K = 1, AW_ZR = 2, AW_STATUS = 3, AW_SKIP = 4, AW_JSTAR = 5, AW_W = 8 };
+// status: 0 running, 1 converged (rank = k), 2 rank cap reached (rank -1);
+// skip: the pivot row of this step was numerically zero (no column step).
+// Double state per block: normS2, then the kmax cross-term products.
+constexpr int kAwCols = 256;
+
+__device__ inline int aw_kcap(const int64_t* b, int kmax) {
+  return (int)((b[LB_K] < kmax) ? b[LB_K] : kmax);
+}
+
+__device__ inline void aw_finish(int64_t* st, int status, int64_t* out_rank, int64_t bid,
+                                 unsigned long long* done) {
+  st[AW_STATUS] = status;
+  out_rank[bid] = (status == 1) ? st[AW_K] : -1;
+  atomicAdd(done, 1ULL);
+}
+
+__global__ void __launch_bounds__(256) k_acaw_init(const int64_t* __restrict__ tab,
+                                                   const int64_t* __restrict__ wsoff, int kmax,
+                                                   uint8_t* __restrict__ flags,
+                                                   int64_t* __restrict__ ist,
+                                                   double* __restrict__ dst, int64_t* out_rank,
+                                                   unsigned long long* done) {
+  const int64_t bid = blockIdx.x;
+  const int64_t* b = tab + bid * LB_W;
+  const int64_t R = 3 * b[LB_M], C = 2 * b[LB_N];
+  uint8_t* urow = flags + wsoff[WS_W * bid + WS_F];
+  for (int64_t r = threadIdx.x; r < R + C; r += blockDim.x) urow[r] = 0;
+  if (threadIdx.x == 0) {
+    int64_t* st = ist + AW_W * bid;
+    for (int q = 0; q < AW_W; ++q) st[q] = 0;
+    dst[(int64_t)(1 + kmax) * bid] = 0.0;
+    if (aw_kcap(b, kmax) < 1) aw_finish(st, 2, out_rank, bid, done);
+  }
+}
+
+// Row residual of every running block: lane <-> stacked column pair (j, n+j).
+template <bool HS>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, kAwCols),
+                               amdgpu_waves_per_eu(HS ? 1 : 2)))
+k_acaw_row(const int64_t* __restrict__ tab, const int64_t* __restrict__ wsoff,
+           const double* __restrict__ recv, const double* __restrict__ rec,
+           const double* __restrict__ hrec, double mu, double lam, double nu,
+           const double* __restrict__ wU, double* __restrict__ wV,
+           const int64_t* __restrict__ ist) {
+  __shared__ double recs[kAwCols * (kSrcRec + 1)];
+  const int64_t bid = blockIdx.y;
+  const int64_t* st = ist + AW_W * bid;
+  if (st[AW_STATUS] != 0) return;
+  const int64_t* b = tab + bid * LB_W;
+  const int64_t r0 = b[LB_R0], m = b[LB_M], c0 = b[LB_C0], n = b[LB_N];
+  const int64_t j = (int64_t)blockIdx.x * kAwCols + threadIdx.x;
+  if (j >= n) return;
+  const int64_t R = 3 * m, C = 2 * n;
+  const int64_t rstar = st[AW_RSTAR];
+  const int k = (int)st[AW_K];
+  const int cs = (int)(rstar / m);
+  const int64_t is = rstar % m;
+  const double* U = wU + wsoff[WS_W * bid + WS_U];
+  double* V = wV + wsoff[WS_W * bid + WS_V];
+  double* r = recs + threadIdx.x * (kSrcRec + 1);  // this lane's slot only
+  load_rec_g(rec + kSrcRec * (c0 + j), r);
+  double ts[3], td[3];
+  pair_traction<HS>(r, hrec, c0 + j, recv + kRecvW * (r0 + is), mu, lam, nu, ts, td);
+  double a = ts[cs], d = td[cs];
+  for (int l = 0; l < k; ++l) {
+    const double ul = U[(int64_t)l * R + rstar];
+    a -= ul * V[(int64_t)l * C + j];
+    d -= ul * V[(int64_t)l * C + n + j];
+  }
+  double* vk = V + (int64_t)k * C;
+  vk[j] = a;
+  vk[n + j] = d;
+}
+
+// Pivot column of the new row (k_aca's scan order and tie rule), or the next
+// unused row when the row is numerically zero; scales the row by the pivot.
+__global__ void __launch_bounds__(kAcaThreads) k_acaw_pivot(
+    const int64_t* __restrict__ tab, const int64_t* __restrict__ wsoff, int kmax,
+    double* __restrict__ wV, uint8_t* __restrict__ flags, int64_t* __restrict__ ist,
+    int64_t* __restrict__ out_rank, unsigned long long* done) {
+  __shared__ double sv[kAcaThreads];
+  __shared__ int64_t si[kAcaThreads];
+  const int tid = threadIdx.x;
+  const int64_t bid = blockIdx.x;
+  int64_t* st = ist + AW_W * bid;
+  if (st[AW_STATUS] != 0) return;
+  const int64_t* b = tab + bid * LB_W;
+  const int64_t m = b[LB_M], n = b[LB_N];
+  const int64_t R = 3 * m, C = 2 * n;
+  const int k = (int)st[AW_K];
+  const int64_t rstar = st[AW_RSTAR];
+  double* vk = wV + wsoff[WS_W * bid + WS_V] + (int64_t)k * C;
+  uint8_t* urow = flags + wsoff[WS_W * bid + WS_F];
+  uint8_t* ucol = urow + R;
+  ArgMax loc{-1.0, -1};
+  for (int64_t j = tid; j < n; j += kAcaThreads) {
+    const double a = vk[j], d = vk[n + j];
+    if (!ucol[j] && fabs(a) > loc.v) { loc.v = fabs(a); loc.i = j; }
+    if (!ucol[n + j] && fabs(d) > loc.v) { loc.v = fabs(d); loc.i = n + j; }
+  }
+  __syncthreads();
+  const ArgMax piv = block_argmax(loc, sv, si);
+  if (tid == 0) urow[rstar] = 1;
+  __syncthreads();
+  if (piv.i < 0 || !(piv.v > 0.0)) {
+    ArgMax nx{-1.0, -1};
+    for (int64_t r = tid; r < R; r += kAcaThreads)
+      if (!urow[r]) { nx.v = 1.0; nx.i = r; break; }
+    if (nx.i >= 0) nx.v = -(double)nx.i;
+    else nx.v = -1e300;
+    const ArgMax nr = block_argmax(nx, sv, si);
+    if (tid == 0) {
+      const int64_t zr = st[AW_ZR] + 1;
+      st[AW_ZR] = zr;
+      if (nr.i < 0 || zr > 3 * 64) {
+        aw_finish(st, 1, out_rank, bid, done);
+      } else {
+        st[AW_RSTAR] = nr.i;
+        st[AW_SKIP] = 1;
+      }
+    }
+    return;
+  }
+  const int64_t jstar = piv.i;
+  const double delta = vk[jstar];
+  __syncthreads();
+  for (int64_t j = tid; j < C; j += kAcaThreads) vk[j] /= delta;
+  if (tid == 0) {
+    ucol[jstar] = 1;
+    st[AW_JSTAR] = jstar;
+  }
+}
+
+// Column residual of every running block: lane <-> receiver (3 stack rows).
+template <bool HS>
+__global__ void __launch_bounds__(kAwCols) k_acaw_col(
+    const int64_t* __restrict__ tab, const int64_t* __restrict__ wsoff,
+    const double* __restrict__ recv, const double* __restrict__ rec,
+    const double* __restrict__ hrec, double mu, double lam, double nu,
+    double* __restrict__ wU, const double* __restrict__ wV, const int64_t* __restrict__ ist) {
+  __shared__ double rs[kSrcRec];
+  const int64_t bid = blockIdx.y;
+  const int64_t* st = ist + AW_W * bid;
+  if (st[AW_STATUS] != 0 || st[AW_SKIP] != 0) return;
+  const int64_t* b = tab + bid * LB_W;
+  const int64_t r0 = b[LB_R0], m = b[LB_M], c0 = b[LB_C0], n = b[LB_N];
+  if ((int64_t)blockIdx.x * kAwCols >= m) return;
+  const int64_t R = 3 * m, C = 2 * n;
+  const int k = (int)st[AW_K];
+  const int64_t jstar = st[AW_JSTAR];
+  const int ss = (int)(jstar / n);
+  const int64_t js = jstar % n;
+  if (threadIdx.x < kSrcRec) rs[threadIdx.x] = rec[kSrcRec * (c0 + js) + threadIdx.x];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kAwCols + threadIdx.x;
+  if (i >= m) return;
+  double* U = wU + wsoff[WS_W * bid + WS_U];
+  const double* V = wV + wsoff[WS_W * bid + WS_V];
+  double* uk = U + (int64_t)k * R;
+  double ts[3], td[3];
+  pair_traction<HS>(rs, hrec, c0 + js, recv + kRecvW * (r0 + i), mu, lam, nu, ts, td);
+  for (int c = 0; c < 3; ++c) {
+    double v = ss ? td[c] : ts[c];
+    const int64_t row = c * m + i;
+    for (int l = 0; l < k; ++l) v -= U[(int64_t)l * R + row] * V[(int64_t)l * C + jstar];
+    uk[row] = v;
+  }
+}
+
+// Cross terms (U_l . u_k)(V_l . v_k) of the Frobenius estimate: one
+// wavefront per previous rank l, in k_aca's lane order and butterfly.
+__global__ void __launch_bounds__(256) k_acaw_cross(const int64_t* __restrict__ tab,
+                                                    const int64_t* __restrict__ wsoff, int kmax,
+                                                    const double* __restrict__ wU,
+                                                    const double* __restrict__ wV,
+                                                    const int64_t* __restrict__ ist,
+                                                    double* __restrict__ dst) {
+  const int64_t bid = blockIdx.y;
+  const int64_t* st = ist + AW_W * bid;
+  if (st[AW_STATUS] != 0 || st[AW_SKIP] != 0) return;
+  const int k = (int)st[AW_K];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int l = blockIdx.x * 4 + wave;
+  if (l >= k) return;
+  const int64_t* b = tab + bid * LB_W;
+  const int64_t R = 3 * b[LB_M], C = 2 * b[LB_N];
+  const double* U = wU + wsoff[WS_W * bid + WS_U];
+  const double* V = wV + wsoff[WS_W * bid + WS_V];
+  const double* uk = U + (int64_t)k * R;
+  const double* vk = V + (int64_t)k * C;
+  double a = 0.0, c2 = 0.0;
+  for (int64_t r = lane; r < R; r += 64) a += U[(int64_t)l * R + r] * uk[r];
+  for (int64_t c = lane; c < C; c += 64) c2 += V[(int64_t)l * C + c] * vk[c];
+  a = wave_sum_d(a);
+  c2 = wave_sum_d(c2);
+  if (lane == 0) dst[(int64_t)(1 + kmax) * bid + 1 + l] = a * c2;
+}
+
+// Norms, Frobenius estimate, convergence and the next pivot row.
+__global__ void __launch_bounds__(kAcaThreads) k_acaw_norm(
+    const int64_t* __restrict__ tab, const int64_t* __restrict__ wsoff, int kmax, double eps,
+    const double* __restrict__ wU, const double* __restrict__ wV,
+    const uint8_t* __restrict__ flags, int64_t* __restrict__ ist, double* __restrict__ dst,
+    int64_t* __restrict__ out_rank, unsigned long long* done) {
+  __shared__ double sv[kAcaThreads];
+  __shared__ int64_t si[kAcaThreads];
+  const int tid = threadIdx.x;
+  const int64_t bid = blockIdx.x;
+  int64_t* st = ist + AW_W * bid;
+  if (st[AW_STATUS] != 0) return;
+  if (st[AW_SKIP] != 0) {
+    __syncthreads();  // every thread has read the flag before it is cleared
+    if (tid == 0) st[AW_SKIP] = 0;
+    return;
+  }
+  const int64_t* b = tab + bid * LB_W;
+  const int64_t R = 3 * b[LB_M], C = 2 * b[LB_N];
+  const int k = (int)st[AW_K];
+  const double* uk = wU + wsoff[WS_W * bid + WS_U] + (int64_t)k * R;
+  const double* vk = wV + wsoff[WS_W * bid + WS_V] + (int64_t)k * C;
+  const uint8_t* urow = flags + wsoff[WS_W * bid + WS_F];
+  double* ds = dst + (int64_t)(1 + kmax) * bid;
+  double pu = 0.0, pv = 0.0;
+  ArgMax nxt{-1.0, -1};
+  for (int64_t r = tid; r < R; r += kAcaThreads) {
+    const double u = uk[r];
+    pu += u * u;
+    if (!urow[r] && fabs(u) > nxt.v) { nxt.v = fabs(u); nxt.i = r; }
+  }
+  for (int64_t c = tid; c < C; c += kAcaThreads) pv += vk[c] * vk[c];
+  const double nu2 = block_sum(pu, sv);
+  const double nv2 = block_sum(pv, sv);
+  double cross = 0.0;
+  for (int l = 0; l < k; ++l) cross += ds[1 + l];
+  const double normS2 = ds[0] + (2.0 * cross + nu2 * nv2);
+  const ArgMax nr = block_argmax(nxt, sv, si);
+  if (tid == 0) {
+    ds[0] = normS2;
+    st[AW_K] = k + 1;
+    if (sqrt(nu2 * nv2) <= eps * sqrt(fabs(normS2)) || nr.i < 0) {
+      aw_finish(st, 1, out_rank, bid, done);
+    } else if (k + 1 >= aw_kcap(b, kmax)) {
+      aw_finish(st, 2, out_rank, bid, done);
+    } else {
+      st[AW_RSTAR] = nr.i;
+    }
+  }
 }
 
 // Copy the converged factors into the storage pool (type T). U goes into one
@@ -1151,6 +1412,75 @@ int pq3d_hmat_aca(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
         d_rank);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
+}
+
+int pq3d_hmat_aca_wide(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,
+                       const double* d_recv, const double* d_rec, const double* d_hrec,
+                       double mu, double lam, double eps, int kmax, double* d_wU, double* d_wV,
+                       uint8_t* d_flags, int64_t* d_rank, int64_t* d_istate, double* d_dstate,
+                       void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(nblocks >= 0, "nblocks must be >= 0");
+  if (nblocks == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_tab && d_wsoff && d_recv && d_rec && d_wU && d_wV && d_flags && d_rank &&
+                   d_istate && d_dstate,
+               "null pointer");
+  PQ3D_REQUIRE(kmax >= 1 && kmax <= 4096, "kmax out of range");
+  PQ3D_REQUIRE(eps > 0.0, "eps must be > 0");
+  PQ3D_REQUIRE(nblocks < 65536, "too many blocks for the wide path (grid.y)");
+  hipStream_t s = as_stream(stream);
+  // block extents on the host: the grids of the row and column steps
+  std::vector<int64_t> tab((size_t)nblocks * LB_W);
+  PQ3D_HIP_CHECK(hipMemcpyAsync(tab.data(), d_tab, tab.size() * sizeof(int64_t),
+                                hipMemcpyDeviceToHost, s));
+  PQ3D_HIP_CHECK(hipStreamSynchronize(s));
+  int64_t mmax = 0, nmax = 0, kc = 0;
+  for (int64_t q = 0; q < nblocks; ++q) {
+    const int64_t* b = tab.data() + q * LB_W;
+    PQ3D_REQUIRE(b[LB_M] >= 1 && b[LB_N] >= 1, "block %lld is empty", (long long)q);
+    mmax = std::max(mmax, b[LB_M]);
+    nmax = std::max(nmax, b[LB_N]);
+    kc = std::max(kc, std::min<int64_t>(b[LB_K], kmax));
+  }
+  const double nu = 1.0 / (1.0 + lam / mu) / 2.0;
+  unsigned long long* done = reinterpret_cast<unsigned long long*>(d_istate + AW_W * nblocks);
+  PQ3D_HIP_CHECK(hipMemsetAsync(done, 0, sizeof(unsigned long long), s));
+  const unsigned nb = (unsigned)nblocks;
+  k_acaw_init<<<nb, 256, 0, s>>>(d_tab, d_wsoff, kmax, d_flags, d_istate, d_dstate, d_rank,
+                                 done);
+  PQ3D_LAUNCH_CHECK();
+  const dim3 grow(blocks_for(nmax, kAwCols), nb), gcol(blocks_for(mmax, kAwCols), nb);
+  const dim3 gcross(blocks_for(kc, 4), nb);
+  // a block takes at most kcap rank steps plus 3*64 + 1 zero-row steps
+  const int64_t max_steps = kc + 3 * 64 + 2;
+  for (int64_t it = 0; it < max_steps; ++it) {
+    if (d_hrec)
+      k_acaw_row<true><<<grow, kAwCols, 0, s>>>(d_tab, d_wsoff, d_recv, d_rec, d_hrec, mu, lam,
+                                                nu, d_wU, d_wV, d_istate);
+    else
+      k_acaw_row<false><<<grow, kAwCols, 0, s>>>(d_tab, d_wsoff, d_recv, d_rec, nullptr, mu,
+                                                 lam, nu, d_wU, d_wV, d_istate);
+    k_acaw_pivot<<<nb, kAcaThreads, 0, s>>>(d_tab, d_wsoff, kmax, d_wV, d_flags, d_istate,
+                                            d_rank, done);
+    if (d_hrec)
+      k_acaw_col<true><<<gcol, kAwCols, 0, s>>>(d_tab, d_wsoff, d_recv, d_rec, d_hrec, mu, lam,
+                                                nu, d_wU, d_wV, d_istate);
+    else
+      k_acaw_col<false><<<gcol, kAwCols, 0, s>>>(d_tab, d_wsoff, d_recv, d_rec, nullptr, mu,
+                                                 lam, nu, d_wU, d_wV, d_istate);
+    k_acaw_cross<<<gcross, 256, 0, s>>>(d_tab, d_wsoff, kmax, d_wU, d_wV, d_istate, d_dstate);
+    k_acaw_norm<<<nb, kAcaThreads, 0, s>>>(d_tab, d_wsoff, kmax, eps, d_wU, d_wV, d_flags,
+                                           d_istate, d_dstate, d_rank, done);
+    PQ3D_LAUNCH_CHECK();
+    if (it % 8 == 7 || it + 1 == max_steps) {
+      unsigned long long h = 0;
+      PQ3D_HIP_CHECK(hipMemcpyAsync(&h, done, sizeof(h), hipMemcpyDeviceToHost, s));
+      PQ3D_HIP_CHECK(hipStreamSynchronize(s));
+      if (h >= (unsigned long long)nblocks) return PQ3D_OK;
+    }
+  }
+  set_error("wide ACA: blocks still running after %lld steps", (long long)max_steps);
+  return PQ3D_EHIP;
 }
 
 int pq3d_hmat_aca_pack(const int64_t* d_tab, const int64_t* d_wsoff, int64_t nblocks,

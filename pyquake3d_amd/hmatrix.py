@@ -315,9 +315,13 @@ def build_worklists(tree, n, r0, m, c0, nn, dense_idx, d_base, lr_idx, l_ub, l_v
 class HMatrix:
     def __init__(self, nodelst, elelst, mu, lambda_, *, leaf=64, eta=2.0, eps=1e-4,
                  dtype=torch.float32, device="cuda", kmax=128, aca_budget_gb=24.0,
-                 jud_ele_order=False, verbose=False, halfspace=False):
+                 jud_ele_order=False, verbose=False, halfspace=False, aca_wide=4096):
         t_all = time.perf_counter()
         self.device = torch.device(device)
+        # admissible blocks with m + n >= aca_wide take the wide ACA (each rank
+        # step spread over many workgroups); the rest one workgroup each. Both
+        # give the same factors bit for bit.
+        self.aca_wide = int(aca_wide)
         self.dtype = dtype
         self.esize = 4 if dtype == torch.float32 else 8
         if dtype not in (torch.float32, torch.float64):
@@ -468,12 +472,25 @@ class HMatrix:
         tab_t = torch.from_numpy(tab).to(dev)
         _lib.check_width("lowrank_block", tab_t)
         _lib.check_width("aca_ws", wsoff)
-        rank = torch.empty(nb, dtype=torch.int64, device=dev)
-        _lib.call("pq3d_hmat_aca", _lib.ptr(tab_t), _lib.ptr(wsoff), nb, _lib.ptr(self.recv),
-                  _lib.ptr(self.rec), _lib.ptr(self.hrec), self.mu, self.lambda_, float(eps),
-                  int(kcap.max()),
-                  _lib.ptr(wsU), _lib.ptr(wsV), _lib.ptr(flags), _lib.ptr(rank), stream)
-        rk = rank.cpu().numpy()
+        rk = np.empty(nb, dtype=np.int64)
+        wide = (mb + nbk) >= self.aca_wide
+        for sel, entry in ((~wide, "pq3d_hmat_aca"), (wide, "pq3d_hmat_aca_wide")):
+            q = np.where(sel)[0]
+            if not len(q):
+                continue
+            qt = torch.from_numpy(q).to(dev)
+            t_q, ws_q = tab_t[qt].contiguous(), wsoff[qt].contiguous()
+            kq = int(kcap[q].max())
+            rank = torch.empty(len(q), dtype=torch.int64, device=dev)
+            args = [_lib.ptr(t_q), _lib.ptr(ws_q), len(q), _lib.ptr(self.recv),
+                    _lib.ptr(self.rec), _lib.ptr(self.hrec), self.mu, self.lambda_, float(eps),
+                    kq, _lib.ptr(wsU), _lib.ptr(wsV), _lib.ptr(flags), _lib.ptr(rank)]
+            if entry == "pq3d_hmat_aca_wide":
+                ist = torch.empty(8 * len(q) + 1, dtype=torch.int64, device=dev)
+                dst = torch.empty(len(q) * (kq + 1), dtype=torch.float64, device=dev)
+                args += [_lib.ptr(ist), _lib.ptr(dst)]
+            _lib.call(entry, *args, stream)
+            rk[q] = rank.cpu().numpy()
         return rk, wsU, wsV, wsoff, blocks
 
     # ------------------------------------------------------------ work lists

@@ -255,3 +255,42 @@ def test_saved_hmatrix_reloads_bit_for_bit(gpu, pq3d_lib, case, tmp_path):
     if H.stats.n_lowrank:
         for a, b in zip(H.lowrank_factors(0), G.lowrank_factors(0)):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("halfspace", [False, True])
+def test_wide_aca_equals_one_workgroup_aca(gpu, pq3d_lib, case, halfspace):
+    """Every admissible block through the wide ACA (rank steps spread over
+    many workgroups) against every block through the one-workgroup ACA: the
+    same ranks and the same stored factors bit for bit, full and half space,
+    FP64 and FP32 storage (with a rank cap of 12 some blocks reach the cap
+    and fall back to dense storage)."""
+    mesh, _ = case
+    for dtype, eps, kmax in ((torch.float64, 1e-7, 128), (torch.float32, 1e-4, 12)):
+        kw = dict(leaf=32, eta=2.0, eps=eps, dtype=dtype, device=gpu, halfspace=halfspace,
+                  kmax=kmax)
+        H1 = HMatrix.from_mesh(mesh, MU, LAM, aca_wide=1 << 62, **kw)
+        Hw = HMatrix.from_mesh(mesh, MU, LAM, aca_wide=0, **kw)
+        if kmax == 128:
+            assert H1.stats.n_lowrank > 0
+        else:
+            assert H1.stats.fallback_dense > 0
+        assert np.array_equal(H1.blocks["ranks"], Hw.blocks["ranks"])
+        assert np.array_equal(H1.blocks["dense_idx"], Hw.blocks["dense_idx"])
+        assert torch.equal(H1.pool, Hw.pool)
+
+
+def test_wide_aca_at_config_size(gpu, pq3d_lib):
+    """N = 100,352 (leaf 64, eta 2, eps 1e-4): the default split (blocks with
+    m + n >= 4,096 wide) stores the same operator as the one-workgroup ACA."""
+    from pyquake3d_amd.problem import undulating_c3
+    p = undulating_c3()
+    kw = dict(leaf=64, eta=2.0, eps=1e-4, dtype=torch.float32, device=gpu)
+    Hd = HMatrix.from_mesh(p.mesh, p.mu, p.lambda_, **kw)
+    pool, ranks, t_default = Hd.pool.cpu(), Hd.blocks["ranks"].copy(), Hd.stats.build_s["aca"]
+    del Hd
+    torch.cuda.empty_cache()
+    H1 = HMatrix.from_mesh(p.mesh, p.mu, p.lambda_, aca_wide=1 << 62, **kw)
+    print(f"ACA: default split {t_default:.3f} s, one workgroup per block "
+          f"{H1.stats.build_s['aca']:.3f} s")
+    assert np.array_equal(H1.blocks["ranks"], ranks)
+    assert torch.equal(H1.pool.cpu(), pool)
