@@ -333,9 +333,14 @@ def bench_hstep(args):
     torch.cuda.empty_cache()
     H32, b32 = build_hmatrix(problem, torch.float32)
     H32.accumulate = args.accumulate
+    tuning = H32.tune(planes=2) if args.accumulate == "mfma" else None
+    if tuning:
+        log(f"hstep: autotuner chose {tuning['choice']} "
+            f"({min(ms for _, ms in tuning['candidates']):.4f} ms)")
     fast, _ = run_cycles(draws, H32, f"fp32_nb{args.nb}" + ("_learned" if mlp else ""), t_end,
                          args.warmup, mlp=mlp, timing_every=20, margin=LEARNED_MARGIN)
     fast["accumulate"] = args.accumulate
+    fast["tuning"] = tuning
     fast["learned_margin"] = LEARNED_MARGIN if mlp else None
     fast["build_s"] = b32
     achieved = fast["matvec_bytes"] / (fast["matvec_ms"] * 1e-3) / 1e9
@@ -389,6 +394,9 @@ def bench_hstep(args):
                      "traffic": MEASURED_TRAFFIC.get(("f32", args.nb, 2, args.accumulate)),
                      "kernel": (f"H-matvec, {args.nb} right-hand sides, two row planes "
                                 f"(k_inputs_f32 + k_hmv_phase1_mf16 + k_hmv_phase1b + "
+                                f"k_hmv_phase2_mfs)" if args.accumulate == "mfma" else
+                                f"H-matvec, {args.nb} right-hand sides, two row planes "
+                                f"(k_inputs_f32 + k_hmv_phase1_mf16 + k_hmv_phase1b + "
                                 f"k_hmv_phase2_mf16)" if args.accumulate.startswith("mfma")
                                 else f"H-matvec, {args.nb} right-hand sides, two row planes "
                                      f"(k_hmv_phase1 + k_hmv_phase1b + k_hmv_phase2)"),
@@ -428,6 +436,8 @@ def bench_matvec(args):
     dtype = torch.float32 if args.dtype == "f32" else torch.float64
     H, build_s = build_hmatrix(problem, dtype)
     H.accumulate = args.accumulate
+    tuning = (H.tune(planes=args.planes) if args.accumulate == "mfma" and args.nb == 16 and
+              args.dtype == "f32" else None)
     n, nb, dev = H.n, args.nb, H.device
     gen = torch.Generator(device=dev)
     gen.manual_seed(0)
@@ -490,6 +500,7 @@ def bench_matvec(args):
         "copy_2GiB_GBs": copy_gbs,
         "build_s": build_s,
         "mfma_rel_err_vs_f64_accumulation": mf_err,
+        "tuning": tuning,
     }
 
 
@@ -558,6 +569,8 @@ def bench_learned(args):
     draws = c3_draws(args.nb)
     H, _ = build_hmatrix(draws[0], torch.float32)
     H.accumulate = args.accumulate
+    if args.accumulate == "mfma":
+        H.tune(planes=2)
     plain, _ = run_cycles(draws, H, "plain_controller", args.cycle_years, args.warmup,
                           max_wall=args.max_wall)
     learned, _ = run_cycles(draws, H, "learned_controller", args.cycle_years, args.warmup,

@@ -286,6 +286,28 @@ int pq3d_hmat_matvec_mfma(const int64_t* d_units, int64_t nunits, const int64_t*
                           double* d_y2, double* d_y3, int64_t y_bstride, float* d_x32,
                           int64_t n, int compensated, void* stream);
 
+/* The 16-right-hand-side product on FP32 matrix cores with phase 2 as a
+ * stream of 4-column steps (replaces the same reference path,
+ * src/Hmatrix.py:1445-1538: the six blocks_process_MVM calls of a step for 16
+ * simulations sharing the operator). FP32 storage, tiles of at least 4 rows.
+ * d_w holds the FP32 weights, (2 n + nt + 1) rows of 16 floats: the rounded
+ * inputs (written by the call), t (written by phase 1) and a last row of
+ * zeros (the caller keeps it zero). Phase-1 work lists as in
+ * pq3d_hmat_matvec; d_part is npart*16 floats. d_steps: one 4 x int32 record
+ * per 4-column step of every phase-2 item (pool element of the step's first
+ * column at its tile's first row, column stride, plane stride, (columns - 1)
+ * << 28 | weight row), grouped by tile; d_wptr: ntiles*waves + 1 offsets, the
+ * contiguous step range of wavefront g of tile q at [wptr[q*waves + g],
+ * wptr[q*waves + g + 1]). depth (2 or 4): steps in flight per wavefront;
+ * waves (1, 2 or 4): wavefronts per tile. FP32 sums per wavefront, FP64
+ * across wavefronts; outputs FP64. d_y3 may be NULL (two planes). */
+int pq3d_hmat_matvec_mfs(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
+                         const int64_t* d_red, int64_t nred, const int32_t* d_steps,
+                         const int64_t* d_wptr, const int64_t* d_tiles, int64_t ntiles,
+                         const void* d_pool, const double* d_v1, const double* d_v2, float* d_w,
+                         int64_t n, int64_t nt, float* d_part, double* d_y1, double* d_y2,
+                         double* d_y3, int64_t y_bstride, int depth, int waves, void* stream);
+
 /* ---------------------------------------------------------- time step
  * Cells may belong to n_inst independent instances (d_inst[i], or NULL for
  * one instance); d_h, d_cellpar (V0, mu, Cs per instance), d_err (4 per

@@ -1183,6 +1183,130 @@ __global__ void __launch_bounds__(64 * kMf2Waves) k_hmv_phase2_mf16(
   }
 }
 
+// Phase 2 on FP32 matrix cores for 16 right-hand sides as a stream of
+// 4-column steps (the default matrix-core product). The host lists every
+// 4-column step of every item of a row tile (steps of one item are
+// consecutive, the items of a tile follow each other), and splits a tile's
+// steps into kW contiguous ranges, one per wavefront (wptr). Step record (4
+// x int32): pool element of the step's first column at the tile's first row
+// (plane 0), column stride, plane stride, (columns - 1) << 28 | weight row.
+// Weight rows are rows of one FP32 array w: x1 rows (n), x2 rows (n), t rows
+// (nt), then one row of zeros (zrow); a row holds the 16 right-hand sides.
+// Every wavefront keeps D steps in flight across item boundaries: a step's
+// loads are issued D steps before its products, its record 2 D steps
+// before. Loads are branch-free: a step past the range re-reads the range's
+// last step with the zero weight row, a column past the step's count reads
+// the step's last column with the zero weight row, so both add exact zeros.
+// The lane layout of the operator loads and of the products is that of
+// k_hmv_phase2_mf16 (lane l loads rows s .. s+3, s = min(4 (l & 15), mt - 4),
+// of column l >> 4; element q feeds product q). Each wavefront sums its steps
+// in FP32; the kW partial sums meet in LDS in FP64 in wave order.
+constexpr uint32_t kStepCntShift = 28;
+constexpr uint32_t kStepRowMask = (1u << kStepCntShift) - 1u;
+
+template <int NP>
+struct MfsStep {
+  f32x4 a[NP];
+  float b;
+};
+
+template <int NP, int D, int kW>
+__global__ void __launch_bounds__(64 * kW) k_hmv_phase2_mfs(
+    const int4* __restrict__ steps, const int64_t* __restrict__ wptr,
+    const int64_t* __restrict__ tiles, const float* __restrict__ pool,
+    const float* __restrict__ w, int64_t zrow, double* __restrict__ y1,
+    double* __restrict__ y2, double* __restrict__ y3, int64_t ybs) {
+  constexpr int NB = 16;
+  __shared__ double red[NP][NB][64];
+  const int64_t tile = xcd_order(blockIdx.x, gridDim.x);
+  const int64_t R0 = tiles[TL_W * tile], mt = tiles[TL_W * tile + 1];
+  const int g = wave_id();
+  const int lane = threadIdx.x & 63;
+  const int kl = lane >> 4, il = lane & 15;
+  const int s0 = 4 * il < (int)mt - 4 ? 4 * il : (int)mt - 4;
+  const int64_t sb = wptr[tile * kW + g], se = wptr[tile * kW + g + 1];
+  // the weight array spans (zrow + 1) rows of 16 floats (host-checked:
+  // under 2^32 bytes)
+  const __amdgpu_buffer_rsrc_t wrs = item_rsrc(w, (uint64_t)(zrow + 1) * NB * sizeof(float));
+  f32x4 acc[NP][4];
+#pragma unroll
+  for (int o = 0; o < NP; ++o)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[o][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if (sb < se) {
+    auto record = [&](int64_t s) -> int4 { return steps[s < se ? s : se - 1]; };
+    auto load = [&](const int4 d, bool valid, MfsStep<NP>& st) {
+      const uint32_t ws = (uint32_t)d.w;
+      const int cnt = (int)(ws >> kStepCntShift) + 1;
+      const int64_t ld = d.y, plane = d.z;
+      const int c = kl < cnt ? kl : cnt - 1;
+      // the step spans (cnt - 1) ld + (NP - 1) plane + mt elements (host-checked:
+      // under 2^31 bytes)
+      const __amdgpu_buffer_rsrc_t rs =
+          item_rsrc(pool + (uint32_t)d.x, (uint64_t)((cnt - 1) * ld + (NP - 1) * plane + mt) * 4);
+#pragma unroll
+      for (int o = 0; o < NP; ++o)
+        st.a[o] = buf_load4(rs, (uint32_t)((c * ld + o * plane + s0) * sizeof(float)), 0);
+      const int64_t xr = (valid && kl < cnt) ? (int64_t)(ws & kStepRowMask) + c : zrow;
+      st.b = __builtin_bit_cast(
+          float, __builtin_amdgcn_raw_buffer_load_b32(wrs, (uint32_t)((xr * NB + il) * 4), 0, 0));
+    };
+    MfsStep<NP> buf[D];
+    int4 rec[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      load(record(sb + j), sb + j < se, buf[j]);
+      rec[j] = record(sb + D + j);
+    }
+    for (int64_t s = sb; s < se; s += D) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+#pragma unroll
+        for (int o = 0; o < NP; ++o)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc[o][q] =
+                __builtin_amdgcn_mfma_f32_16x16x4f32(buf[j].a[o][q], buf[j].b, acc[o][q], 0, 0, 0);
+        load(rec[j], s + j + D < se, buf[j]);
+        rec[j] = record(s + j + 2 * D);
+        // keeps each step's loads right after its products
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  // deterministic cross-wave sum in FP64 (wave 0 stores, the others add in
+  // order); rows the clamp duplicated are taken once
+  for (int gg = 0; gg < kW; ++gg) {
+    if (g == gg) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 4 * kl + r;
+        const int si = 4 * i < (int)mt - 4 ? 4 * i : (int)mt - 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = si + q;
+          if (row >= 4 * i && row < mt) {
+#pragma unroll
+            for (int o = 0; o < NP; ++o) {
+              const double v = (double)acc[o][q][r];
+              double* p = &red[o][il][row];
+              *p = (gg == 0) ? v : *p + v;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double* const ys[3] = {y1, y2, y3};
+  for (int e = threadIdx.x; e < NP * NB * 64; e += 64 * kW) {
+    const int o = e / (NB * 64), m = e % (NB * 64);
+    const int row = ybs > 0 ? m % 64 : m / NB, b = ybs > 0 ? m / 64 : m % NB;
+    if (row < mt)
+      ys[o][ybs > 0 ? b * ybs + R0 + row : (R0 + row) * NB + b] = red[o][b][row];
+  }
+}
+
 // FP32 copies of the two interleaved input vectors (m values each): the
 // matrix-core products take their weights in FP32.
 __global__ void __launch_bounds__(256) k_inputs_f32(int64_t m, const double* __restrict__ v1,
@@ -1645,6 +1769,64 @@ int pq3d_hmat_matvec_mfma(const int64_t* d_units, int64_t nunits, const int64_t*
       else PQ3D_P2MF(2, false, nullptr);
     }
 #undef PQ3D_P2MF
+  }
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_hmat_matvec_mfs(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
+                         const int64_t* d_red, int64_t nred, const int32_t* d_steps,
+                         const int64_t* d_wptr, const int64_t* d_tiles, int64_t ntiles,
+                         const void* d_pool, const double* d_v1, const double* d_v2, float* d_w,
+                         int64_t n, int64_t nt, float* d_part, double* d_y1, double* d_y2,
+                         double* d_y3, int64_t y_bstride, int depth, int waves, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(ntiles >= 0 && nunits >= 0 && nred >= 0 && n >= 0 && nt >= 0,
+               "sizes must be >= 0");
+  PQ3D_REQUIRE(y_bstride >= 0, "y_bstride must be >= 0");
+  PQ3D_REQUIRE(depth == 2 || depth == 4, "depth must be 2 or 4");
+  PQ3D_REQUIRE(waves == 1 || waves == 2 || waves == 4, "waves must be 1, 2 or 4");
+  PQ3D_REQUIRE(d_w && d_v1 && d_v2 && d_y1 && d_y2, "null pointer");
+  PQ3D_REQUIRE(ntiles == 0 || (d_steps && d_wptr && d_tiles && d_pool), "null pointer");
+  PQ3D_REQUIRE(nunits == 0 || (d_units && d_members && d_pool), "null pointer");
+  PQ3D_REQUIRE(nred == 0 || d_part, "null pointer");
+  PQ3D_REQUIRE(nunits < (1LL << 26) && nred < (1LL << 26) && ntiles < (1LL << 22) &&
+                   n < (1LL << 26),
+               "too many work items");
+  // weight rows are addressed with 32-bit byte offsets and 28-bit row indices
+  PQ3D_REQUIRE((2 * n + nt + 1) * 16 * (int64_t)sizeof(float) < (1LL << 32) &&
+                   2 * n + nt < (1LL << 28),
+               "weight array too large (2 n + nt = %lld rows)", (long long)(2 * n + nt));
+  hipStream_t s = as_stream(stream);
+  const float* pool = (const float*)d_pool;
+  const int64_t m = n * 16;
+  float* x1 = d_w;
+  float* x2 = d_w + m;
+  float* t32 = d_w + 2 * m;
+  const int64_t zrow = 2 * n + nt;
+  if (m > 0) k_inputs_f32<<<blocks_for(m, 256), 256, 0, s>>>(m, d_v1, d_v2, x1, x2);
+  if (nunits > 0)
+    k_hmv_phase1_mf16<<<(unsigned)((nunits + 3) / 4), 256, 0, s>>>(d_units, nunits, d_members,
+                                                                   pool, x1, x2, t32, d_part);
+  if (nred > 0)
+    k_hmv_phase1b<16, float><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(d_red, nred, d_part, t32);
+  if (ntiles > 0) {
+    const int4* st = reinterpret_cast<const int4*>(d_steps);
+#define PQ3D_MFS(NPV, DV, WV, Y3)                                                          \
+  if (depth == DV && waves == WV) {                                                        \
+    k_hmv_phase2_mfs<NPV, DV, WV><<<(unsigned)ntiles, 64 * WV, 0, s>>>(                    \
+        st, d_wptr, d_tiles, pool, d_w, zrow, d_y1, d_y2, Y3, y_bstride);                  \
+  }
+#define PQ3D_MFS_ALL(NPV, Y3)                                                              \
+  PQ3D_MFS(NPV, 2, 1, Y3) PQ3D_MFS(NPV, 2, 2, Y3) PQ3D_MFS(NPV, 2, 4, Y3)                  \
+  PQ3D_MFS(NPV, 4, 1, Y3) PQ3D_MFS(NPV, 4, 2, Y3) PQ3D_MFS(NPV, 4, 4, Y3)
+    if (d_y3) {
+      PQ3D_MFS_ALL(3, d_y3)
+    } else {
+      PQ3D_MFS_ALL(2, nullptr)
+    }
+#undef PQ3D_MFS_ALL
+#undef PQ3D_MFS
   }
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;

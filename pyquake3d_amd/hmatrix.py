@@ -37,9 +37,17 @@ UNIT_COLS = 512    # stacked V columns per phase-1 unit
 NRHS = (1, 2, 4, 8, 16)
 # "f64": every product summed in FP64 (bitwise independent of nrhs); with FP32
 # storage and 16 right-hand sides, "mfma" runs every block product on FP32
-# matrix cores (FP32 sums per phase-1 unit and per phase-2 item group) and
-# "mfma_comp" adds the items of a group in compensated FP32 (hi + lo)
+# matrix cores (FP32 sums per phase-1 unit and per phase-2 wavefront, phase 2
+# as a stream of 4-column steps, pq3d_hmat_matvec_mfs) and "mfma_comp" runs
+# phase 2 per item group with the items added in compensated FP32 (hi + lo)
 ACCUMULATE = ("f64", "mfma", "mfma_comp")
+# matrix-core step stream: candidates of the autotuner (HMatrix.tune) and the
+# configuration used before tuning
+MFS_DEPTHS = (2, 4)           # 4-column steps in flight per wavefront
+MFS_WAVES = (1, 2, 4)         # wavefronts per row tile
+MFS_ORDERS = ("row", "cost")  # tile launch order: leaf (row) order, or XCD-balanced by cost
+MFS_DEFAULT = {"depth": 4, "waves": 1, "order": "row"}
+STEP_CNT_SHIFT = 28           # step record word 3: (columns - 1) << 28 | weight row
 
 
 def _sync(dev):
@@ -214,6 +222,66 @@ def xcd_balanced_order(cost, per_group):
         idx = np.arange(lo, hi)
         out.append(idx[np.argsort(-cost[lo:hi], kind="stable")])
     return np.concatenate(out).astype(np.int64)
+
+
+def mfs_steps(items, tile_ptr, tiles, n, nt, waves=1, order="row"):
+    """Phase-2 step stream of the 16-RHS matrix-core product
+    (``pq3d_hmat_matvec_mfs``): every item of every row tile cut into steps
+    of 4 columns. Returns (steps (S, 4) int32, wptr (T*waves + 1,) int64,
+    tiles (T, 2) int64 in launch order).
+
+    Step record: pool element of the step's first column at the tile's
+    first row (plane 0), column stride, plane stride, (columns - 1) << 28 |
+    weight row, where weight rows are x1 rows [0, n), x2 rows [n, 2n) and t
+    rows [2n, 2n + nt) of the FP32 weight array. A tile's steps keep the
+    order of its items; ``waves`` contiguous ranges of (nearly) equal length
+    per tile go to its wavefronts. ``order``: "row" launches the tiles in
+    leaf (row) order, "cost" keeps the work lists' XCD-balanced order."""
+    it = np.asarray(items, dtype=np.int64).reshape(-1, 6)
+    tp = np.asarray(tile_ptr, dtype=np.int64)
+    T = np.asarray(tiles, dtype=np.int64).reshape(-1, 2)
+    ntiles = len(T)
+    if waves not in MFS_WAVES:
+        raise ValueError(f"waves must be one of {MFS_WAVES}")
+    item_tile = np.repeat(np.arange(ntiles, dtype=np.int64), np.diff(tp))
+    if len(item_tile) != len(it):
+        raise ValueError("items and tile_ptr disagree")
+    if order == "row":
+        o = np.argsort(T[:, 0], kind="stable")
+        rank = np.empty(ntiles, dtype=np.int64)
+        rank[o] = np.arange(ntiles)
+        T = T[o]
+        item_tile = rank[item_tile]
+        io = np.argsort(item_tile, kind="stable")
+        it, item_tile = it[io], item_tile[io]
+    elif order != "cost":
+        raise ValueError(f"order must be one of {MFS_ORDERS}")
+    kind, base, ld, plane, cnt, xoff = (it[:, i] for i in range(6))
+    nst = (cnt + 3) // 4
+    rep = np.repeat(np.arange(len(it), dtype=np.int64), nst)
+    first = np.concatenate([[0], np.cumsum(nst)[:-1]]).astype(np.int64)
+    j = np.arange(len(rep), dtype=np.int64) - first[rep]
+    rowbase = np.where(kind == 0, 0, np.where(kind == 2, n, 2 * n))[rep]
+    sbase = base[rep] + 4 * j * ld[rep]
+    scnt = np.minimum(4, cnt[rep] - 4 * j)
+    srow = rowbase + xoff[rep] + 4 * j
+    if len(rep):
+        if sbase.max() >= 2 ** 32 or ld.max() >= 2 ** 31 or plane.max() >= 2 ** 31:
+            raise ValueError("step record fields out of range (pool too large for the step stream)")
+        if (srow + scnt).max() > 2 * n + nt or scnt.min() < 1 or 2 * n + nt >= 2 ** STEP_CNT_SHIFT:
+            raise ValueError("step weight rows out of range")
+    rec = np.empty((len(rep), 4), dtype=np.uint32)
+    rec[:, 0] = sbase
+    rec[:, 1] = ld[rep]
+    rec[:, 2] = plane[rep]
+    rec[:, 3] = ((scnt - 1).astype(np.uint32) << STEP_CNT_SHIFT) | srow.astype(np.uint32)
+    per_tile = np.bincount(item_tile[rep], minlength=ntiles).astype(np.int64)
+    tsp = np.concatenate([[0], np.cumsum(per_tile)]).astype(np.int64)
+    wptr = np.empty(ntiles * waves + 1, dtype=np.int64)
+    for g in range(waves):
+        wptr[g:ntiles * waves:waves] = tsp[:-1] + (g * per_tile) // waves
+    wptr[-1] = tsp[-1]
+    return np.ascontiguousarray(rec.view(np.int32)), wptr, np.ascontiguousarray(T)
 
 
 def check_item_extents(items, tiles, tile_ptr, esize) -> None:
@@ -530,7 +598,10 @@ class HMatrix:
                 torch.zeros(max(self.npart, 1) * nrhs, dtype=torch.float64, device=self.device),
                 red_d)
             if nrhs == 16 and self.esize == 4:
-                self._x32(nrhs)      # created here so that graph capture finds it
+                # created here so that graph capture finds them
+                self._x32(nrhs)
+                if self.min_tile_rows >= 4:
+                    self._mfs_lists()
         return self._scratch[nrhs]
 
     # ----------------------------------------------------------------- apply
@@ -569,7 +640,17 @@ class HMatrix:
                 _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
                 _lib.ptr(v2), _lib.ptr(t), _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]),
                 y3, n if (instance_major and nrhs > 1) else 0]
-        if mfma:
+        if mfma and self.accumulate == "mfma":
+            steps, wptr, tl = self._mfs_lists()
+            cfg = self._tuning()
+            _lib.call("pq3d_hmat_matvec_mfs", _lib.ptr(self.units), self.nunits,
+                      _lib.ptr(self.members), _lib.ptr(red), int(red.shape[0]), _lib.ptr(steps),
+                      _lib.ptr(wptr), _lib.ptr(tl), self.ntiles, _lib.ptr(self.pool),
+                      _lib.ptr(v1), _lib.ptr(v2), _lib.ptr(self._x32(nrhs)), n, self.nt,
+                      _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]), y3,
+                      n if (instance_major and nrhs > 1) else 0, int(cfg["depth"]),
+                      int(cfg["waves"]), _lib.stream_handle(stream))
+        elif mfma:
             _lib.call("pq3d_hmat_matvec_mfma", *args, _lib.ptr(self._x32(nrhs)), n,
                       int(self.accumulate == "mfma_comp"), _lib.stream_handle(stream))
         else:
@@ -577,12 +658,88 @@ class HMatrix:
         return out
 
     def _x32(self, nrhs):
-        """FP32 scratch of the matrix-core product: the two inputs rounded."""
+        """FP32 weights of the matrix-core products: the two inputs rounded
+        (rows [0, 2n)), t (rows [2n, 2n + nt), written by phase 1 of the step
+        stream product) and one row of zeros (read by the step stream for
+        masked columns; nothing writes it)."""
         buf = getattr(self, "_x32_buf", None)
-        if buf is None or buf.numel() < 2 * self.n * nrhs:
-            buf = torch.empty(2 * self.n * nrhs, dtype=torch.float32, device=self.device)
+        rows = 2 * self.n + self.nt + 1
+        if buf is None or buf.numel() < rows * nrhs:
+            buf = torch.zeros(rows * nrhs, dtype=torch.float32, device=self.device)
             self._x32_buf = buf
         return buf
+
+    def _tuning(self) -> dict:
+        cfg = getattr(self, "tuning", None)
+        return dict(MFS_DEFAULT) if cfg is None else cfg
+
+    def _mfs_lists(self, cfg=None):
+        """Device step stream (steps, wptr, tiles) of the matrix-core product
+        for the tuning ``cfg`` (default: the current one), built once per
+        (tile order, wavefronts per tile)."""
+        cfg = self._tuning() if cfg is None else cfg
+        cache = getattr(self, "_mfs_cache", None)
+        if cache is None:
+            cache = self._mfs_cache = {}
+        key = (cfg["order"], int(cfg["waves"]))
+        if key not in cache:
+            host = getattr(self, "_wl_host", None)
+            if host is None:
+                host = self._wl_host = (self.items.cpu().numpy(), self.tile_ptr.cpu().numpy(),
+                                        self.tiles.cpu().numpy())
+            st, wp, tl = mfs_steps(*host, self.n, self.nt, waves=key[1], order=key[0])
+            dev = self.device
+            cache[key] = (torch.from_numpy(st).to(dev), torch.from_numpy(wp).to(dev),
+                          torch.from_numpy(tl).to(dev))
+        return cache[key]
+
+    def tune(self, planes=2, reps=10, depths=MFS_DEPTHS, waves=(1, 2), orders=MFS_ORDERS,
+             seed=0) -> dict:
+        """Autotuner of the 16-RHS matrix-core product: times every
+        candidate (steps in flight per wavefront, wavefronts per row tile,
+        tile launch order) on this operator with HIP events (``reps``
+        products each, after two warm-up products, ``planes`` row planes,
+        seeded inputs) and keeps the fastest as ``tuning``. A choice changes
+        only the FP32 summation order within a tile. Returns {"choice",
+        "planes", "candidates": [(config, ms)]}, also kept as
+        ``tuning_report``."""
+        if self.esize != 4 or self.min_tile_rows < 4:
+            raise ValueError("the matrix-core product needs FP32 storage and tiles of >= 4 rows")
+        n, nb, dev = self.n, 16, self.device
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        v1 = torch.randn(n * nb, dtype=torch.float64, device=dev, generator=gen)
+        v2 = torch.randn(n * nb, dtype=torch.float64, device=dev, generator=gen)
+        out = torch.empty((3, n * nb), dtype=torch.float64, device=dev)
+        acc0, cfg0 = self.accumulate, getattr(self, "tuning", None)
+        self.accumulate = "mfma"
+        results = []
+        try:
+            for order in orders:
+                for w in waves:
+                    for d in depths:
+                        self.tuning = {"depth": int(d), "waves": int(w), "order": order}
+                        for _ in range(2):
+                            self.matvec(v1, v2, out=out, nrhs=nb, planes=planes)
+                        ev = [(torch.cuda.Event(enable_timing=True),
+                               torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+                        for a, b in ev:
+                            a.record()
+                            self.matvec(v1, v2, out=out, nrhs=nb, planes=planes)
+                            b.record()
+                        torch.cuda.synchronize(dev)
+                        ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+                        results.append((dict(self.tuning), ms))
+        finally:
+            self.accumulate = acc0
+            self.tuning = cfg0
+        best = dict(min(results, key=lambda r: r[1])[0])
+        self.tuning = best
+        # only the chosen step stream stays on the device
+        key = (best["order"], best["waves"])
+        self._mfs_cache = {key: self._mfs_cache[key]}
+        self.tuning_report = {"choice": best, "planes": planes, "candidates": results}
+        return self.tuning_report
 
     def algorithmic_bytes(self, nrhs=1, planes=3):
         """Bytes one product must move at least: every stored operator
@@ -637,6 +794,9 @@ class HMatrix:
                   "n_lowrank": str(st.n_lowrank), "dense_entries": str(st.dense_entries),
                   "lowrank_entries": str(st.lowrank_entries), "ranks_mean": repr(st.ranks_mean),
                   "ranks_max": str(st.ranks_max), "fallback_dense": str(st.fallback_dense)}
+        cfg = getattr(self, "tuning", None)
+        if cfg is not None:
+            header["mfs_tuning"] = f"{cfg['depth']},{cfg['waves']},{cfg['order']}"
         header.update({"user_" + k: str(v) for k, v in meta.items()})
         save_file(tens, str(path), metadata=header)
 
@@ -661,6 +821,11 @@ class HMatrix:
         self.halfspace = h["halfspace"] == "1"
         self.mu, self.lambda_ = float(h["mu"]), float(h["lambda"])
         self.meta = {k[5:]: v for k, v in h.items() if k.startswith("user_")}
+        if "mfs_tuning" in h:
+            d, w, o = h["mfs_tuning"].split(",")
+            if int(d) not in MFS_DEPTHS or int(w) not in MFS_WAVES or o not in MFS_ORDERS:
+                raise ValueError(f"{path}: bad mfs_tuning {h['mfs_tuning']!r}")
+            self.tuning = {"depth": int(d), "waves": int(w), "order": o}
         self.pool = t["pool"].to(dev)
         for k in ("units", "members", "reduce", "tiles", "tile_ptr", "items"):
             setattr(self, k, t[k].to(dev))
@@ -787,6 +952,9 @@ class MultiHMatrix:
     _bufs = HMatrix._bufs
     matvec = HMatrix.matvec
     _x32 = HMatrix._x32
+    _tuning = HMatrix._tuning
+    _mfs_lists = HMatrix._mfs_lists
+    tune = HMatrix.tune
 
     def algorithmic_bytes(self, nrhs=1, planes=3):
         """``HMatrix.algorithmic_bytes`` summed over the instances (one

@@ -171,6 +171,47 @@ def test_mfma_matvec_16_rhs(gpu, pq3d_lib, case, leaf, mode):
     assert np.array_equal(y1, y64[:, :, 0])
 
 
+@pytest.mark.parametrize("planes", [2, 3])
+def test_step_stream_configurations_and_autotuner(gpu, pq3d_lib, case, planes, tmp_path):
+    """Every configuration of the step-stream product (steps in flight,
+    wavefronts per tile, tile order) stays within 1e-5 of the
+    FP64-accumulated product of the same storage and is repeatable; the
+    autotuner times them all, keeps the fastest, and the tuned choice
+    survives save/load."""
+    mesh, _ = case
+    H = HMatrix.from_mesh(mesh, MU, LAM, leaf=32, eta=2.0, eps=1e-4, dtype=torch.float32,
+                          device=gpu)
+    n = H.n
+    rng = np.random.default_rng(23)
+    X1 = torch.from_numpy(rng.standard_normal(n * 16)).to(gpu)
+    X2 = torch.from_numpy(rng.standard_normal(n * 16)).to(gpu)
+    y64 = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()
+    H.accumulate = "mfma"
+    for order in ("row", "cost"):
+        for waves in (1, 2, 4):
+            for depth in (2, 4):
+                H.tuning = {"depth": depth, "waves": waves, "order": order}
+                y = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes]
+                assert torch.equal(y, H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes])
+                y = y.cpu().numpy()
+                for b in range(16):
+                    assert _relerr(y[:, :, b], y64[:, :, b]) < 1e-5, (order, waves, depth, b)
+    H.tuning = None
+    rep = H.tune(planes=planes, reps=3)
+    assert len(rep["candidates"]) == 8
+    assert rep["choice"] == min(rep["candidates"], key=lambda c: c[1])[0]
+    assert H.tuning == rep["choice"] and H.accumulate == "mfma"
+    y = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()
+    for b in range(16):
+        assert _relerr(y[:, :, b], y64[:, :, b]) < 1e-5
+    H.save(tmp_path / "h.safetensors")
+    G = HMatrix.load(tmp_path / "h.safetensors", device=gpu)
+    assert G.tuning == H.tuning
+    G.accumulate = "mfma"
+    yg = G.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()
+    assert np.array_equal(yg, y)
+
+
 def test_halfspace_hmatrix_matches_dense_halfspace_operators(gpu, pq3d_lib):
     """Half-space H-matrix of a surface-breaking fault: all-dense storage
     equals the half-space traction block; ACA at eps 1e-7 stays within 1e-5
