@@ -181,8 +181,9 @@ int pq3d_block_level(const int64_t* d_pairs, int64_t npairs, const int64_t* d_no
 /* Row widths (int64 elements) of the tables the kernels below read, in the
  * order: dense block, low-rank block, ACA workspace offsets, matvec item,
  * phase-1 unit, phase-1b reduction, tile, receiver record, source record,
- * half-space source record, phase-1 unit member. Host-only; returns the
- * number of values written (at most n). */
+ * half-space source record, phase-1 unit member, V panel packing row, panel
+ * phase-1 unit. Host-only; returns the number of values written (at most
+ * n). */
 int pq3d_layout_constants(int64_t* out, int64_t n);
 
 /* Near-field blocks (replaces ACA_worker's dense branch with
@@ -286,27 +287,48 @@ int pq3d_hmat_matvec_mfma(const int64_t* d_units, int64_t nunits, const int64_t*
                           double* d_y2, double* d_y3, int64_t y_bstride, float* d_x32,
                           int64_t n, int compensated, void* stream);
 
-/* The 16-right-hand-side product on FP32 matrix cores with phase 2 as a
- * stream of 4-column steps (replaces the same reference path,
- * src/Hmatrix.py:1445-1538: the six blocks_process_MVM calls of a step for 16
- * simulations sharing the operator). FP32 storage, tiles of at least 4 rows.
- * d_w holds the FP32 weights, (2 n + nt + 1) rows of 16 floats: the rounded
- * inputs (written by the call), t (written by phase 1) and a last row of
- * zeros (the caller keeps it zero). Phase-1 work lists as in
- * pq3d_hmat_matvec; d_part is npart*16 floats. d_steps: one 4 x int32 record
- * per 4-column step of every phase-2 item (pool element of the step's first
- * column at its tile's first row, column stride, plane stride, (columns - 1)
- * << 28 | weight row), grouped by tile; d_wptr: ntiles*waves + 1 offsets, the
- * contiguous step range of wavefront g of tile q at [wptr[q*waves + g],
- * wptr[q*waves + g + 1]). depth (2 or 4): steps in flight per wavefront;
- * waves (1, 2 or 4): wavefronts per tile. FP32 sums per wavefront, FP64
- * across wavefronts; outputs FP64. d_y3 may be NULL (two planes). */
-int pq3d_hmat_matvec_mfs(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
+/* V panels of the matrix-core product (pq3d_hmat_matvec_mfs), packed from an
+ * FP32 pool. d_tab: nblocks x 6 int64 per low-rank block (its V^T base in
+ * the pool (2n x k, row-major), k, n, panel base in d_vp, panel rank K,
+ * first rank of the block in the panel). A panel holds every low-rank block
+ * of one source cluster side by side (2n x K), chunk-major: ranks
+ * [64 j, 64 j + kc) form a contiguous 2n x kc row-major chunk at
+ * base + 2n 64 j. No counterpart in the reference (its blocks keep separate
+ * factors, src/Hmatrix.py:1014-1066); it is the storage the product below
+ * streams. */
+int pq3d_hmat_vpanel_pack(const int64_t* d_tab, int64_t nblocks, const void* d_pool,
+                          float* d_vp, void* stream);
+
+/* The 16-right-hand-side product on FP32 matrix cores (replaces the same
+ * reference path, src/Hmatrix.py:1445-1538: the six blocks_process_MVM calls
+ * of a step for 16 simulations sharing the operator). FP32 storage, tiles
+ * of at least 4 rows.
+ * Weights d_w: (2 n + nt + 1) rows of 16 floats: the inputs rounded to FP32
+ * (written by the call), t (written by phase 1) and a last row of zeros
+ * (the caller keeps it zero).
+ * Phase 1 (t = V^T x): d_punits, npunits x 8 int64 units over the V panels
+ * d_vpanels (pq3d_hmat_vpanel_pack): (chunk element of the unit's first
+ * column, ranks kc <= 64, output row, first stacked column, columns, the
+ * panel's cluster n and c0, direct flag); a unit is p1_waves (2 or 4)
+ * wavefronts with p1_depth (2 or 4) 4-column steps in flight each; direct
+ * units write t rows, the others partial rows of d_part (npart*16 floats)
+ * summed by the d_red rows as in pq3d_hmat_matvec (t rows numbered panel by
+ * panel).
+ * Phase 2: d_steps, one 4 x int32 record per 4-column step of every item
+ * (pool element of the step's first column at its tile's first row, column
+ * stride, plane stride, (columns - 1) << 28 | weight row), grouped by tile;
+ * d_wptr: ntiles*waves + 1 offsets, the contiguous step range of wavefront g
+ * of tile q at [wptr[q*waves + g], wptr[q*waves + g + 1]); depth (2 or 4)
+ * steps in flight per wavefront, waves (1, 2 or 4) wavefronts per tile.
+ * FP32 sums per wavefront, FP64 across wavefronts; outputs FP64 as in
+ * pq3d_hmat_matvec. d_y3 may be NULL (two planes). */
+int pq3d_hmat_matvec_mfs(const int64_t* d_punits, int64_t npunits, const float* d_vpanels,
                          const int64_t* d_red, int64_t nred, const int32_t* d_steps,
                          const int64_t* d_wptr, const int64_t* d_tiles, int64_t ntiles,
                          const void* d_pool, const double* d_v1, const double* d_v2, float* d_w,
                          int64_t n, int64_t nt, float* d_part, double* d_y1, double* d_y2,
-                         double* d_y3, int64_t y_bstride, int depth, int waves, void* stream);
+                         double* d_y3, int64_t y_bstride, int p1_depth, int p1_waves, int depth,
+                         int waves, void* stream);
 
 /* ---------------------------------------------------------- time step
  * Cells may belong to n_inst independent instances (d_inst[i], or NULL for

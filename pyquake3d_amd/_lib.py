@@ -95,10 +95,11 @@ SIGNATURES = {
                                       c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int,
                                       c_void_p]),
+    "pq3d_hmat_vpanel_pack": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     "pq3d_hmat_matvec_mfs": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p,
                                      c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
-                                     c_void_p, c_int64, c_int, c_int, c_void_p]),
+                                     c_void_p, c_int64, c_int, c_int, c_int, c_int, c_void_p]),
     # time step
     "pq3d_rk45_attempt": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -162,7 +163,7 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
 
 
 LAYOUT_NAMES = ("dense_block", "lowrank_block", "aca_ws", "item", "unit", "reduce", "tile",
-                "recv", "rec", "hrec", "member")
+                "recv", "rec", "hrec", "member", "vpanel", "punit")
 
 
 def layout_constants() -> dict:

@@ -53,6 +53,16 @@ enum { MB_VT = 0, MB_K = 1, MB_OUT = 2, MB_LANE = 3, MB_W = 4 };
 enum { RD_T = 0, RD_K = 1, RD_P = 2, RD_NU = 3, RD_L0 = 4, RD_LC = 5, RD_W = 6 };
 // tile: first row, rows
 enum { TL_W = 2 };
+// V panel packing (matrix-core product): one row per low-rank block: its V^T
+// base in the pool, k, n, the panel's base, the panel's rank K, the block's
+// first rank in the panel
+enum { VP_SRC = 0, VP_K = 1, VP_N = 2, VP_BASE = 3, VP_KP = 4, VP_KOFF = 5, VP_W = 6 };
+// panel phase-1 unit: V^T element (cb, first rank) of a 64-rank chunk of a
+// panel (the chunk is 2n x kc, row-major, so the row stride is kc), ranks kc,
+// output row (t row, or partial row), first stacked column cb, columns cc,
+// the panel's source cluster (n, c0), direct flag
+enum { PU_VB = 0, PU_KC = 1, PU_OUT = 2, PU_CB = 3, PU_CC = 4, PU_N = 5, PU_C0 = 6,
+       PU_DIRECT = 7, PU_W = 8 };
 constexpr int kRecvW = 12;  // receiver record: centroid + ev1, ev2, ev3
 
 __device__ inline void load_rec_g(const double* __restrict__ src, double* r) {
@@ -605,6 +615,26 @@ __global__ void __launch_bounds__(256) k_aca_pack(const int64_t* __restrict__ ta
   for (int64_t p = threadIdx.x; p < nvv; p += blockDim.x) {
     const int64_t c = p / k, l = p - c * k;
     pv[p] = (T)V[l * C + c];
+  }
+}
+
+// V panels of the matrix-core product: every low-rank block of one source
+// cluster (n columns, 2n stacked) side by side as one 2n x K matrix (K = the
+// sum of their ranks), stored chunk-major: ranks [64 j, 64 j + kc_j) form a
+// contiguous 2n x kc_j row-major chunk at base + 2n 64 j. Element (c, l) of
+// a block's V^T (pool: src + c k + l) goes to panel rank koff + l. One
+// workgroup per block; reads coalesced.
+__global__ void __launch_bounds__(256) k_vpanel_pack(const int64_t* __restrict__ tab,
+                                                     const float* __restrict__ pool,
+                                                     float* __restrict__ vp) {
+  const int64_t* b = tab + (int64_t)blockIdx.x * VP_W;
+  const int64_t src = b[VP_SRC], k = b[VP_K], C = 2 * b[VP_N], base = b[VP_BASE];
+  const int64_t K = b[VP_KP], koff = b[VP_KOFF];
+  const int64_t tot = C * k;
+  for (int64_t p = threadIdx.x; p < tot; p += blockDim.x) {
+    const int64_t c = p / k, l = p - c * k, r = koff + l, j = r >> 6;
+    const int64_t kc = K - 64 * j < 64 ? K - 64 * j : 64;
+    vp[base + C * 64 * j + c * kc + (r - 64 * j)] = pool[src + p];
   }
 }
 
@@ -1183,6 +1213,98 @@ __global__ void __launch_bounds__(64 * kMf2Waves) k_hmv_phase2_mf16(
   }
 }
 
+// Phase 1 of the step-stream product: t = V^T x over V panels on FP32
+// matrix cores, 16 right-hand sides. A unit is a column range [cb, cb + cc)
+// of one 64-rank chunk of a panel (a contiguous cc x kc row-major block);
+// its kW wavefronts take its 4-column steps in turn (wavefront g: steps g,
+// g + kW, ...), so a workgroup reads one contiguous window of kW steps at a
+// time, each wavefront with D steps in flight. Lane l loads ranks s .. s+3
+// (s = min(4 (l & 15), kc - 4)) of column 4 step + (l >> 4) with one 16-byte
+// load; element q of it is the A operand of product q (rows = ranks 4 i + q),
+// B the column's 16 weights (x1 rows for stacked columns < n, else x2 rows;
+// the zero row for columns past the unit). The kW partial sums meet in LDS
+// and are added in wavefront order in FP32; t rows (or partial rows of a
+// column-split chunk) are written as FP32.
+template <int D, int kW>
+__global__ void __launch_bounds__(64 * kW) k_hmv_phase1_mfp(
+    const int64_t* __restrict__ units, const float* __restrict__ vp, const float* __restrict__ w,
+    int64_t nrows, int64_t zrow, float* __restrict__ t, float* __restrict__ part) {
+  constexpr int NB = 16;
+  __shared__ float sacc[kW][16][64];
+  const int64_t u = xcd_order(blockIdx.x, gridDim.x);
+  const int g = wave_id();
+  const int lane = threadIdx.x & 63;
+  const int kl = lane >> 4, il = lane & 15;
+  const int64_t* it = units + u * PU_W;
+  const int64_t vb = it[PU_VB], out = it[PU_OUT], cb = it[PU_CB], cc = it[PU_CC];
+  const int64_t n = it[PU_N], c0 = it[PU_C0];
+  const int kc = (int)it[PU_KC];
+  float* dst = it[PU_DIRECT] ? t : part;
+  const int64_t nsteps = (cc + 3) / 4;
+  // weights: (zrow + 1) rows of 16 floats; the chunk: cc rows of kc floats
+  // (both host-checked: under 2^32 bytes)
+  const __amdgpu_buffer_rsrc_t wrs = item_rsrc(w, (uint64_t)(zrow + 1) * NB * sizeof(float));
+  const __amdgpu_buffer_rsrc_t rs = item_rsrc(vp + vb, (uint64_t)(cc * kc) * sizeof(float));
+  const int ldb = kc * (int)sizeof(float);
+  const int s0 = kc >= 4 ? (4 * il < kc - 4 ? 4 * il : kc - 4) : 0;
+  const uint32_t voff = (uint32_t)(s0 * sizeof(float)) + (uint32_t)(kl * ldb);
+  f32x4 acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  struct St {
+    f32x4 a;
+    float b;
+  };
+  // branch-free: a step past the unit re-reads its last step, a column past
+  // the unit reads the last column; both take the zero weight row
+  auto load = [&](int64_t s, St& st) {
+    const bool sv = s < nsteps;
+    const int64_t sc = sv ? s : nsteps - 1;
+    const int64_t col = 4 * sc + kl;
+    const bool in = sv && col < cc;
+    const int64_t cl = col < cc ? col : cc - 1;
+    const int64_t gc = cb + cl;
+    const int64_t xr = !in ? zrow : gc < n ? c0 + gc : nrows + c0 + gc - n;
+    st.b = __builtin_bit_cast(
+        float, __builtin_amdgcn_raw_buffer_load_b32(wrs, (uint32_t)((xr * NB + il) * 4), 0, 0));
+    st.a = buf_load4(rs, voff - (uint32_t)((col - cl) * ldb), (int)(4 * sc * ldb));
+  };
+  St buf[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) load(g + (int64_t)j * kW, buf[j]);
+  for (int64_t s = g; s < nsteps; s += (int64_t)D * kW) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(buf[j].a[q], buf[j].b, acc[q], 0, 0, 0);
+      load(s + (int64_t)(j + D) * kW, buf[j]);
+    }
+  }
+  // D layout: lane l, register r of product q -> rank row 4 (l >> 4) + r of
+  // product q (rank s_i + q), right-hand side l & 15
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sacc[g][4 * q + r][lane] = acc[q][r];
+  __syncthreads();
+  if (g == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 4 * kl + r;
+      const int si = kc >= 4 ? (4 * i < kc - 4 ? 4 * i : kc - 4) : 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float v = sacc[0][4 * q + r][lane];
+#pragma unroll
+        for (int ww = 1; ww < kW; ++ww) v += sacc[ww][4 * q + r][lane];
+        const int rk = si + q;
+        if (rk >= 4 * i && rk < kc) dst[(out + rk) * NB + il] = v;
+      }
+    }
+  }
+}
+
 // Phase 2 on FP32 matrix cores for 16 right-hand sides as a stream of
 // 4-column steps (the default matrix-core product). The host lists every
 // 4-column step of every item of a row tile (steps of one item are
@@ -1475,9 +1597,10 @@ extern "C" {
 // unit, phase-1b reduction, tile, receiver record, source record, half-space
 // source record, phase-1 unit member. Returns the number written.
 int pq3d_layout_constants(int64_t* out, int64_t n) {
-  const int64_t v[11] = {DB_W, LB_W, WS_W, IT_W, UN_W, RD_W, TL_W, kRecvW, kSrcRec, kHsRec, MB_W};
+  const int64_t v[13] = {DB_W, LB_W, WS_W, IT_W, UN_W, RD_W, TL_W, kRecvW, kSrcRec, kHsRec,
+                         MB_W, VP_W, PU_W};
   int64_t k = 0;
-  for (; k < n && k < 11; ++k) out[k] = v[k];
+  for (; k < n && k < 13; ++k) out[k] = v[k];
   return (int)k;
 }
 
@@ -1774,23 +1897,39 @@ int pq3d_hmat_matvec_mfma(const int64_t* d_units, int64_t nunits, const int64_t*
   return PQ3D_OK;
 }
 
-int pq3d_hmat_matvec_mfs(const int64_t* d_units, int64_t nunits, const int64_t* d_members,
+int pq3d_hmat_vpanel_pack(const int64_t* d_tab, int64_t nblocks, const void* d_pool,
+                          float* d_vp, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(nblocks >= 0, "nblocks must be >= 0");
+  if (nblocks == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_tab && d_pool && d_vp, "null pointer");
+  PQ3D_REQUIRE(nblocks < (1LL << 31), "too many blocks");
+  k_vpanel_pack<<<(unsigned)nblocks, 256, 0, as_stream(stream)>>>(d_tab, (const float*)d_pool,
+                                                                 d_vp);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_hmat_matvec_mfs(const int64_t* d_punits, int64_t npunits, const float* d_vpanels,
                          const int64_t* d_red, int64_t nred, const int32_t* d_steps,
                          const int64_t* d_wptr, const int64_t* d_tiles, int64_t ntiles,
                          const void* d_pool, const double* d_v1, const double* d_v2, float* d_w,
                          int64_t n, int64_t nt, float* d_part, double* d_y1, double* d_y2,
-                         double* d_y3, int64_t y_bstride, int depth, int waves, void* stream) {
+                         double* d_y3, int64_t y_bstride, int p1_depth, int p1_waves, int depth,
+                         int waves, void* stream) {
   clear_error();
-  PQ3D_REQUIRE(ntiles >= 0 && nunits >= 0 && nred >= 0 && n >= 0 && nt >= 0,
+  PQ3D_REQUIRE(ntiles >= 0 && npunits >= 0 && nred >= 0 && n >= 0 && nt >= 0,
                "sizes must be >= 0");
   PQ3D_REQUIRE(y_bstride >= 0, "y_bstride must be >= 0");
   PQ3D_REQUIRE(depth == 2 || depth == 4, "depth must be 2 or 4");
   PQ3D_REQUIRE(waves == 1 || waves == 2 || waves == 4, "waves must be 1, 2 or 4");
+  PQ3D_REQUIRE(p1_depth == 2 || p1_depth == 4, "p1_depth must be 2 or 4");
+  PQ3D_REQUIRE(p1_waves == 2 || p1_waves == 4, "p1_waves must be 2 or 4");
   PQ3D_REQUIRE(d_w && d_v1 && d_v2 && d_y1 && d_y2, "null pointer");
   PQ3D_REQUIRE(ntiles == 0 || (d_steps && d_wptr && d_tiles && d_pool), "null pointer");
-  PQ3D_REQUIRE(nunits == 0 || (d_units && d_members && d_pool), "null pointer");
+  PQ3D_REQUIRE(npunits == 0 || (d_punits && d_vpanels), "null pointer");
   PQ3D_REQUIRE(nred == 0 || d_part, "null pointer");
-  PQ3D_REQUIRE(nunits < (1LL << 26) && nred < (1LL << 26) && ntiles < (1LL << 22) &&
+  PQ3D_REQUIRE(npunits < (1LL << 31) && nred < (1LL << 26) && ntiles < (1LL << 22) &&
                    n < (1LL << 26),
                "too many work items");
   // weight rows are addressed with 32-bit byte offsets and 28-bit row indices
@@ -1805,9 +1944,14 @@ int pq3d_hmat_matvec_mfs(const int64_t* d_units, int64_t nunits, const int64_t* 
   float* t32 = d_w + 2 * m;
   const int64_t zrow = 2 * n + nt;
   if (m > 0) k_inputs_f32<<<blocks_for(m, 256), 256, 0, s>>>(m, d_v1, d_v2, x1, x2);
-  if (nunits > 0)
-    k_hmv_phase1_mf16<<<(unsigned)((nunits + 3) / 4), 256, 0, s>>>(d_units, nunits, d_members,
-                                                                   pool, x1, x2, t32, d_part);
+  if (npunits > 0) {
+#define PQ3D_MFP(DV, WV)                                                                    \
+  if (p1_depth == DV && p1_waves == WV)                                                     \
+    k_hmv_phase1_mfp<DV, WV><<<(unsigned)npunits, 64 * WV, 0, s>>>(d_punits, d_vpanels, d_w, \
+                                                                   n, zrow, t32, d_part);
+    PQ3D_MFP(2, 2) PQ3D_MFP(2, 4) PQ3D_MFP(4, 2) PQ3D_MFP(4, 4)
+#undef PQ3D_MFP
+  }
   if (nred > 0)
     k_hmv_phase1b<16, float><<<(unsigned)((nred + 3) / 4), 256, 0, s>>>(d_red, nred, d_part, t32);
   if (ntiles > 0) {

@@ -46,7 +46,10 @@ ACCUMULATE = ("f64", "mfma", "mfma_comp")
 MFS_DEPTHS = (2, 4)           # 4-column steps in flight per wavefront
 MFS_WAVES = (1, 2, 4)         # wavefronts per row tile
 MFS_ORDERS = ("row", "cost")  # tile launch order: leaf (row) order, or XCD-balanced by cost
-MFS_DEFAULT = {"depth": 4, "waves": 1, "order": "row"}
+MFP_DEPTHS = (2, 4)           # phase 1: steps in flight per wavefront
+MFP_WAVES = (2, 4)            # phase 1: wavefronts per unit
+MFS_DEFAULT = {"depth": 4, "waves": 1, "order": "row", "p1_depth": 4, "p1_waves": 4}
+VP_UNIT_COLS = 2048           # stacked V columns per phase-1 unit of the V panels
 STEP_CNT_SHIFT = 28           # step record word 3: (columns - 1) << 28 | weight row
 
 
@@ -282,6 +285,71 @@ def mfs_steps(items, tile_ptr, tiles, n, nt, waves=1, order="row"):
         wptr[g:ntiles * waves:waves] = tsp[:-1] + (g * per_tile) // waves
     wptr[-1] = tsp[-1]
     return np.ascontiguousarray(rec.view(np.int32)), wptr, np.ascontiguousarray(T)
+
+
+def vpanel_plan(c0, nn, lr_idx, ranks, l_vb, unit_cols=VP_UNIT_COLS):
+    """V panels of the matrix-core product (``pq3d_hmat_vpanel_pack``,
+    ``pq3d_hmat_matvec_mfs``). The kept low-rank blocks ``lr_idx`` (ranks,
+    V^T bases ``l_vb`` in the pool) are grouped by source cluster (c0, n);
+    a group's V^T are stored side by side as one 2n x K panel (K = the sum of
+    their ranks, blocks in ``lr_idx`` order), chunk-major: ranks
+    [64 j, 64 j + kc) of a panel form one contiguous 2n x kc row-major chunk.
+    t rows are numbered panel by panel, so a chunk's t rows are contiguous.
+
+    Returns dict with
+      tab    (B, 6): packing rows (V^T base, k, n, panel base, K, first rank)
+      units  (U, 8): phase-1 units (chunk element of the first column, kc,
+             output row, first stacked column, columns, n, c0, direct), one
+             per column range of at most ``unit_cols`` stacked columns of a
+             chunk, in XCD-balanced cost order
+      reduce (R, 4): (t row, kc, first partial row, units) of chunks split
+             over several units
+      toff   (B,):   first t row of every block in the panel numbering
+      size, nt, npart: panel elements, t rows, partial rows"""
+    lr = np.asarray(lr_idx, dtype=np.int64)
+    k = np.asarray(ranks, dtype=np.int64)
+    vb = np.asarray(l_vb, dtype=np.int64)
+    bc0 = np.asarray(c0, dtype=np.int64)[lr]
+    bn = np.asarray(nn, dtype=np.int64)[lr]
+    order = np.lexsort((bn, bc0))          # stable: lr order inside a group
+    tab = np.zeros((len(lr), 6), dtype=np.int64)
+    toff = np.zeros(len(lr), dtype=np.int64)
+    units, reds = [], []
+    pbase = tp = poff = 0
+    i = 0
+    while i < len(order):
+        j = i
+        while j < len(order) and bc0[order[j]] == bc0[order[i]] and bn[order[j]] == bn[order[i]]:
+            j += 1
+        mem = order[i:j]
+        n_, c0_ = int(bn[mem[0]]), int(bc0[mem[0]])
+        K = int(k[mem].sum())
+        koff = np.concatenate([[0], np.cumsum(k[mem])[:-1]]).astype(np.int64)
+        tab[mem] = np.stack([vb[mem], k[mem], np.full(len(mem), n_), np.full(len(mem), pbase),
+                             np.full(len(mem), K), koff], axis=1)
+        toff[mem] = tp + koff
+        C = 2 * n_
+        nc = -(-C // unit_cols)
+        for l0 in range(0, K, 64):
+            kc = min(64, K - l0)
+            cbase = pbase + C * l0
+            for u in range(nc):
+                cb = u * unit_cols
+                cc = min(unit_cols, C - cb)
+                out = tp + l0 if nc == 1 else poff + u * kc
+                units.append((cbase + cb * kc, kc, out, cb, cc, n_, c0_, 1 if nc == 1 else 0))
+            if nc > 1:
+                reds.append((tp + l0, kc, poff, nc))
+                poff += nc * kc
+        pbase += C * K
+        tp += K
+        i = j
+    U = np.array(units, dtype=np.int64).reshape(-1, 8)
+    if len(U):
+        U = U[xcd_balanced_order(U[:, 4] * U[:, 1], 1)]
+    return dict(tab=np.ascontiguousarray(tab), units=np.ascontiguousarray(U),
+                reduce=np.array(reds, dtype=np.int64).reshape(-1, 4), toff=toff,
+                size=int(pbase), nt=int(tp), npart=int(poff))
 
 
 def check_item_extents(items, tiles, tile_ptr, esize) -> None:
@@ -600,7 +668,7 @@ class HMatrix:
             if nrhs == 16 and self.esize == 4:
                 # created here so that graph capture finds them
                 self._x32(nrhs)
-                if self.min_tile_rows >= 4:
+                if isinstance(self, HMatrix) and self.min_tile_rows >= 4:
                     self._mfs_lists()
         return self._scratch[nrhs]
 
@@ -640,16 +708,18 @@ class HMatrix:
                 _lib.ptr(self.items), _lib.ptr(self.pool), self.esize, nrhs, _lib.ptr(v1),
                 _lib.ptr(v2), _lib.ptr(t), _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]),
                 y3, n if (instance_major and nrhs > 1) else 0]
-        if mfma and self.accumulate == "mfma":
+        if mfma and self.accumulate == "mfma" and isinstance(self, HMatrix):
             steps, wptr, tl = self._mfs_lists()
+            vp = self._vpanels()
             cfg = self._tuning()
-            _lib.call("pq3d_hmat_matvec_mfs", _lib.ptr(self.units), self.nunits,
-                      _lib.ptr(self.members), _lib.ptr(red), int(red.shape[0]), _lib.ptr(steps),
+            _lib.call("pq3d_hmat_matvec_mfs", _lib.ptr(vp["units"]), vp["nunits"],
+                      _lib.ptr(vp["vp"]), _lib.ptr(vp["red"]), vp["nred"], _lib.ptr(steps),
                       _lib.ptr(wptr), _lib.ptr(tl), self.ntiles, _lib.ptr(self.pool),
                       _lib.ptr(v1), _lib.ptr(v2), _lib.ptr(self._x32(nrhs)), n, self.nt,
-                      _lib.ptr(part), _lib.ptr(out[0]), _lib.ptr(out[1]), y3,
-                      n if (instance_major and nrhs > 1) else 0, int(cfg["depth"]),
-                      int(cfg["waves"]), _lib.stream_handle(stream))
+                      _lib.ptr(vp["part"]), _lib.ptr(out[0]), _lib.ptr(out[1]), y3,
+                      n if (instance_major and nrhs > 1) else 0, int(cfg["p1_depth"]),
+                      int(cfg["p1_waves"]), int(cfg["depth"]), int(cfg["waves"]),
+                      _lib.stream_handle(stream))
         elif mfma:
             _lib.call("pq3d_hmat_matvec_mfma", *args, _lib.ptr(self._x32(nrhs)), n,
                       int(self.accumulate == "mfma_comp"), _lib.stream_handle(stream))
@@ -673,10 +743,41 @@ class HMatrix:
         cfg = getattr(self, "tuning", None)
         return dict(MFS_DEFAULT) if cfg is None else cfg
 
+    def _vpanels(self) -> dict:
+        """The matrix-core product's V panels (built and packed on the device
+        on first use, ``vpanel_plan``): panel storage, phase-1 units and
+        phase-1b rows for 16 right-hand sides, partial rows, and the t-row
+        numbering of every low-rank block."""
+        vp = getattr(self, "_vp", None)
+        if vp is None:
+            B = self.blocks
+            plan = vpanel_plan(B["c0"], B["n"], B["lr_idx"], B["ranks"], B["l_vb"])
+            if plan["nt"] != self.nt:
+                raise RuntimeError("V panels and work lists disagree on the ranks")
+            dev = self.device
+            vp = {"vp": torch.empty(max(plan["size"], 1), dtype=torch.float32, device=dev)}
+            tab = torch.from_numpy(plan["tab"]).to(dev)
+            _lib.check_width("vpanel", tab)
+            _lib.call("pq3d_hmat_vpanel_pack", _lib.ptr(tab), len(plan["tab"]),
+                      _lib.ptr(self.pool), _lib.ptr(vp["vp"]), _lib.stream_handle())
+            vp["units"] = torch.from_numpy(plan["units"]).to(dev)
+            _lib.check_width("punit", vp["units"])
+            vp["nunits"] = len(plan["units"])
+            red = reduce_rows(plan["reduce"], 16)
+            vp["red"] = torch.from_numpy(red).to(dev)
+            vp["nred"] = len(red)
+            vp["part"] = torch.zeros(max(plan["npart"], 1) * 16, dtype=torch.float32, device=dev)
+            vp["toff"] = plan["toff"]
+            vp["bytes"] = plan["size"] * 4
+            _sync(dev)
+            self._vp = vp
+        return vp
+
     def _mfs_lists(self, cfg=None):
         """Device step stream (steps, wptr, tiles) of the matrix-core product
         for the tuning ``cfg`` (default: the current one), built once per
-        (tile order, wavefronts per tile)."""
+        (tile order, wavefronts per tile). Its t rows follow the V panels'
+        numbering."""
         cfg = self._tuning() if cfg is None else cfg
         cache = getattr(self, "_mfs_cache", None)
         if cache is None:
@@ -685,7 +786,16 @@ class HMatrix:
         if key not in cache:
             host = getattr(self, "_wl_host", None)
             if host is None:
-                host = self._wl_host = (self.items.cpu().numpy(), self.tile_ptr.cpu().numpy(),
+                items = self.items.cpu().numpy().copy()
+                # U items address t by the work lists' numbering (blocks in
+                # lr order); renumber to the panels'
+                k = np.asarray(self.blocks["ranks"], dtype=np.int64)
+                old = np.concatenate([[0], np.cumsum(k)[:-1]]).astype(np.int64)
+                new = self._vpanels()["toff"]
+                u = items[:, 0] == 1
+                b = np.searchsorted(old, items[u, 5], side="right") - 1
+                items[u, 5] = new[b] + (items[u, 5] - old[b])
+                host = self._wl_host = (items, self.tile_ptr.cpu().numpy(),
                                         self.tiles.cpu().numpy())
             st, wp, tl = mfs_steps(*host, self.n, self.nt, waves=key[1], order=key[0])
             dev = self.device
@@ -694,17 +804,19 @@ class HMatrix:
         return cache[key]
 
     def tune(self, planes=2, reps=10, depths=MFS_DEPTHS, waves=(1, 2), orders=MFS_ORDERS,
-             seed=0) -> dict:
-        """Autotuner of the 16-RHS matrix-core product: times every
-        candidate (steps in flight per wavefront, wavefronts per row tile,
-        tile launch order) on this operator with HIP events (``reps``
-        products each, after two warm-up products, ``planes`` row planes,
-        seeded inputs) and keeps the fastest as ``tuning``. A choice changes
-        only the FP32 summation order within a tile. Returns {"choice",
-        "planes", "candidates": [(config, ms)]}, also kept as
+             p1_depths=MFP_DEPTHS, p1_waves=MFP_WAVES, seed=0) -> dict:
+        """Autotuner of the 16-RHS matrix-core product, timed on this
+        operator with HIP events (``reps`` products per candidate after two
+        warm-up products, ``planes`` row planes, seeded inputs): first every
+        phase-2 candidate (steps in flight per wavefront, wavefronts per row
+        tile, tile launch order) with the default phase 1, then every
+        phase-1 candidate (steps in flight, wavefronts per unit) with the
+        chosen phase 2; the fastest is kept as ``tuning``. A choice changes
+        only the FP32 summation order within a tile or unit. Returns
+        {"choice", "planes", "candidates": [(config, ms)]}, also kept as
         ``tuning_report``."""
-        if self.esize != 4 or self.min_tile_rows < 4:
-            raise ValueError("the matrix-core product needs FP32 storage and tiles of >= 4 rows")
+        if not isinstance(self, HMatrix) or self.esize != 4 or self.min_tile_rows < 4:
+            raise ValueError("the matrix-core product needs an FP32 HMatrix with tiles of >= 4 rows")
         n, nb, dev = self.n, 16, self.device
         gen = torch.Generator(device=dev)
         gen.manual_seed(seed)
@@ -714,22 +826,32 @@ class HMatrix:
         acc0, cfg0 = self.accumulate, getattr(self, "tuning", None)
         self.accumulate = "mfma"
         results = []
+
+        def run(cfg):
+            self.tuning = dict(cfg)
+            for _ in range(2):
+                self.matvec(v1, v2, out=out, nrhs=nb, planes=planes)
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(reps)]
+            for a, b in ev:
+                a.record()
+                self.matvec(v1, v2, out=out, nrhs=nb, planes=planes)
+                b.record()
+            torch.cuda.synchronize(dev)
+            ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+            results.append((dict(cfg), ms))
+            return ms
+
         try:
-            for order in orders:
-                for w in waves:
-                    for d in depths:
-                        self.tuning = {"depth": int(d), "waves": int(w), "order": order}
-                        for _ in range(2):
-                            self.matvec(v1, v2, out=out, nrhs=nb, planes=planes)
-                        ev = [(torch.cuda.Event(enable_timing=True),
-                               torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-                        for a, b in ev:
-                            a.record()
-                            self.matvec(v1, v2, out=out, nrhs=nb, planes=planes)
-                            b.record()
-                        torch.cuda.synchronize(dev)
-                        ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-                        results.append((dict(self.tuning), ms))
+            base = dict(MFS_DEFAULT)
+            p2 = [dict(base, order=o, waves=int(w), depth=int(d))
+                  for o in orders for w in waves for d in depths]
+            best = min(p2, key=run)
+            p1 = [dict(best, p1_depth=int(d), p1_waves=int(w))
+                  for w in p1_waves for d in p1_depths
+                  if (int(d), int(w)) != (best["p1_depth"], best["p1_waves"])]
+            for cfg in p1:
+                run(cfg)
         finally:
             self.accumulate = acc0
             self.tuning = cfg0
@@ -796,7 +918,8 @@ class HMatrix:
                   "ranks_max": str(st.ranks_max), "fallback_dense": str(st.fallback_dense)}
         cfg = getattr(self, "tuning", None)
         if cfg is not None:
-            header["mfs_tuning"] = f"{cfg['depth']},{cfg['waves']},{cfg['order']}"
+            header["mfs_tuning"] = (f"{cfg['depth']},{cfg['waves']},{cfg['order']},"
+                                    f"{cfg['p1_depth']},{cfg['p1_waves']}")
         header.update({"user_" + k: str(v) for k, v in meta.items()})
         save_file(tens, str(path), metadata=header)
 
@@ -822,10 +945,17 @@ class HMatrix:
         self.mu, self.lambda_ = float(h["mu"]), float(h["lambda"])
         self.meta = {k[5:]: v for k, v in h.items() if k.startswith("user_")}
         if "mfs_tuning" in h:
-            d, w, o = h["mfs_tuning"].split(",")
-            if int(d) not in MFS_DEPTHS or int(w) not in MFS_WAVES or o not in MFS_ORDERS:
+            f5 = h["mfs_tuning"].split(",")
+            try:
+                cfg = {"depth": int(f5[0]), "waves": int(f5[1]), "order": f5[2],
+                       "p1_depth": int(f5[3]), "p1_waves": int(f5[4])}
+            except (IndexError, ValueError):
+                raise ValueError(f"{path}: bad mfs_tuning {h['mfs_tuning']!r}") from None
+            if (len(f5) != 5 or cfg["depth"] not in MFS_DEPTHS or cfg["waves"] not in MFS_WAVES
+                    or cfg["order"] not in MFS_ORDERS or cfg["p1_depth"] not in MFP_DEPTHS
+                    or cfg["p1_waves"] not in MFP_WAVES):
                 raise ValueError(f"{path}: bad mfs_tuning {h['mfs_tuning']!r}")
-            self.tuning = {"depth": int(d), "waves": int(w), "order": o}
+            self.tuning = cfg
         self.pool = t["pool"].to(dev)
         for k in ("units", "members", "reduce", "tiles", "tile_ptr", "items"):
             setattr(self, k, t[k].to(dev))
@@ -953,8 +1083,6 @@ class MultiHMatrix:
     matvec = HMatrix.matvec
     _x32 = HMatrix._x32
     _tuning = HMatrix._tuning
-    _mfs_lists = HMatrix._mfs_lists
-    tune = HMatrix.tune
 
     def algorithmic_bytes(self, nrhs=1, planes=3):
         """``HMatrix.algorithmic_bytes`` summed over the instances (one

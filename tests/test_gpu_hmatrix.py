@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 import torch
 
-from pyquake3d_amd.hmatrix import HMatrix
+from pyquake3d_amd.hmatrix import MFS_DEFAULT, HMatrix
 from pyquake3d_amd.mesh import get_eleVec, undulating_fault
 
 pytestmark = pytest.mark.gpu
@@ -173,10 +173,11 @@ def test_mfma_matvec_16_rhs(gpu, pq3d_lib, case, leaf, mode):
 
 @pytest.mark.parametrize("planes", [2, 3])
 def test_step_stream_configurations_and_autotuner(gpu, pq3d_lib, case, planes, tmp_path):
-    """Every configuration of the step-stream product (steps in flight,
-    wavefronts per tile, tile order) stays within 1e-5 of the
+    """Every configuration of the step-stream product (phase 2: steps in
+    flight, wavefronts per tile, tile order; phase 1 over the V panels:
+    steps in flight, wavefronts per unit) stays within 1e-5 of the
     FP64-accumulated product of the same storage and is repeatable; the
-    autotuner times them all, keeps the fastest, and the tuned choice
+    autotuner times its candidates, keeps the fastest, and the tuned choice
     survives save/load."""
     mesh, _ = case
     H = HMatrix.from_mesh(mesh, MU, LAM, leaf=32, eta=2.0, eps=1e-4, dtype=torch.float32,
@@ -187,18 +188,19 @@ def test_step_stream_configurations_and_autotuner(gpu, pq3d_lib, case, planes, t
     X2 = torch.from_numpy(rng.standard_normal(n * 16)).to(gpu)
     y64 = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()
     H.accumulate = "mfma"
-    for order in ("row", "cost"):
-        for waves in (1, 2, 4):
-            for depth in (2, 4):
-                H.tuning = {"depth": depth, "waves": waves, "order": order}
-                y = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes]
-                assert torch.equal(y, H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes])
-                y = y.cpu().numpy()
-                for b in range(16):
-                    assert _relerr(y[:, :, b], y64[:, :, b]) < 1e-5, (order, waves, depth, b)
+    cfgs = [dict(MFS_DEFAULT, order=o, waves=w, depth=d)
+            for o in ("row", "cost") for w in (1, 2, 4) for d in (2, 4)]
+    cfgs += [dict(MFS_DEFAULT, p1_depth=d, p1_waves=w) for d in (2, 4) for w in (2, 4)]
+    for cfg in cfgs:
+        H.tuning = cfg
+        y = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes]
+        assert torch.equal(y, H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes])
+        y = y.cpu().numpy()
+        for b in range(16):
+            assert _relerr(y[:, :, b], y64[:, :, b]) < 1e-5, (cfg, b)
     H.tuning = None
     rep = H.tune(planes=planes, reps=3)
-    assert len(rep["candidates"]) == 8
+    assert len(rep["candidates"]) == 11
     assert rep["choice"] == min(rep["candidates"], key=lambda c: c[1])[0]
     assert H.tuning == rep["choice"] and H.accumulate == "mfma"
     y = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()
