@@ -68,7 +68,7 @@ YEAR = 3.15576e7
 # for a 2 GiB device-to-device copy (2 GiB read as 1,048,588 KiB fetched,
 # 2 GiB written as 2,097,152 KiB). Key: (storage, right-hand sides, row
 # planes, accumulation); N = 100,352 (profiles/hmatvec_traffic_N100352.md).
-MEASURED_TRAFFIC = {("f32", 16, 2, "mfma"): 5_349_731_072}
+MEASURED_TRAFFIC = {("f32", 16, 2, "mfma"): 5_282_207_854}
 
 # FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
 # (unit strike and dip slip, both projected), counted operator by operator
@@ -290,6 +290,19 @@ def cpu_baseline_hstep(problem, H, gpu_res, nsteps=3):
                       f"{gpu_res['sim_years']:.1f} simulated years; host setup {setup:.1f} s"}
 
 
+def tune_product(H, args, planes):
+    """The autotuner's choice for the matrix-core product (HMatrix.tune), or
+    the configuration given with --tuning; None for other products."""
+    if H.accumulate != "mfma" or H.esize != 4:
+        return None
+    if args.tuning:
+        d, w, o, p1d, p1w = args.tuning.split(",")
+        H.tuning = {"depth": int(d), "waves": int(w), "order": o, "p1_depth": int(p1d),
+                    "p1_waves": int(p1w)}
+        return {"choice": H.tuning, "planes": planes, "candidates": []}
+    return H.tune(planes=planes)
+
+
 def load_step_mlp(dev):
     from pyquake3d_amd.learned_dt import StepMLP
     if not STEP_MLP.exists():
@@ -333,8 +346,8 @@ def bench_hstep(args):
     torch.cuda.empty_cache()
     H32, b32 = build_hmatrix(problem, torch.float32)
     H32.accumulate = args.accumulate
-    tuning = H32.tune(planes=2) if args.accumulate == "mfma" else None
-    if tuning:
+    tuning = tune_product(H32, args, 2)
+    if tuning and tuning["candidates"]:
         log(f"hstep: autotuner chose {tuning['choice']} "
             f"({min(ms for _, ms in tuning['candidates']):.4f} ms)")
     fast, _ = run_cycles(draws, H32, f"fp32_nb{args.nb}" + ("_learned" if mlp else ""), t_end,
@@ -436,8 +449,7 @@ def bench_matvec(args):
     dtype = torch.float32 if args.dtype == "f32" else torch.float64
     H, build_s = build_hmatrix(problem, dtype)
     H.accumulate = args.accumulate
-    tuning = (H.tune(planes=args.planes) if args.accumulate == "mfma" and args.nb == 16 and
-              args.dtype == "f32" else None)
+    tuning = tune_product(H, args, args.planes) if args.nb == 16 else None
     n, nb, dev = H.n, args.nb, H.device
     gen = torch.Generator(device=dev)
     gen.manual_seed(0)
@@ -569,8 +581,7 @@ def bench_learned(args):
     draws = c3_draws(args.nb)
     H, _ = build_hmatrix(draws[0], torch.float32)
     H.accumulate = args.accumulate
-    if args.accumulate == "mfma":
-        H.tune(planes=2)
+    tune_product(H, args, 2)
     plain, _ = run_cycles(draws, H, "plain_controller", args.cycle_years, args.warmup,
                           max_wall=args.max_wall)
     learned, _ = run_cycles(draws, H, "learned_controller", args.cycle_years, args.warmup,
@@ -1072,6 +1083,9 @@ def main():
                     help="hstep: the batched variant with the reference controller")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="hstep: skip the FP64 reference run of every draw")
+    ap.add_argument("--tuning", default=None,
+                    help="hstep/matvec/learned: fixed matrix-core configuration "
+                         "'depth,waves,order,p1_depth,p1_waves' instead of the autotuner")
     ap.add_argument("--noise-floor", action="store_true",
                     help="hstep: also rerun the FP64 draws from dt = 2e-3 s (event-onset "
                          "spread of the reference method itself)")

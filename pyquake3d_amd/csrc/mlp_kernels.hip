@@ -155,6 +155,21 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float* __restrict__ prm,
 constexpr int kFeatMaxInst = 1024;
 constexpr double kFeatScale = 4294967296.0;  // 2^32
 
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 __global__ void __launch_bounds__(256) k_features(int64_t ne, int nb,
                                                   const int32_t* __restrict__ inst,
                                                   const double* __restrict__ slipv,
@@ -165,18 +180,49 @@ __global__ void __launch_bounds__(256) k_features(int64_t ne, int nb,
   __shared__ unsigned long long sh[4 * kFeatMaxInst];
   for (int q = threadIdx.x; q < 4 * nb; q += blockDim.x) sh[q] = 0ull;
   __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
-    const int b = inst ? inst[e] : 0;
-    const double V = slipv[e];
-    double lv = log10(V);
-    lv = fmax(fmin(lv, 999.0), -999.0);
-    // log10(V theta / Dc) = log10(V/V0) + (psi - f0)/(b ln 10)
-    const double lvt = log10(V / V0) + (psi[e] - pf0[e]) / (pb[e] * 2.302585092994046);
-    atomicMax(&sh[4 * b + 0], (unsigned long long)__double_as_longlong(1000.0 + lv));
-    atomicMax(&sh[4 * b + 1], (unsigned long long)__double_as_longlong(1000.0 - lv));
-    atomicAdd(&sh[4 * b + 2], (unsigned long long)llrint(lvt * kFeatScale));
-    if (V > 1e-3) atomicAdd(&sh[4 * b + 3], 1ull);
+  const int lane = threadIdx.x & 63;
+  const int64_t wstride = (int64_t)gridDim.x * blockDim.x;
+  // every wavefront takes 64 consecutive cells at a time; a wavefront whose
+  // cells all belong to one instance (the usual case: cells are stored
+  // instance-major) reduces them in registers and adds them to LDS with one
+  // atomic per slot, a mixed one adds cell by cell. Maxima and integer sums
+  // are exact, so both orders give the same slots.
+  for (int64_t e0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); e0 < ne;
+       e0 += wstride) {
+    const int64_t e = e0 + lane;
+    const bool valid = e < ne;
+    const int b = valid ? (inst ? inst[e] : 0) : -1;
+    unsigned long long kmax = 0ull, kmin = 0ull;
+    long long ksum = 0;
+    bool fast = false;
+    if (valid) {
+      const double V = slipv[e];
+      double lv = log10(V);
+      lv = fmax(fmin(lv, 999.0), -999.0);
+      // log10(V theta / Dc) = log10(V/V0) + (psi - f0)/(b ln 10)
+      const double lvt = log10(V / V0) + (psi[e] - pf0[e]) / (pb[e] * 2.302585092994046);
+      kmax = (unsigned long long)__double_as_longlong(1000.0 + lv);
+      kmin = (unsigned long long)__double_as_longlong(1000.0 - lv);
+      ksum = llrint(lvt * kFeatScale);
+      fast = V > 1e-3;
+    }
+    const int b0 = __builtin_amdgcn_readfirstlane(b);
+    if (__all(b == b0)) {
+      const unsigned long long mx = wave_max_u64(kmax), mn = wave_max_u64(kmin);
+      const long long sm = wave_sum_i64(ksum);
+      const unsigned long long ct = __popcll(__ballot(fast));
+      if (lane == 0) {
+        atomicMax(&sh[4 * b0 + 0], mx);
+        atomicMax(&sh[4 * b0 + 1], mn);
+        atomicAdd(&sh[4 * b0 + 2], (unsigned long long)sm);
+        if (ct) atomicAdd(&sh[4 * b0 + 3], ct);
+      }
+    } else if (valid) {
+      atomicMax(&sh[4 * b + 0], kmax);
+      atomicMax(&sh[4 * b + 1], kmin);
+      atomicAdd(&sh[4 * b + 2], (unsigned long long)ksum);
+      if (fast) atomicAdd(&sh[4 * b + 3], 1ull);
+    }
   }
   __syncthreads();
   for (int q = threadIdx.x; q < 4 * nb; q += blockDim.x) {
