@@ -253,15 +253,19 @@ def event_errors(fast, ref):
     return out
 
 
-def cpu_baseline_hstep(problem, H, gpu_res, nsteps=3):
-    """Reference-style CPU step (per-block numpy FP64 matvec of the four A
-    operators, as the reference applies with a fixed normal traction, and the
-    numpy RK45 step) on the same blocks, timed over ``nsteps`` steps from
-    the initial state; the rate is expressed for the GPU run's simulated
-    interval and accepted steps."""
+def cpu_baseline_hstep(problem, H, gpu_res, nsteps=3, nproc=8):
+    """Reference-style CPU step on the same blocks, timed over ``nsteps``
+    steps from the initial state: the per-block numpy FP64 matvec of the
+    four A operators (as the reference applies them with a fixed normal
+    traction) and the numpy RK45 step, once on one process (one BLAS
+    thread) and once with the blocks scattered over ``nproc`` processes as
+    the reference's MPI ranks hold them (partial products summed); the
+    rates are expressed for the plain GPU run's simulated interval and
+    accepted steps. ``value`` is the ``nproc``-process rate."""
     import numpy as np
+    from threadpoolctl import threadpool_limits
 
-    from oracle.hmat_cpu import CpuHMatrix
+    from oracle.hmat_cpu import CpuHMatrix, ScatterMatvec
     from oracle.qdsim_oracle import OracleSim
     t0 = time.perf_counter()
     C = CpuHMatrix(H)
@@ -270,24 +274,37 @@ def cpu_baseline_hstep(problem, H, gpu_res, nsteps=3):
     init = [np.asarray(x)[perm] for x in problem.initial_state()]
     g = lambda x: np.asarray(x)[perm]
     planes = 2 if problem.fix_Tn else 3
-    ora = OracleSim(None, g(problem.a), g(problem.b), g(problem.dc), g(problem.f0), problem.V0,
-                    problem.mu, problem.Cs, g(problem.Vpl), g(problem.rake), init,
-                    normal_loading=g(problem.normal_loading), fix_Tn=problem.fix_Tn,
-                    matvec=lambda v1, v2: C.matvec(v1, v2, planes=planes))
-    dt = 1e-3
-    t0 = time.perf_counter()
-    for _ in range(nsteps):
-        _, dt = ora.simu_forward(dt)
-    per_step = (time.perf_counter() - t0) / nsteps
     steps = gpu_res["accepted_steps"][0]
-    cpu_hours = per_step * steps / 3600.0
-    return {"value": gpu_res["sim_years"] / cpu_hours, "unit": "sim-years/wall-hour",
-            "cores": 1, "kind": "port",
+
+    def timed(matvec):
+        ora = OracleSim(None, g(problem.a), g(problem.b), g(problem.dc), g(problem.f0),
+                        problem.V0, problem.mu, problem.Cs, g(problem.Vpl), g(problem.rake), init,
+                        normal_loading=g(problem.normal_loading), fix_Tn=problem.fix_Tn,
+                        matvec=matvec)
+        dt = 1e-3
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            _, dt = ora.simu_forward(dt)
+        return (time.perf_counter() - t0) / nsteps
+
+    with threadpool_limits(limits=1):
+        one = timed(lambda v1, v2: C.matvec(v1, v2, planes=planes))
+        S = ScatterMatvec(C, nproc)
+        try:
+            many = timed(lambda v1, v2: S.matvec(v1, v2, planes=planes))
+        finally:
+            S.close()
+    rate = lambda per_step: gpu_res["sim_years"] / (per_step * steps / 3600.0)
+    blocks = f"{len(C.dense)} dense + {len(C.lowrank)} low-rank blocks"
+    return {"value": rate(many), "unit": "sim-years/wall-hour", "cores": nproc, "kind": "port",
             "sample": f"{nsteps} steps of the reference-style CPU step (per-block numpy FP64 "
-                      f"matvec of the {2 * planes} operators over the same {len(C.dense)} dense "
-                      f"+ {len(C.lowrank)} low-rank blocks, numpy RK45) at {per_step:.2f} "
-                      f"s/step, applied to the plain GPU run's {steps} accepted steps / "
-                      f"{gpu_res['sim_years']:.1f} simulated years; host setup {setup:.1f} s"}
+                      f"matvec of the {2 * planes} operators over the same {blocks}, scattered "
+                      f"over {nproc} processes with one BLAS thread each and reduced, numpy "
+                      f"RK45) at {many:.3f} s/step, applied to the plain GPU run's {steps} "
+                      f"accepted steps / {gpu_res['sim_years']:.1f} simulated years; host "
+                      f"setup {setup:.1f} s",
+            "single_process": {"value": rate(one), "cores": 1, "s_per_step": one},
+            "s_per_step": many}
 
 
 def tune_product(H, args, planes):
@@ -420,6 +437,64 @@ def bench_hstep(args):
     return res
 
 
+def per_block_errors(H, mesh, nsample=1000, seed=0, max_entries=4_000_000):
+    """Relative Frobenius error of the stored blocks of H against the same
+    blocks assembled in FP64 by the TDE kernel: up to ``nsample`` dense
+    (near-field) and ``nsample`` low-rank blocks drawn with ``seed``, blocks
+    with m n <= ``max_entries``. A dense block's error is the rounding of its
+    storage type; a low-rank block's adds the ACA truncation (eps) and the
+    rounding of U and V. Returns quantiles per kind."""
+    import numpy as np
+    import torch
+
+    from pyquake3d_amd.tde import DeviceMesh
+    dm = DeviceMesh.from_mesh(mesh, H.mu, H.lambda_, device=H.device)
+    B = H.blocks
+    rng = np.random.default_rng(seed)
+    m_all, n_all = np.asarray(B["m"]), np.asarray(B["n"])
+    perm = H.perm
+
+    def exact(bi):
+        r = perm[B["r0"][bi]:B["r0"][bi] + B["m"][bi]]
+        c = perm[B["c0"][bi]:B["c0"][bi] + B["n"][bi]]
+        T = dm.traction_block(r, c).cpu().numpy()
+        return T
+
+    out = {}
+    di = np.asarray(B["dense_idx"])
+    ok = np.where(m_all[di] * n_all[di] <= max_entries)[0]
+    pick = rng.choice(ok, size=min(nsample, len(ok)), replace=False)
+    errs = []
+    for q in pick:
+        bi, base = int(di[q]), int(B["d_base"][q])
+        m, n = int(m_all[bi]), int(n_all[bi])
+        T = exact(bi)
+        St = H.pool[base:base + 6 * m * n].double().cpu().numpy().reshape(6, n, m).transpose(0, 2, 1)
+        errs.append(np.linalg.norm(St - T) / np.linalg.norm(T))
+    out["dense"] = errs
+    li = np.asarray(B["lr_idx"])
+    ok = np.where(m_all[li] * n_all[li] <= max_entries)[0]
+    pick = rng.choice(ok, size=min(nsample, len(ok)), replace=False)
+    errs = []
+    for q in pick:
+        bi = int(li[q])
+        T = exact(bi)
+        S = np.block([[T[0], T[3]], [T[1], T[4]], [T[2], T[5]]])
+        U, V = H.lowrank_factors(int(q))
+        errs.append(np.linalg.norm(U @ V - S) / np.linalg.norm(S))
+    out["lowrank"] = errs
+    res = {}
+    for kind, e in out.items():
+        e = np.asarray(e)
+        res[kind] = {"blocks": int(len(e)), "max": float(e.max()) if len(e) else None,
+                     "p50": float(np.median(e)) if len(e) else None,
+                     "p99": float(np.quantile(e, 0.99)) if len(e) else None}
+    res["storage"] = str(H.dtype).replace("torch.", "")
+    res["note"] = ("relative Frobenius error per block against the FP64 TDE assembly of the "
+                   "same block; sampled blocks")
+    return res
+
+
 # ------------------------------------------------------------- matvec
 def bench_matvec(args):
     """One H-matvec of --nb right-hand sides per step on the hstep operator,
@@ -513,6 +588,8 @@ def bench_matvec(args):
         "build_s": build_s,
         "mfma_rel_err_vs_f64_accumulation": mf_err,
         "tuning": tuning,
+        "per_block_error": (per_block_errors(H, problem.mesh, nsample=args.block_sample)
+                            if args.block_sample > 0 else None),
     }
 
 
@@ -1083,6 +1160,9 @@ def main():
                     help="hstep: the batched variant with the reference controller")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="hstep: skip the FP64 reference run of every draw")
+    ap.add_argument("--block-sample", type=int, default=500,
+                    help="matvec: blocks of each kind whose stored entries are compared with "
+                         "the FP64 assembly (0: none)")
     ap.add_argument("--tuning", default=None,
                     help="hstep/matvec/learned: fixed matrix-core configuration "
                          "'depth,waves,order,p1_depth,p1_waves' instead of the autotuner")

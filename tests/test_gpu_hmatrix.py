@@ -337,3 +337,46 @@ def test_wide_aca_at_config_size(gpu, pq3d_lib):
           f"{H1.stats.build_s['aca']:.3f} s")
     assert np.array_equal(H1.blocks["ranks"], ranks)
     assert torch.equal(H1.pool.cpu(), pool)
+
+
+def test_fp32_storage_error_per_block(gpu, pq3d_lib, case):
+    """Per-block error statement of the FP32 storage against the FP64 dense
+    oracle: every near-field block within FP32 rounding (relative Frobenius
+    error <= 1e-6), every low-rank block within the ACA bound (10 eps, the
+    same allowance as the FP64 factors) and at most FP32 rounding above the
+    FP64 factors of the same block."""
+    mesh, K = case
+    eps = 1e-4
+    H = HMatrix.from_mesh(mesh, MU, LAM, leaf=32, eta=2.0, eps=eps, dtype=torch.float32,
+                          device=gpu)
+    H64 = HMatrix.from_mesh(mesh, MU, LAM, leaf=32, eta=2.0, eps=eps, dtype=torch.float64,
+                            device=gpu)
+    B = H.blocks
+    perm = H.perm
+    assert np.array_equal(B["lr_idx"], H64.blocks["lr_idx"])
+
+    def exact(bi):
+        r = perm[B["r0"][bi]:B["r0"][bi] + B["m"][bi]]
+        c = perm[B["c0"][bi]:B["c0"][bi] + B["n"][bi]]
+        return [K[o][np.ix_(r, c)] for o in range(6)]
+
+    worst_dense = 0.0
+    for q, bi in enumerate(B["dense_idx"]):
+        m, n, base = int(B["m"][bi]), int(B["n"][bi]), int(B["d_base"][q])
+        St = H.pool[base:base + 6 * m * n].double().cpu().numpy().reshape(6, n, m)
+        T = np.stack(exact(bi))
+        worst_dense = max(worst_dense,
+                          np.linalg.norm(St.transpose(0, 2, 1) - T) / np.linalg.norm(T))
+    assert worst_dense < 1e-6, worst_dense
+    worst_lr = worst_gap = 0.0
+    for q, bi in enumerate(B["lr_idx"]):
+        T = exact(bi)
+        S = np.block([[T[0], T[3]], [T[1], T[4]], [T[2], T[5]]])
+        U, V = H.lowrank_factors(q)
+        U64, V64 = H64.lowrank_factors(q)
+        e32 = np.linalg.norm(U @ V - S) / np.linalg.norm(S)
+        e64 = np.linalg.norm(U64 @ V64 - S) / np.linalg.norm(S)
+        worst_lr = max(worst_lr, e32)
+        worst_gap = max(worst_gap, abs(e32 - e64))
+    assert worst_lr < 10 * eps, worst_lr
+    assert worst_gap < 1e-6, worst_gap

@@ -98,3 +98,25 @@ def test_low_rank_blocks_within_their_truncation(oracle):
     yc = CpuHMatrix(hc).matvec(v1[:h.n], v2[:h.n])
     err = np.abs(yc - ref).max() / np.abs(ref).max()
     assert 1e-7 < err < 1e-1
+
+
+@pytest.mark.parametrize("nproc", [1, 3])
+def test_scattered_product_equals_single_process(oracle, nproc):
+    """The block scatter over host processes (blocks dealt to ranks, partial
+    products summed) gives the single-process product to rounding, for
+    three and two row planes, on repeated calls."""
+    from oracle.hmat_cpu import ScatterMatvec
+    rng = np.random.default_rng(4)
+    h, _ = host_hmatrix(oracle, eta=2.0, rel_cut=1e-6)
+    C = CpuHMatrix(h)
+    S = ScatterMatvec(C, nproc)
+    try:
+        for planes in (3, 2, 3):
+            v1, v2 = rng.standard_normal(h.n), rng.standard_normal(h.n)
+            ref = C.matvec(v1, v2, planes=planes)
+            got = S.matvec(v1, v2, planes=planes)
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+            if planes == 2:
+                assert not got[2].any()
+    finally:
+        S.close()
