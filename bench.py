@@ -133,6 +133,13 @@ STEP_MLP = REPO / "pyquake3d_amd" / "data" / "step_mlp_c2.npz"
 # margin 0 took 57,300 iterations, -0.03 56,100, -0.06 54,500; the reference
 # controller 57,300.
 LEARNED_MARGIN = -0.06
+# attempt rounds per iteration of the device controller in the cycle runs: a
+# draw that needs more attempts continues in the next iteration (same
+# trajectory), an empty round still costs two launches. Over [0, 120] yr of
+# the 16 draws (learned proposal, matrix-core product): 2 rounds 24,800
+# iterations / 31.4 s, 3: 24,000 / 30.8 s, 4: 23,800 / 30.8 s, 6: 23,600 /
+# 31.0 s, 8: 23,600 / 31.2 s.
+CYCLE_ROUNDS = 4
 
 
 def c3_draws(nb, nx=448, nz=112, dc_scale=C3_DC_SCALE):
@@ -159,7 +166,8 @@ def merge_events(ev_s):
     return out
 
 
-def run_cycles(problems, H, name, t_end_years, warmup, rounds=8, mlp=None, timing_every=0,
+def run_cycles(problems, H, name, t_end_years, warmup, rounds=CYCLE_ROUNDS, mlp=None,
+               timing_every=0,
                check_every=100, max_wall=900.0, margin=0.0, hi=0.0, dt0=1e-3):
     """``problems`` (friction draws on the operator's mesh) stepped by the
     device controller from their initial state until every draw reaches
@@ -404,7 +412,7 @@ def bench_hstep(args):
                                f"normal traction fixed (reference Fix_Tn), every draw "
                                f"integrated over [0, {t_end:g}] yr (>= 3 whole cycles) "
                                f"whatever --steps is; a step: one iteration of the device "
-                               f"controller (one product, up to 8 attempt rounds, commit)",
+                               f"controller (one product, up to {CYCLE_ROUNDS} attempt rounds, commit)",
                    "variant": f"fp32+batched+learned: {args.nb} friction draws sharing the "
                               f"operator, one {args.nb}-RHS H-matvec per step, learned "
                               f"first-attempt step size (RK45 estimator accepts or rejects, PI "
@@ -677,7 +685,7 @@ def bench_learned(args):
                                f"controller ({args.nb} draws, N=20000, {args.record} "
                                f"iterations), evaluated over [0, {args.cycle_years:g}] yr of "
                                f"{args.nb} draws at N={draws[0].n}, rtol 1e-4, FP32 H-matrix, "
-                               f"device controller, 8 attempt rounds per iteration, proposal "
+                               f"device controller, {CYCLE_ROUNDS} attempt rounds per iteration, proposal "
                                f"margin {LEARNED_MARGIN}"},
         "rejected_fraction": {"reference": plain["rejected_fraction"],
                               "learned": learned["rejected_fraction"]},
