@@ -178,19 +178,24 @@ __global__ void __launch_bounds__(256) k_features(int64_t ne, int nb,
                                                   const double* __restrict__ pf0, double V0,
                                                   unsigned long long* __restrict__ acc) {
   __shared__ unsigned long long sh[4 * kFeatMaxInst];
+  __shared__ int touched[kFeatMaxInst];
   for (int q = threadIdx.x; q < 4 * nb; q += blockDim.x) sh[q] = 0ull;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) touched[q] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int64_t wstride = (int64_t)gridDim.x * blockDim.x;
-  // every wavefront takes 64 consecutive cells at a time; a wavefront whose
-  // cells all belong to one instance (the usual case: cells are stored
-  // instance-major) reduces them in registers and adds them to LDS with one
-  // atomic per slot, a mixed one adds cell by cell. Maxima and integer sums
-  // are exact, so both orders give the same slots.
-  for (int64_t e0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); e0 < ne;
-       e0 += wstride) {
+  // every workgroup takes one contiguous range of cells, so (cells being
+  // stored instance-major) it touches one or two instances and adds only
+  // their slots to the global ones; every wavefront takes 64 consecutive
+  // cells at a time and, when they all belong to one instance (the usual
+  // case), reduces them in registers with one LDS atomic per slot, else adds
+  // cell by cell. Maxima and integer sums are exact, so any order gives the
+  // same slots.
+  const int64_t chunk = (ne + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < ne ? lo + chunk : ne;
+  for (int64_t e0 = lo + (threadIdx.x & ~63); e0 < hi; e0 += blockDim.x) {
     const int64_t e = e0 + lane;
-    const bool valid = e < ne;
+    const bool valid = e < hi;
     const int b = valid ? (inst ? inst[e] : 0) : -1;
     unsigned long long kmax = 0ull, kmin = 0ull;
     long long ksum = 0;
@@ -216,20 +221,23 @@ __global__ void __launch_bounds__(256) k_features(int64_t ne, int nb,
         atomicMax(&sh[4 * b0 + 1], mn);
         atomicAdd(&sh[4 * b0 + 2], (unsigned long long)sm);
         if (ct) atomicAdd(&sh[4 * b0 + 3], ct);
+        touched[b0] = 1;
       }
     } else if (valid) {
       atomicMax(&sh[4 * b + 0], kmax);
       atomicMax(&sh[4 * b + 1], kmin);
       atomicAdd(&sh[4 * b + 2], (unsigned long long)ksum);
       if (fast) atomicAdd(&sh[4 * b + 3], 1ull);
+      touched[b] = 1;
     }
   }
   __syncthreads();
   for (int q = threadIdx.x; q < 4 * nb; q += blockDim.x) {
+    if (!touched[q >> 2]) continue;
     const int slot = q & 3;
     if (slot < 2)
       atomicMax(&acc[q], sh[q]);
-    else
+    else if (sh[q])
       atomicAdd(&acc[q], sh[q]);
   }
 }
@@ -363,7 +371,7 @@ int pq3d_features(int64_t ne, int nb, const int32_t* d_inst, const double* d_sli
   k_feat_clear<<<blocks_for(4 * nb, 256), 256, 0, s>>>((unsigned long long*)d_acc, 4 * nb);
   if (ne > 0) {
     const int64_t want = (ne + 255) / 256;
-    const unsigned grid = (unsigned)(want < 512 ? want : 512);
+    const unsigned grid = (unsigned)(want < 2048 ? want : 2048);
     k_features<<<grid, 256, 0, s>>>(ne, nb, d_inst, d_slipv, d_psi, d_b, d_f0, V0,
                                     (unsigned long long*)d_acc);
   }
