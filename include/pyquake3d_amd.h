@@ -304,8 +304,9 @@ int pq3d_hmat_vpanel_pack(const int64_t* d_tab, int64_t nblocks, const void* d_p
  * of a step for 16 simulations sharing the operator). FP32 storage, tiles
  * of at least 4 rows.
  * Weights d_w: (2 n + nt + 1) rows of 16 floats: the inputs rounded to FP32
- * (written by the call), t (written by phase 1) and a last row of zeros
- * (the caller keeps it zero).
+ * (written by the call from d_v1, d_v2, or, with both NULL, already written
+ * by the caller, e.g. with pq3d_deficit_w32), t (written by phase 1) and a
+ * last row of zeros (the caller keeps it zero).
  * Phase 1 (t = V^T x): d_punits, npunits x 8 int64 units over the V panels
  * d_vpanels (pq3d_hmat_vpanel_pack): (chunk element of the unit's first
  * column, ranks kc <= 64, output row, first stacked column, columns, the
@@ -347,6 +348,13 @@ int pq3d_deficit(int64_t n, const double* d_slipv1, const double* d_slipv2,
 int pq3d_deficit_interleave(int64_t n, int64_t nb, const double* d_slipv1,
                             const double* d_slipv2, const double* d_pl1, const double* d_pl2,
                             double* d_x1, double* d_x2, void* stream);
+
+/* The same interleaved deficit written straight into the FP32 weight rows of
+ * pq3d_hmat_matvec_mfs: d_w[c*nb + b] = x1 and d_w[n*nb + c*nb + b] = x2 of
+ * cell c, instance b, each the FP64 difference rounded to FP32 (what the
+ * product's own input conversion would give). */
+int pq3d_deficit_w32(int64_t n, int64_t nb, const double* d_slipv1, const double* d_slipv2,
+                     const double* d_pl1, const double* d_pl2, float* d_w, void* stream);
 
 /* dsigma/dt = K part + normal loading (src/QDsim.py:911-913); fix_tn: loading only. */
 int pq3d_add_loading(int64_t n, double* d_dsig, const double* d_load, int fix_tn,

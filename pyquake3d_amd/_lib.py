@@ -121,6 +121,8 @@ SIGNATURES = {
                              c_void_p, c_void_p]),
     "pq3d_deficit_interleave": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pq3d_deficit_w32": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p]),
     "pq3d_qd_rhs": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_double, c_double, c_double, c_void_p, c_void_p]),

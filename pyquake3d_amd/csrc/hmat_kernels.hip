@@ -1925,7 +1925,9 @@ int pq3d_hmat_matvec_mfs(const int64_t* d_punits, int64_t npunits, const float* 
   PQ3D_REQUIRE(waves == 1 || waves == 2 || waves == 4, "waves must be 1, 2 or 4");
   PQ3D_REQUIRE(p1_depth == 2 || p1_depth == 4, "p1_depth must be 2 or 4");
   PQ3D_REQUIRE(p1_waves == 2 || p1_waves == 4, "p1_waves must be 2 or 4");
-  PQ3D_REQUIRE(d_w && d_v1 && d_v2 && d_y1 && d_y2, "null pointer");
+  PQ3D_REQUIRE(d_w && d_y1 && d_y2, "null pointer");
+  PQ3D_REQUIRE((d_v1 == nullptr) == (d_v2 == nullptr),
+               "d_v1 and d_v2 must both be given or both be NULL");
   PQ3D_REQUIRE(ntiles == 0 || (d_steps && d_wptr && d_tiles && d_pool), "null pointer");
   PQ3D_REQUIRE(npunits == 0 || (d_punits && d_vpanels), "null pointer");
   PQ3D_REQUIRE(nred == 0 || d_part, "null pointer");
@@ -1943,7 +1945,8 @@ int pq3d_hmat_matvec_mfs(const int64_t* d_punits, int64_t npunits, const float* 
   float* x2 = d_w + m;
   float* t32 = d_w + 2 * m;
   const int64_t zrow = 2 * n + nt;
-  if (m > 0) k_inputs_f32<<<blocks_for(m, 256), 256, 0, s>>>(m, d_v1, d_v2, x1, x2);
+  // NULL inputs: the caller has written the rounded inputs into d_w already
+  if (m > 0 && d_v1) k_inputs_f32<<<blocks_for(m, 256), 256, 0, s>>>(m, d_v1, d_v2, x1, x2);
   if (npunits > 0) {
 #define PQ3D_MFP(DV, WV)                                                                    \
   if (p1_depth == DV && p1_waves == WV)                                                     \

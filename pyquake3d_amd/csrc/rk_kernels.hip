@@ -294,6 +294,20 @@ __global__ void __launch_bounds__(256) k_deficit_interleave(
   x2[o] = slipv2[i] - pl2[i];
 }
 
+// The same deficit written straight into the FP32 weight rows of the
+// matrix-core product: w[c*nb + b] = x1 of cell c, instance b, and
+// w[n*nb + c*nb + b] = x2 (the FP64 differences rounded once, as the
+// product's own input conversion would round them).
+__global__ void __launch_bounds__(256) k_deficit_w32(
+    int64_t n, int64_t nb, const double* __restrict__ slipv1, const double* __restrict__ slipv2,
+    const double* __restrict__ pl1, const double* __restrict__ pl2, float* __restrict__ w) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n * nb) return;
+  const int64_t c = o / nb, b = o - c * nb, i = b * n + c;
+  w[o] = (float)(slipv1[i] - pl1[i]);
+  w[n * nb + o] = (float)(slipv2[i] - pl2[i]);
+}
+
 // dsigma/dt = K part + normal loading (src/QDsim.py:911-913), or the loading
 // alone when the normal traction is fixed.
 __global__ void __launch_bounds__(256) k_add_loading(int64_t n, double* __restrict__ dsig,
@@ -526,6 +540,19 @@ int pq3d_deficit_interleave(int64_t n, int64_t nb, const double* d_slipv1,
   PQ3D_REQUIRE(d_slipv1 && d_slipv2 && d_pl1 && d_pl2 && d_x1 && d_x2, "null pointer");
   k_deficit_interleave<<<blocks_for(n * nb, 256), 256, 0, as_stream(stream)>>>(
       n, nb, d_slipv1, d_slipv2, d_pl1, d_pl2, d_x1, d_x2);
+  PQ3D_LAUNCH_CHECK();
+  return PQ3D_OK;
+}
+
+int pq3d_deficit_w32(int64_t n, int64_t nb, const double* d_slipv1, const double* d_slipv2,
+                     const double* d_pl1, const double* d_pl2, float* d_w, void* stream) {
+  clear_error();
+  PQ3D_REQUIRE(n >= 0 && nb >= 1, "bad sizes");
+  if (n == 0) return PQ3D_OK;
+  PQ3D_REQUIRE(d_slipv1 && d_slipv2 && d_pl1 && d_pl2 && d_w, "null pointer");
+  PQ3D_REQUIRE(n * nb < (1LL << 40), "too many cells");
+  k_deficit_w32<<<blocks_for(n * nb, 256), 256, 0, as_stream(stream)>>>(
+      n, nb, d_slipv1, d_slipv2, d_pl1, d_pl2, d_w);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }

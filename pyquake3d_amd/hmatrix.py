@@ -688,11 +688,18 @@ class HMatrix:
             raise ValueError("planes must be 2 or 3")
         if out is None:
             out = torch.empty((3, n * nrhs), dtype=torch.float64, device=self.device)
-        if not (v1.dtype == torch.float64 and v2.dtype == torch.float64):
+        prefilled = v1 is None and v2 is None
+        if prefilled:
+            # the rounded inputs are already in the matrix-core weights
+            # (pq3d_deficit_w32 into weights())
+            if not (self.accumulate == "mfma" and nrhs == 16 and self.esize == 4 and
+                    isinstance(self, HMatrix)):
+                raise ValueError("v1 = v2 = None needs the 16-RHS matrix-core product")
+        elif not (v1.dtype == torch.float64 and v2.dtype == torch.float64):
             raise ValueError("v1, v2 must be float64")
-        if not (v1.is_contiguous() and v2.is_contiguous()):
+        elif not (v1.is_contiguous() and v2.is_contiguous()):
             raise ValueError("v1, v2 must be contiguous")
-        if v1.numel() != n * nrhs or v2.numel() != n * nrhs:
+        elif v1.numel() != n * nrhs or v2.numel() != n * nrhs:
             raise ValueError("v1, v2 must hold n*nrhs values")
         if tuple(out.shape) != (3, n * nrhs) or not out.is_contiguous() or out.dtype != torch.float64:
             raise ValueError("out must be a contiguous (3, n*nrhs) float64 tensor")
@@ -726,6 +733,13 @@ class HMatrix:
         else:
             _lib.call("pq3d_hmat_matvec", *args, _lib.stream_handle(stream))
         return out
+
+    def weights(self) -> torch.Tensor:
+        """The FP32 weight array of the 16-RHS matrix-core product (rows of
+        16 floats: x1, x2, t, one zero row). Writing x1 and x2 there (e.g.
+        with pq3d_deficit_w32) and calling ``matvec(None, None, nrhs=16)``
+        skips the product's own input conversion."""
+        return self._x32(16)
 
     def _x32(self, nrhs):
         """FP32 weights of the matrix-core products: the two inputs rounded

@@ -409,7 +409,16 @@ class EnsembleQDsim:
 
     def compute_K(self):
         s = _lib.stream_handle()
-        if self.nrhs > 1:
+        op = self.op
+        # the 16-RHS matrix-core product takes its inputs rounded to FP32: the
+        # deficit goes straight into its weight rows
+        w32 = (self.nrhs == 16 and getattr(op, "accumulate", None) == "mfma" and
+               getattr(op, "esize", 8) == 4 and hasattr(op, "weights"))
+        if w32:
+            _lib.call("pq3d_deficit_w32", self.n, self.nrhs, _lib.ptr(self.slipv1),
+                      _lib.ptr(self.slipv2), _lib.ptr(self.pl1), _lib.ptr(self.pl2),
+                      _lib.ptr(op.weights()), s)
+        elif self.nrhs > 1:
             _lib.call("pq3d_deficit_interleave", self.n, self.nrhs, _lib.ptr(self.slipv1),
                       _lib.ptr(self.slipv2), _lib.ptr(self.pl1), _lib.ptr(self.pl2),
                       _lib.ptr(self.x[0]), _lib.ptr(self.x[1]), s)
@@ -417,19 +426,18 @@ class EnsembleQDsim:
             _lib.call("pq3d_deficit", self.ne, _lib.ptr(self.slipv1), _lib.ptr(self.slipv2),
                       _lib.ptr(self.pl1), _lib.ptr(self.pl2), _lib.ptr(self.x[0]),
                       _lib.ptr(self.x[1]), s)
+        x1, x2 = (None, None) if w32 else (self.x[0], self.x[1])
         planes = 2 if self.fix_Tn else 3
         self._kcalls = getattr(self, "_kcalls", 0) + 1
         if self.timing and self._kcalls % max(int(getattr(self, "timing_every", 1)), 1) == 0:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs, instance_major=True,
-                           planes=planes)
+            op.matvec(x1, x2, out=self.K, nrhs=self.nrhs, instance_major=True, planes=planes)
             e1.record()
             self.matvec_events.append((e0, e1))
         else:
-            self.op.matvec(self.x[0], self.x[1], out=self.K, nrhs=self.nrhs, instance_major=True,
-                           planes=planes)
+            op.matvec(x1, x2, out=self.K, nrhs=self.nrhs, instance_major=True, planes=planes)
         _lib.call("pq3d_add_loading", self.ne, _lib.ptr(self.K[2]),
                   _lib.ptr(self.normal_loading), int(self.fix_Tn), s)
 
@@ -609,15 +617,20 @@ class DeviceEnsemble(EnsembleQDsim):
     _graph_key = None
 
     def _graph_state(self):
-        return (getattr(self.op, "accumulate", None), self.mlp is not None, self.rounds)
+        # the matrix-core product's configuration selects the step lists the
+        # graph points at (the autotuner frees the ones it does not keep)
+        cfg = getattr(self.op, "tuning", None)
+        return (getattr(self.op, "accumulate", None), self.mlp is not None, self.rounds,
+                None if cfg is None else tuple(sorted(cfg.items())))
 
     def capture(self):
         """Record one step (operator product, attempt rounds, commit/advance:
         device launches on fixed buffers, no host synchronisation) as a HIP
         graph; later ``step()`` calls replay it. Recording runs nothing, so
         the state is unchanged. The operator's scratch buffers are created
-        first; a later change of the operator's accumulation mode (or of
-        ``rounds``) drops the graph."""
+        first; a later change of the operator's accumulation mode, of its
+        matrix-core configuration (``tuning``) or of ``rounds`` drops the
+        graph."""
         if self.timing:
             raise ValueError("capture needs timing off (events are not recorded into the graph)")
         if hasattr(self.op, "_bufs"):

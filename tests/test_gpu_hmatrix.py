@@ -380,3 +380,34 @@ def test_fp32_storage_error_per_block(gpu, pq3d_lib, case):
         worst_gap = max(worst_gap, abs(e32 - e64))
     assert worst_lr < 10 * eps, worst_lr
     assert worst_gap < 1e-6, worst_gap
+
+
+def test_prefilled_weights_equal_fp64_inputs(gpu, pq3d_lib, case):
+    """pq3d_deficit_w32 writes the interleaved deficit of instance-major slip
+    rates into the matrix-core weights as the FP32 rounding of the FP64
+    differences; the product on those prefilled weights (v1 = v2 = None)
+    equals the product on the FP64 deficit bit for bit."""
+    from pyquake3d_amd import _lib
+    mesh, _ = case
+    H = HMatrix.from_mesh(mesh, MU, LAM, leaf=32, eta=2.0, eps=1e-4, dtype=torch.float32,
+                          device=gpu)
+    H.accumulate = "mfma"
+    n, nb = H.n, 16
+    rng = np.random.default_rng(31)
+    sv1 = torch.from_numpy(rng.standard_normal(nb * n) * 1e-9).to(gpu)   # instance-major
+    sv2 = torch.from_numpy(rng.standard_normal(nb * n) * 1e-9).to(gpu)
+    pl1 = torch.from_numpy(rng.standard_normal(nb * n) * 1e-9).to(gpu)
+    pl2 = torch.from_numpy(rng.standard_normal(nb * n) * 1e-9).to(gpu)
+    x1 = (sv1 - pl1).view(nb, n).t().contiguous().view(-1)      # interleaved FP64 deficit
+    x2 = (sv2 - pl2).view(nb, n).t().contiguous().view(-1)
+    ref = H.matvec(x1, x2, nrhs=nb, instance_major=True, planes=2).clone()
+    W = H.weights()
+    W[:2 * n * nb].fill_(float("nan"))
+    _lib.call("pq3d_deficit_w32", n, nb, _lib.ptr(sv1), _lib.ptr(sv2), _lib.ptr(pl1),
+              _lib.ptr(pl2), _lib.ptr(W), _lib.stream_handle())
+    assert torch.equal(W[:n * nb], x1.float()) and torch.equal(W[n * nb:2 * n * nb], x2.float())
+    got = H.matvec(None, None, nrhs=nb, instance_major=True, planes=2)
+    assert torch.equal(got[:2], ref[:2])
+    with pytest.raises(ValueError):
+        H.accumulate = "f64"
+        H.matvec(None, None, nrhs=nb)

@@ -329,11 +329,13 @@ def test_restore_and_vtk_restart(gpu, pq3d_lib, small, tmp_path):
     assert np.abs(np.log(sc["slipv"] / sa["slipv"])).max() <= 1e-2
 
 
-def test_device_ensemble_graph_replay_matches_eager(gpu, pq3d_lib):
+@pytest.mark.parametrize("accumulate", ["f64", "mfma"])
+def test_device_ensemble_graph_replay_matches_eager(gpu, pq3d_lib, accumulate):
     """A recorded step (HIP graph) replayed many times gives, after every
     replay, the state of the same number of eager steps bit for bit, with
     eager torch work and synchronisations between the replays; 16 draws
-    (16-RHS matvec) sharing one FP32 operator."""
+    (16-RHS matvec, FP64 sums or the matrix-core product) sharing one FP32
+    operator. A new matrix-core configuration drops the graph."""
     from dataclasses import replace
 
     from pyquake3d_amd.hmatrix import HMatrix
@@ -345,6 +347,7 @@ def test_device_ensemble_graph_replay_matches_eager(gpu, pq3d_lib):
                      dc=base.dc * rng.uniform(0.8, 1.2)) for _ in range(16)]
     H = HMatrix.from_mesh(base.mesh, base.mu, base.lambda_, leaf=16, eta=2.0, eps=1e-6,
                           dtype=torch.float32, device=gpu)
+    H.accumulate = accumulate
     eager = DeviceEnsemble(probs, H, device=gpu, rounds=4)
     graph = DeviceEnsemble(probs, H, device=gpu, rounds=4)
     fields = ("Tno", "Tt1o", "Tt2o", "state", "slipv", "slip1", "tdev", "hcur", "nacc", "flag")
@@ -364,3 +367,11 @@ def test_device_ensemble_graph_replay_matches_eager(gpu, pq3d_lib):
     eager.sync()
     assert graph.iterations == eager.iterations == 43
     assert np.array_equal(graph.naccepted, eager.naccepted) and graph.naccepted.min() > 0
+    if accumulate == "mfma":
+        H.tuning = {"depth": 2, "waves": 1, "order": "cost", "p1_depth": 2, "p1_waves": 2}
+        for _ in range(3):
+            eager.step()
+            graph.step()
+        assert graph._graph is None
+        for f in fields:
+            assert torch.equal(getattr(eager, f), getattr(graph, f)), ("retuned", f)
