@@ -45,7 +45,8 @@ ACCUMULATE = ("f64", "mfma", "mfma_comp")
 # configuration used before tuning
 MFS_DEPTHS = (2, 4)           # 4-column steps in flight per wavefront
 MFS_WAVES = (1, 2, 4)         # wavefronts per row tile
-MFS_ORDERS = ("row", "cost")  # tile launch order: leaf (row) order, or XCD-balanced by cost
+MFS_ORDERS = ("row", "cost", "rowbal")  # tile launch order: leaf (row) order, XCD-balanced
+                                        # by cost, or leaf order in cost-balanced XCD ranges
 MFP_DEPTHS = (2, 4)           # phase 1: steps in flight per wavefront
 MFP_WAVES = (2, 4)            # phase 1: wavefronts per unit
 MFS_DEFAULT = {"depth": 4, "waves": 1, "order": "row", "p1_depth": 4, "p1_waves": 4}
@@ -239,7 +240,9 @@ def mfs_steps(items, tile_ptr, tiles, n, nt, waves=1, order="row"):
     rows [2n, 2n + nt) of the FP32 weight array. A tile's steps keep the
     order of its items; ``waves`` contiguous ranges of (nearly) equal length
     per tile go to its wavefronts. ``order``: "row" launches the tiles in
-    leaf (row) order, "cost" keeps the work lists' XCD-balanced order."""
+    leaf (row) order, "cost" keeps the work lists' XCD-balanced order,
+    "rowbal" gives each XCD one run of leaf-order tiles of equal cost
+    (tiles padded with empty ones, no rows and no steps, to 8 equal runs)."""
     it = np.asarray(items, dtype=np.int64).reshape(-1, 6)
     tp = np.asarray(tile_ptr, dtype=np.int64)
     T = np.asarray(tiles, dtype=np.int64).reshape(-1, 2)
@@ -257,6 +260,37 @@ def mfs_steps(items, tile_ptr, tiles, n, nt, waves=1, order="row"):
         item_tile = rank[item_tile]
         io = np.argsort(item_tile, kind="stable")
         it, item_tile = it[io], item_tile[io]
+    elif order == "rowbal":
+        # leaf order, cut into eight ranges of equal cost (columns), one per
+        # XCD; launch slot b runs on XCD b mod 8 and the kernel reads tile
+        # xcd_order(b), so range x is laid out at the xcd_order positions of
+        # slots x, x + 8, ...; ranges shorter than the longest are padded
+        # with empty tiles (no rows, no steps)
+        o = np.argsort(T[:, 0], kind="stable")
+        rank = np.empty(ntiles, dtype=np.int64)
+        rank[o] = np.arange(ntiles)
+        cost = np.bincount(rank[item_tile], weights=it[:, 4], minlength=ntiles)
+        cuts = np.searchsorted(np.cumsum(cost), np.arange(1, 8) * cost.sum() / 8.0)
+        bounds = np.concatenate([[0], cuts, [ntiles]])
+        per = int(np.diff(bounds).max()) if ntiles else 0
+        g = 8 * per
+        q, r = divmod(g, 8)
+        slot_pos = np.empty(g, dtype=np.int64)      # xcd_order(b) for every slot b
+        b = np.arange(g)
+        x, i = b % 8, b // 8
+        slot_pos[:] = np.where(x < r, x * (q + 1) + i, r * (q + 1) + (x - r) * q + i)
+        tile_at = np.full(g, -1, dtype=np.int64)    # position -> row-order tile (or -1: empty)
+        for xx in range(8):
+            rng = np.arange(bounds[xx], bounds[xx + 1])
+            tile_at[slot_pos[xx + 8 * np.arange(len(rng))]] = rng
+        Tn = np.zeros((g, 2), dtype=np.int64)
+        Tn[tile_at >= 0] = T[o][tile_at[tile_at >= 0]]
+        newpos = np.empty(ntiles, dtype=np.int64)
+        newpos[tile_at[tile_at >= 0]] = np.where(tile_at >= 0)[0]
+        item_tile = newpos[rank[item_tile]]
+        io = np.argsort(item_tile, kind="stable")
+        it, item_tile = it[io], item_tile[io]
+        T, ntiles = Tn, g
     elif order != "cost":
         raise ValueError(f"order must be one of {MFS_ORDERS}")
     kind, base, ld, plane, cnt, xoff = (it[:, i] for i in range(6))
@@ -721,7 +755,7 @@ class HMatrix:
             cfg = self._tuning()
             _lib.call("pq3d_hmat_matvec_mfs", _lib.ptr(vp["units"]), vp["nunits"],
                       _lib.ptr(vp["vp"]), _lib.ptr(vp["red"]), vp["nred"], _lib.ptr(steps),
-                      _lib.ptr(wptr), _lib.ptr(tl), self.ntiles, _lib.ptr(self.pool),
+                      _lib.ptr(wptr), _lib.ptr(tl), int(tl.shape[0]), _lib.ptr(self.pool),
                       _lib.ptr(v1), _lib.ptr(v2), _lib.ptr(self._x32(nrhs)), n, self.nt,
                       _lib.ptr(vp["part"]), _lib.ptr(out[0]), _lib.ptr(out[1]), y3,
                       n if (instance_major and nrhs > 1) else 0, int(cfg["p1_depth"]),

@@ -189,7 +189,7 @@ def test_step_stream_configurations_and_autotuner(gpu, pq3d_lib, case, planes, t
     y64 = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()
     H.accumulate = "mfma"
     cfgs = [dict(MFS_DEFAULT, order=o, waves=w, depth=d)
-            for o in ("row", "cost") for w in (1, 2, 4) for d in (2, 4)]
+            for o in ("row", "cost", "rowbal") for w in (1, 2, 4) for d in (2, 4)]
     cfgs += [dict(MFS_DEFAULT, p1_depth=d, p1_waves=w) for d in (2, 4) for w in (2, 4)]
     for cfg in cfgs:
         H.tuning = cfg
@@ -200,7 +200,7 @@ def test_step_stream_configurations_and_autotuner(gpu, pq3d_lib, case, planes, t
             assert _relerr(y[:, :, b], y64[:, :, b]) < 1e-5, (cfg, b)
     H.tuning = None
     rep = H.tune(planes=planes, reps=3)
-    assert len(rep["candidates"]) == 11
+    assert len(rep["candidates"]) == 15
     assert rep["choice"] == min(rep["candidates"], key=lambda c: c[1])[0]
     assert H.tuning == rep["choice"] and H.accumulate == "mfma"
     y = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()

@@ -65,7 +65,7 @@ def _product_steps(steps, wptr, tiles, waves, pool, W, zrow, NP=2):
     return y
 
 
-@pytest.mark.parametrize("order", ["row", "cost"])
+@pytest.mark.parametrize("order", ["row", "cost", "rowbal"])
 @pytest.mark.parametrize("waves", [1, 2, 4])
 def test_step_stream_equals_item_product(order, waves):
     rng = np.random.default_rng(5)
@@ -81,12 +81,20 @@ def test_step_stream_equals_item_product(order, waves):
     ref = _product_items(items_p, ptr_p, tiles_p, pool, W, n)
     steps, wptr, tl = mfs_steps(items_p, ptr_p, tiles_p, n, nt, waves=waves, order=order)
     assert steps.dtype == np.int32 and steps.shape[1] == 4
-    assert len(wptr) == len(tiles) * waves + 1 and wptr[0] == 0 and wptr[-1] == len(steps)
+    assert len(wptr) == len(tl) * waves + 1 and wptr[0] == 0 and wptr[-1] == len(steps)
     assert np.all(np.diff(wptr) >= 0)
     if order == "row":
         assert np.array_equal(tl, tiles)
-    else:
+    elif order == "cost":
         assert np.array_equal(tl, tiles_p)
+    else:
+        # leaf order in eight ranges, padded with empty tiles to a multiple of 8
+        assert len(tl) % 8 == 0 and len(tl) >= len(tiles)
+        real = tl[tl[:, 1] > 0]
+        assert np.array_equal(real[np.argsort(real[:, 0])], tiles)
+        # padding tiles own no steps
+        per_tile = np.diff(wptr[::waves])
+        assert np.all(per_tile[tl[:, 1] == 0] == 0)
     # one step per 4 columns of every item
     assert len(steps) == int(((items[:, 4] + 3) // 4).sum())
     got = _product_steps(steps, wptr, tl, waves, pool, W, 2 * n + nt)
@@ -120,3 +128,28 @@ def test_step_record_limits():
         mfs_steps(bad, ptr, tiles, n, nt)
     with pytest.raises(ValueError):
         mfs_steps(items, ptr[:-1], tiles, n, nt)
+
+
+def _xcd_order(b, g):
+    """The kernels' launch-slot -> work-index map (consecutive slots go to
+    the eight XCDs in turn; XCD x owns one contiguous range of indices)."""
+    q, r = divmod(g, 8)
+    x, i = b % 8, b // 8
+    return x * (q + 1) + i if x < r else r * (q + 1) + (x - r) * q + i
+
+
+def test_rowbal_gives_every_xcd_one_run_of_leaves():
+    rng = np.random.default_rng(9)
+    items, ptr, tiles, n, nt, _ = _random_worklist(rng, ntiles=37)
+    _, _, tl = mfs_steps(items, ptr, tiles, n, nt, waves=1, order="rowbal")
+    g = len(tl)
+    assert sorted(_xcd_order(b, g) for b in range(g)) == list(range(g))
+    start_rank = {int(r0): k for k, r0 in enumerate(tiles[:, 0])}
+    runs = []
+    for x in range(8):
+        ranks = [start_rank[int(tl[_xcd_order(b, g), 0])] for b in range(x, g, 8)
+                 if tl[_xcd_order(b, g), 1] > 0]
+        if ranks:
+            assert ranks == list(range(ranks[0], ranks[0] + len(ranks)))
+        runs.append(ranks)
+    assert sorted(r for run in runs for r in run) == list(range(len(tiles)))
