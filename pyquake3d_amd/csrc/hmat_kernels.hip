@@ -1375,10 +1375,15 @@ __global__ void __launch_bounds__(64 * kW) k_hmv_phase2_mfs(
     };
     MfsStep<NP> buf[D];
     int4 rec[D];
+    // the prologue issues steps 0 .. D-1 in the order the loop re-issues
+    // them: the wait counters at the loop header are merged from both
+    // paths, and a reordered prologue made the header wait for every load
+    // in flight instead of step 0's
 #pragma unroll
     for (int j = 0; j < D; ++j) {
       load(record(sb + j), sb + j < se, buf[j]);
       rec[j] = record(sb + D + j);
+      __builtin_amdgcn_sched_barrier(0);
     }
     for (int64_t s = sb; s < se; s += D) {
 #pragma unroll
