@@ -51,6 +51,7 @@ MFP_DEPTHS = (2, 4)           # phase 1: steps in flight per wavefront
 MFP_WAVES = (2, 4)            # phase 1: wavefronts per unit
 MFS_DEFAULT = {"depth": 4, "waves": 1, "order": "row", "p1_depth": 4, "p1_waves": 4}
 VP_UNIT_COLS = 2048           # stacked V columns per phase-1 unit of the V panels
+VP_ALIGN = 32                 # panel bases in floats (128 B: every full 64-rank chunk starts a cache line)
 STEP_CNT_SHIFT = 28           # step record word 3: (columns - 1) << 28 | weight row
 
 
@@ -321,14 +322,18 @@ def mfs_steps(items, tile_ptr, tiles, n, nt, waves=1, order="row"):
     return np.ascontiguousarray(rec.view(np.int32)), wptr, np.ascontiguousarray(T)
 
 
-def vpanel_plan(c0, nn, lr_idx, ranks, l_vb, unit_cols=VP_UNIT_COLS):
+def vpanel_plan(c0, nn, lr_idx, ranks, l_vb, unit_cols=VP_UNIT_COLS, align=VP_ALIGN):
     """V panels of the matrix-core product (``pq3d_hmat_vpanel_pack``,
     ``pq3d_hmat_matvec_mfs``). The kept low-rank blocks ``lr_idx`` (ranks,
     V^T bases ``l_vb`` in the pool) are grouped by source cluster (c0, n);
     a group's V^T are stored side by side as one 2n x K panel (K = the sum of
     their ranks, blocks in ``lr_idx`` order), chunk-major: ranks
     [64 j, 64 j + kc) of a panel form one contiguous 2n x kc row-major chunk.
-    t rows are numbered panel by panel, so a chunk's t rows are contiguous.
+    Panels start at multiples of ``align`` floats, so with the default 128 B
+    every full chunk (2n x 64 floats after the panel base) starts a cache
+    line and its 4-column steps (1 KiB) do not straddle lines; the padding
+    is never read. t rows are numbered panel by panel, so a chunk's t rows
+    are contiguous.
 
     Returns dict with
       tab    (B, 6): packing rows (V^T base, k, n, panel base, K, first rank)
@@ -339,7 +344,8 @@ def vpanel_plan(c0, nn, lr_idx, ranks, l_vb, unit_cols=VP_UNIT_COLS):
       reduce (R, 4): (t row, kc, first partial row, units) of chunks split
              over several units
       toff   (B,):   first t row of every block in the panel numbering
-      size, nt, npart: panel elements, t rows, partial rows"""
+      size, nt, npart: panel storage in floats (with the padding), t rows,
+             partial rows"""
     lr = np.asarray(lr_idx, dtype=np.int64)
     k = np.asarray(ranks, dtype=np.int64)
     vb = np.asarray(l_vb, dtype=np.int64)
@@ -375,7 +381,7 @@ def vpanel_plan(c0, nn, lr_idx, ranks, l_vb, unit_cols=VP_UNIT_COLS):
             if nc > 1:
                 reds.append((tp + l0, kc, poff, nc))
                 poff += nc * kc
-        pbase += C * K
+        pbase += -(-C * K // align) * align
         tp += K
         i = j
     U = np.array(units, dtype=np.int64).reshape(-1, 8)
