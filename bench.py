@@ -65,10 +65,10 @@ YEAR = 3.15576e7
 # HBM bytes per H-matvec from two rocprofv3 --pmc passes (FETCH_SIZE, then
 # WRITE_SIZE) of the matvec workload, summed over the product's kernels,
 # FETCH scaled by 2.0 and WRITE by 1.0: the factors the same counters read
-# for a 2 GiB device-to-device copy (2 GiB read as 1,048,588 KiB fetched,
+# for a 2 GiB device-to-device copy (2 GiB read as 1,048,608 KiB fetched,
 # 2 GiB written as 2,097,152 KiB). Key: (storage, right-hand sides, row
 # planes, accumulation); N = 100,352 (profiles/hmatvec_traffic_N100352.md).
-MEASURED_TRAFFIC = {("f32", 16, 2, "mfma"): 5_282_207_854}
+MEASURED_TRAFFIC = {("f32", 16, 2, "mfma"): 5_169_650_432}
 
 # FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
 # (unit strike and dip slip, both projected), counted operator by operator
