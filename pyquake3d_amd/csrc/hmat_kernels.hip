@@ -173,6 +173,81 @@ __device__ inline double block_sum(double v, double* sv) {
   return r;
 }
 
+// Ordered reductions of the ACA with their loads issued eight at a time:
+// each helper performs exactly the operations of the plain loop it replaces,
+// in the same order (so results are bitwise those of the plain loop), but
+// the loads of eight terms are in flight together instead of one after
+// another.
+constexpr int kAcaGroup = 8;
+
+// acc - x[0] y[0] - x[xs] y[ys] - ... - x[(k-1) xs] y[(k-1) ys], in order of l
+__device__ __forceinline__ double sub_dots_in_order(double acc, const double* __restrict__ x,
+                                                    int64_t xs, const double* __restrict__ y,
+                                                    int64_t ys, int k) {
+  int l = 0;
+  for (; l + kAcaGroup <= k; l += kAcaGroup) {
+    double a[kAcaGroup], b[kAcaGroup];
+#pragma unroll
+    for (int u = 0; u < kAcaGroup; ++u) {
+      a[u] = x[(int64_t)(l + u) * xs];
+      b[u] = y[(int64_t)(l + u) * ys];
+    }
+#pragma unroll
+    for (int u = 0; u < kAcaGroup; ++u) acc -= a[u] * b[u];
+  }
+  for (; l < k; ++l) acc -= x[(int64_t)l * xs] * y[(int64_t)l * ys];
+  return acc;
+}
+
+// The row residual's pair of updates sharing the U row value:
+// for l < k: ul = x[l xs]; a -= ul y1[l ys]; d -= ul y2[l ys]
+__device__ __forceinline__ void sub_dots2_in_order(double& a, double& d,
+                                                   const double* __restrict__ x, int64_t xs,
+                                                   const double* __restrict__ y1,
+                                                   const double* __restrict__ y2, int64_t ys,
+                                                   int k) {
+  int l = 0;
+  for (; l + kAcaGroup <= k; l += kAcaGroup) {
+    double u[kAcaGroup], p[kAcaGroup], q[kAcaGroup];
+#pragma unroll
+    for (int g = 0; g < kAcaGroup; ++g) {
+      u[g] = x[(int64_t)(l + g) * xs];
+      p[g] = y1[(int64_t)(l + g) * ys];
+      q[g] = y2[(int64_t)(l + g) * ys];
+    }
+#pragma unroll
+    for (int g = 0; g < kAcaGroup; ++g) {
+      a -= u[g] * p[g];
+      d -= u[g] * q[g];
+    }
+  }
+  for (; l < k; ++l) {
+    const double ul = x[(int64_t)l * xs];
+    a -= ul * y1[(int64_t)l * ys];
+    d -= ul * y2[(int64_t)l * ys];
+  }
+}
+
+// acc + x[r0] y[r0] + x[r0 + 64] y[r0 + 64] + ... over r < R, in order of r
+// (one lane's share of a wavefront dot product)
+__device__ __forceinline__ double add_lane_dot_in_order(double acc, const double* __restrict__ x,
+                                                        const double* __restrict__ y, int64_t r0,
+                                                        int64_t R) {
+  int64_t r = r0;
+  for (; r + (kAcaGroup - 1) * 64 < R; r += kAcaGroup * 64) {
+    double a[kAcaGroup], b[kAcaGroup];
+#pragma unroll
+    for (int u = 0; u < kAcaGroup; ++u) {
+      a[u] = x[r + u * 64];
+      b[u] = y[r + u * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < kAcaGroup; ++u) acc += a[u] * b[u];
+  }
+  for (; r < R; r += 64) acc += x[r] * y[r];
+  return acc;
+}
+
 // Partially pivoted ACA of the stacked 3m x 2n block, one workgroup per block.
 // Entries are evaluated in-kernel (TDE pair kernel), never materialised.
 // Workspace: U column l at wU[ub + l*3m], V row l at wV[vb + l*2n] (FP64).
@@ -181,8 +256,13 @@ __device__ inline double block_sum(double v, double* sv) {
 // workspace size). out_rank[b] = rank, or -1 when the cap was reached without
 // convergence (the caller then stores that block densely).
 // Bit masks of used rows/cols live in global scratch (flags, 1 byte each).
+// The full-space kernel is held to three waves per SIMD (the pair
+// evaluation's FP64 chains then overlap with other waves'; some registers
+// spill to scratch: at N = 100,352 the ACA takes 0.44 s against 0.64 s at
+// one wave per SIMD, 0.47 s at two, 0.46 s at four); the half-space kernel
+// would spill heavily and keeps one.
 template <bool HS>
-__global__ void __launch_bounds__(kAcaThreads) k_aca(
+__global__ void __launch_bounds__(kAcaThreads) __attribute__((amdgpu_waves_per_eu(HS ? 1 : 3))) k_aca(
     const int64_t* __restrict__ tab, const int64_t* __restrict__ wsoff,
     const double* __restrict__ recv, const double* __restrict__ rec,
     const double* __restrict__ hrec, double mu, double lam,
@@ -224,11 +304,7 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
       double ts[3], td[3];
       pair_traction<HS>(rr, hrec, c0 + j, q, mu, lam, nu, ts, td);
       double a = ts[cs], d = td[cs];
-      for (int l = 0; l < k; ++l) {
-        const double ul = U[(int64_t)l * R + rstar];
-        a -= ul * V[(int64_t)l * C + j];
-        d -= ul * V[(int64_t)l * C + n + j];
-      }
+      sub_dots2_in_order(a, d, U + rstar, R, V + j, V + n + j, C, k);
       vk[j] = a;
       vk[n + j] = d;
       if (!ucol[j] && fabs(a) > loc.v) { loc.v = fabs(a); loc.i = j; }
@@ -270,10 +346,8 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
         double ts[3], td[3];
         pair_traction<HS>(rr, hrec, c0 + js, qi, mu, lam, nu, ts, td);
         for (int c = 0; c < 3; ++c) {
-          double v = ss ? td[c] : ts[c];
           const int64_t row = c * m + i;
-          for (int l = 0; l < k; ++l) v -= U[(int64_t)l * R + row] * V[(int64_t)l * C + jstar];
-          uk[row] = v;
+          uk[row] = sub_dots_in_order(ss ? td[c] : ts[c], U + row, R, V + jstar, C, k);
         }
       }
     }
@@ -300,9 +374,8 @@ __global__ void __launch_bounds__(kAcaThreads) k_aca(
     for (int l0 = 0; l0 < k; l0 += kAcaThreads) {
       const int lend = (k - l0 < kAcaThreads) ? k : l0 + kAcaThreads;
       for (int l = l0 + wave; l < lend; l += kAcaThreads / 64) {
-        double a = 0.0, c2 = 0.0;
-        for (int64_t r = lane; r < R; r += 64) a += U[(int64_t)l * R + r] * uk[r];
-        for (int64_t c = lane; c < C; c += 64) c2 += V[(int64_t)l * C + c] * vk[c];
+        double a = add_lane_dot_in_order(0.0, U + (int64_t)l * R, uk, lane, R);
+        double c2 = add_lane_dot_in_order(0.0, V + (int64_t)l * C, vk, lane, C);
         a = wave_sum_d(a);
         c2 = wave_sum_d(c2);
         if (lane == 0) sx[l - l0] = a * c2;
@@ -396,11 +469,7 @@ k_acaw_row(const int64_t* __restrict__ tab, const int64_t* __restrict__ wsoff,
   double ts[3], td[3];
   pair_traction<HS>(r, hrec, c0 + j, recv + kRecvW * (r0 + is), mu, lam, nu, ts, td);
   double a = ts[cs], d = td[cs];
-  for (int l = 0; l < k; ++l) {
-    const double ul = U[(int64_t)l * R + rstar];
-    a -= ul * V[(int64_t)l * C + j];
-    d -= ul * V[(int64_t)l * C + n + j];
-  }
+  sub_dots2_in_order(a, d, U + rstar, R, V + j, V + n + j, C, k);
   double* vk = V + (int64_t)k * C;
   vk[j] = a;
   vk[n + j] = d;
@@ -494,10 +563,8 @@ __global__ void __launch_bounds__(kAwCols) k_acaw_col(
   double ts[3], td[3];
   pair_traction<HS>(rs, hrec, c0 + js, recv + kRecvW * (r0 + i), mu, lam, nu, ts, td);
   for (int c = 0; c < 3; ++c) {
-    double v = ss ? td[c] : ts[c];
     const int64_t row = c * m + i;
-    for (int l = 0; l < k; ++l) v -= U[(int64_t)l * R + row] * V[(int64_t)l * C + jstar];
-    uk[row] = v;
+    uk[row] = sub_dots_in_order(ss ? td[c] : ts[c], U + row, R, V + jstar, C, k);
   }
 }
 
@@ -522,9 +589,8 @@ __global__ void __launch_bounds__(256) k_acaw_cross(const int64_t* __restrict__ 
   const double* V = wV + wsoff[WS_W * bid + WS_V];
   const double* uk = U + (int64_t)k * R;
   const double* vk = V + (int64_t)k * C;
-  double a = 0.0, c2 = 0.0;
-  for (int64_t r = lane; r < R; r += 64) a += U[(int64_t)l * R + r] * uk[r];
-  for (int64_t c = lane; c < C; c += 64) c2 += V[(int64_t)l * C + c] * vk[c];
+  double a = add_lane_dot_in_order(0.0, U + (int64_t)l * R, uk, lane, R);
+  double c2 = add_lane_dot_in_order(0.0, V + (int64_t)l * C, vk, lane, C);
   a = wave_sum_d(a);
   c2 = wave_sum_d(c2);
   if (lane == 0) dst[(int64_t)(1 + kmax) * bid + 1 + l] = a * c2;
