@@ -434,7 +434,13 @@ int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_f
  *   d_samples with d_lstate: one training row per accepted step
  *             (PQ3D_SAMPLE_W floats: the six features of the step's start,
  *             log10(h / dtnext base), 1) at row nacc*n_inst + b while it is
- *             below sample_cap. */
+ *             below sample_cap;
+ *   d_facc with d_b, d_f0 (per cell) and V0 > 0: the feature slots of
+ *             pq3d_features (4 per instance); the accepted instances' slots
+ *             are cleared and recomputed from their new state within the
+ *             commit, so they equal what pq3d_features would give for those
+ *             instances at the start of their next step (the slots of the
+ *             other instances are left unchanged). */
 int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
                       int32_t* d_nrj, const double* d_dtnext, double* d_t, int64_t* d_nacc,
                       const double* d_a, const double* d_cellpar, const double* d_yTno,
@@ -443,7 +449,8 @@ int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_
                       double* d_slipv1, double* d_slipv2, double* d_slipv, double* d_rake,
                       double* d_slip1, double* d_slip2, unsigned long long* d_vmax,
                       double* d_run, const double* d_lstate, float* d_samples,
-                      int64_t sample_cap, int64_t n_inst, void* stream);
+                      int64_t sample_cap, int64_t n_inst, const double* d_b,
+                      const double* d_f0, double V0, uint64_t* d_facc, void* stream);
 
 /* Widths of the controller records, in this order: run record
  * (PQ3D_RUN_W = 64), learned-state record (PQ3D_LS_W = 16), training row

@@ -115,7 +115,8 @@ SIGNATURES = {
                          [c_double, c_int64, c_void_p, c_void_p]),
     "pq3d_rk45_advance": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p] + [c_void_p] * 20 +
-                          [c_int64, c_int64, c_void_p]),
+                          [c_int64, c_int64, c_void_p, c_void_p, c_double, c_void_p,
+                           c_void_p]),
     "pq3d_controller_widths": (c_int, [_L, c_int64]),
     "pq3d_deficit": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p]),

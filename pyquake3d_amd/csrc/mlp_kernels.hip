@@ -16,6 +16,7 @@
 #include <cstdint>
 
 #include "pq3d_common.h"
+#include "features_device.h"
 
 namespace pq3d {
 
@@ -143,32 +144,10 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float* __restrict__ prm,
 }
 
 // Per-instance solver-state features over the cells of each instance
-// (element e belongs to instance inst[e]):
-//   {max log10 V, min log10 V, sum log10(V theta/Dc), count(V > 1e-3)}
-// with theta = Dc/V0 exp((psi - f0)/b) (src/QDsim.py:642 steady-state law).
-// Integer accumulation keeps the result independent of the order of the
-// atomics: slots 0/1 hold the largest bit pattern of (1000 + log10 V) and
-// (1000 - log10 V) (positive doubles order like their bit patterns; log10 V is
-// clamped to [-999, 999]), slot 2 the sum of log10(V theta/Dc) in fixed point
-// (2^-32 units), slot 3 the count. Grid-stride over elements, LDS per block,
-// one global atomic per block and slot.
+// (element e belongs to instance inst[e]); slots and their encoding in
+// features_device.h. Contiguous cell ranges per workgroup, LDS slots, one
+// global atomic per touched instance and slot.
 constexpr int kFeatMaxInst = 1024;
-constexpr double kFeatScale = 4294967296.0;  // 2^32
-
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const unsigned long long w = __shfl_xor(v, o, 64);
-    v = w > v ? w : v;
-  }
-  return v;
-}
-
-__device__ __forceinline__ long long wave_sum_i64(long long v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 __global__ void __launch_bounds__(256) k_features(int64_t ne, int nb,
                                                   const int32_t* __restrict__ inst,
@@ -197,39 +176,9 @@ __global__ void __launch_bounds__(256) k_features(int64_t ne, int nb,
     const int64_t e = e0 + lane;
     const bool valid = e < hi;
     const int b = valid ? (inst ? inst[e] : 0) : -1;
-    unsigned long long kmax = 0ull, kmin = 0ull;
-    long long ksum = 0;
-    bool fast = false;
-    if (valid) {
-      const double V = slipv[e];
-      double lv = log10(V);
-      lv = fmax(fmin(lv, 999.0), -999.0);
-      // log10(V theta / Dc) = log10(V/V0) + (psi - f0)/(b ln 10)
-      const double lvt = log10(V / V0) + (psi[e] - pf0[e]) / (pb[e] * 2.302585092994046);
-      kmax = (unsigned long long)__double_as_longlong(1000.0 + lv);
-      kmin = (unsigned long long)__double_as_longlong(1000.0 - lv);
-      ksum = llrint(lvt * kFeatScale);
-      fast = V > 1e-3;
-    }
-    const int b0 = __builtin_amdgcn_readfirstlane(b);
-    if (__all(b == b0)) {
-      const unsigned long long mx = wave_max_u64(kmax), mn = wave_max_u64(kmin);
-      const long long sm = wave_sum_i64(ksum);
-      const unsigned long long ct = __popcll(__ballot(fast));
-      if (lane == 0) {
-        atomicMax(&sh[4 * b0 + 0], mx);
-        atomicMax(&sh[4 * b0 + 1], mn);
-        atomicAdd(&sh[4 * b0 + 2], (unsigned long long)sm);
-        if (ct) atomicAdd(&sh[4 * b0 + 3], ct);
-        touched[b0] = 1;
-      }
-    } else if (valid) {
-      atomicMax(&sh[4 * b + 0], kmax);
-      atomicMax(&sh[4 * b + 1], kmin);
-      atomicAdd(&sh[4 * b + 2], (unsigned long long)ksum);
-      if (fast) atomicAdd(&sh[4 * b + 3], 1ull);
-      touched[b] = 1;
-    }
+    FeatKeys k{0ull, 0ull, 0, false};
+    if (valid) k = feature_keys(slipv[e], psi[e], pb[e], pf0[e], V0);
+    feature_add_wave(b, k, sh, touched);
   }
   __syncthreads();
   for (int q = threadIdx.x; q < 4 * nb; q += blockDim.x) {

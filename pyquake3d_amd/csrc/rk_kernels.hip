@@ -20,6 +20,7 @@
 #include <cstdint>
 
 #include "pq3d_common.h"
+#include "features_device.h"
 
 namespace pq3d {
 
@@ -222,6 +223,15 @@ __global__ void __launch_bounds__(256) k_rk45_attempt(
 
 // Accept a step: state <- candidate, slip rates from the regularized law per
 // component, rake, V floor, slip (src/QDsim.py:960-993). vmax |= max V.
+// With feature slots facc (the learned proposal's, 4 per instance), the
+// committed cells also add the features of their new state
+// (features_device.h), which is what the proposal of the instance's next
+// step reads; the caller clears the accepted instances' slots first
+// (k_clear_accepted). Every workgroup takes one contiguous range of cells
+// (the grid is capped), so it adds to the global slots of one or two
+// instances only.
+constexpr unsigned kCommitMaxGrid = 2048;
+
 __global__ void __launch_bounds__(256) k_rk45_commit(
     int64_t n, const int32_t* __restrict__ inst, const double* __restrict__ hvec,
     const uint8_t* __restrict__ accept, const double* __restrict__ pa,
@@ -231,39 +241,68 @@ __global__ void __launch_bounds__(256) k_rk45_commit(
     double* __restrict__ Tt2o, double* __restrict__ st, double* __restrict__ slipv1,
     double* __restrict__ slipv2, double* __restrict__ slipv, double* __restrict__ rake,
     double* __restrict__ slip1, double* __restrict__ slip2,
-    unsigned long long* __restrict__ vmax, int64_t n_inst) {
+    unsigned long long* __restrict__ vmax, int64_t n_inst, const double* __restrict__ pb,
+    const double* __restrict__ pf0, double V0, unsigned long long* __restrict__ facc) {
   __shared__ unsigned long long s_v[kLdsInst];
+  __shared__ unsigned long long s_f[4 * kLdsInst];
+  __shared__ int touched[kLdsInst];
   const bool lds = n_inst <= kLdsInst;
-  if (lds)
-    for (int t = threadIdx.x; t < n_inst; t += blockDim.x) s_v[t] = 0ull;
-  __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = (i < n && inst) ? inst[i] : 0;
-  if (i < n && !(accept && accept[g] != 1)) {
-    const double h = hvec[g];
-    const double V0 = cellpar[3 * g + 0];
-    const double a = pa[i];
-    const double s = yTno[i], t1 = yTt1o[i], t2 = yTt2o[i], p = yst[i];
-    Tno[i] = s;
-    Tt1o[i] = t1;
-    Tt2o[i] = t2;
-    st[i] = p;
-    const double v1 = (2.0 * V0) * exp(-p / a) * sinh(t1 / s / a);
-    const double v2 = (2.0 * V0) * exp(-p / a) * sinh(t2 / s / a);
-    double v = sqrt(v1 * v1 + v2 * v2);
-    slipv1[i] = v1;
-    slipv2[i] = v2;
-    rake[i] = atan2(t2, t1);
-    if (v < 1e-30) v = 1e-30;
-    slipv[i] = v;
-    slip1[i] = slip1[i] + v1 * h;
-    slip2[i] = slip2[i] + v2 * h;
-    atomicMax(lds ? &s_v[g] : &vmax[g], abs_bits(v));
+  if (lds) {
+    for (int t = threadIdx.x; t < n_inst; t += blockDim.x) {
+      s_v[t] = 0ull;
+      touched[t] = 0;
+    }
+    for (int t = threadIdx.x; t < 4 * n_inst; t += blockDim.x) s_f[t] = 0ull;
   }
   __syncthreads();
-  if (lds)
+  const int lane = threadIdx.x & 63;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  // every wavefront walks 64 consecutive cells at a time (the feature
+  // reduction needs all its lanes at every call)
+  for (int64_t e0 = lo + (threadIdx.x & ~63); e0 < hi; e0 += blockDim.x) {
+    const int64_t i = e0 + lane;
+    const int g = (i < hi && inst) ? inst[i] : 0;
+    const bool commit = i < hi && !(accept && accept[g] != 1);
+    FeatKeys fk{0ull, 0ull, 0, false};
+    if (commit) {
+      const double h = hvec[g];
+      const double V0g = cellpar[3 * g + 0];
+      const double a = pa[i];
+      const double s = yTno[i], t1 = yTt1o[i], t2 = yTt2o[i], p = yst[i];
+      Tno[i] = s;
+      Tt1o[i] = t1;
+      Tt2o[i] = t2;
+      st[i] = p;
+      const double v1 = (2.0 * V0g) * exp(-p / a) * sinh(t1 / s / a);
+      const double v2 = (2.0 * V0g) * exp(-p / a) * sinh(t2 / s / a);
+      double v = sqrt(v1 * v1 + v2 * v2);
+      slipv1[i] = v1;
+      slipv2[i] = v2;
+      rake[i] = atan2(t2, t1);
+      if (v < 1e-30) v = 1e-30;
+      slipv[i] = v;
+      slip1[i] = slip1[i] + v1 * h;
+      slip2[i] = slip2[i] + v2 * h;
+      atomicMax(lds ? &s_v[g] : &vmax[g], abs_bits(v));
+      if (facc) fk = feature_keys(v, p, pb[i], pf0[i], V0);
+    }
+    if (facc) feature_add_wave(commit ? g : -1, fk, lds ? s_f : facc, lds ? touched : nullptr);
+  }
+  __syncthreads();
+  if (lds) {
     for (int t = threadIdx.x; t < n_inst; t += blockDim.x)
       if (s_v[t]) atomicMax(&vmax[t], s_v[t]);
+    if (facc)
+      for (int q = threadIdx.x; q < 4 * n_inst; q += blockDim.x) {
+        if (!touched[q >> 2]) continue;
+        if ((q & 3) < 2)
+          atomicMax(&facc[q], s_f[q]);
+        else if (s_f[q])
+          atomicAdd(&facc[q], s_f[q]);
+      }
+  }
 }
 
 // Slip-rate deficit (src/QDsim.py:902-903): x1 = V1 - Vpl cos(rake0),
@@ -408,11 +447,17 @@ __global__ void k_rk45_control(int64_t n_inst, unsigned long long* __restrict__ 
   }
 }
 
-// Before the commit of the accepted instances: clear their slip-rate maxima.
+// Before the commit of the accepted instances: clear their slip-rate maxima
+// and (facc != null) their feature slots.
 __global__ void k_clear_accepted(int64_t n_inst, const uint8_t* __restrict__ flag,
-                                 unsigned long long* __restrict__ vmax) {
+                                 unsigned long long* __restrict__ vmax,
+                                 unsigned long long* __restrict__ facc) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < n_inst && flag[b] == 1) vmax[b] = 0ull;
+  if (b < n_inst && flag[b] == 1) {
+    vmax[b] = 0ull;
+    if (facc)
+      for (int q = 0; q < 4; ++q) facc[4 * b + q] = 0ull;
+  }
 }
 
 // After the commit: accepted instances advance t += h, take h = dtnext for
@@ -511,10 +556,11 @@ int pq3d_rk45_commit(int64_t n, const int32_t* d_inst, const double* d_h,
   if (n == 0) return PQ3D_OK;
   hipStream_t s = as_stream(stream);
   k_zero_u64<<<blocks_for(n_inst, 256), 256, 0, s>>>(d_vmax, n_inst);
-  k_rk45_commit<<<blocks_for(n, 256), 256, 0, s>>>(
+  const unsigned cgrid = blocks_for(n, 256) < kCommitMaxGrid ? blocks_for(n, 256) : kCommitMaxGrid;
+  k_rk45_commit<<<cgrid, 256, 0, s>>>(
       n, d_inst, d_h, d_accept, d_a, d_cellpar, d_yTno, d_yTt1o, d_yTt2o, d_ystate, d_Tno,
       d_Tt1o, d_Tt2o, d_state, d_slipv1, d_slipv2, d_slipv, d_rake, d_slip1, d_slip2, d_vmax,
-      n_inst);
+      n_inst, nullptr, nullptr, 0.0, nullptr);
   PQ3D_LAUNCH_CHECK();
   return PQ3D_OK;
 }
@@ -627,7 +673,10 @@ int pq3d_rk45_rounds(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_f
 // d_run, per-instance run records (PQ3D_RUN_W doubles: end time, event
 // detection, peak slip rate; flag 3 once the end time is reached);
 // d_lstate with d_samples, one training row (PQ3D_SAMPLE_W floats) per
-// accepted step at row nacc*n_inst + b, kept while row < sample_cap.
+// accepted step at row nacc*n_inst + b, kept while row < sample_cap;
+// d_facc with d_b, d_f0 and V0, the feature slots of pq3d_features (4 per
+// instance): the accepted instances' slots are recomputed from their new
+// state in the same pass (the other instances' slots are left as they are).
 int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_flag,
                       int32_t* d_nrj, const double* d_dtnext, double* d_t, int64_t* d_nacc,
                       const double* d_a, const double* d_cellpar, const double* d_yTno,
@@ -636,17 +685,22 @@ int pq3d_rk45_advance(int64_t n, const int32_t* d_inst, double* d_h, uint8_t* d_
                       double* d_slipv1, double* d_slipv2, double* d_slipv, double* d_rake,
                       double* d_slip1, double* d_slip2, unsigned long long* d_vmax,
                       double* d_run, const double* d_lstate, float* d_samples,
-                      int64_t sample_cap, int64_t n_inst, void* stream) {
+                      int64_t sample_cap, int64_t n_inst, const double* d_b,
+                      const double* d_f0, double V0, uint64_t* d_facc_u, void* stream) {
   clear_error();
   PQ3D_REQUIRE(n >= 0 && n_inst >= 1 && sample_cap >= 0, "bad sizes");
   PQ3D_REQUIRE(!d_samples || d_lstate, "training rows need the learned-state records");
+  PQ3D_REQUIRE(!d_facc_u || (d_b && d_f0 && V0 > 0.0),
+               "feature slots need the b and f0 arrays and V0 > 0");
+  unsigned long long* d_facc = reinterpret_cast<unsigned long long*>(d_facc_u);
   if (n == 0) return PQ3D_OK;
   hipStream_t s = as_stream(stream);
-  k_clear_accepted<<<blocks_for(n_inst, 64), 64, 0, s>>>(n_inst, d_flag, d_vmax);
-  k_rk45_commit<<<blocks_for(n, 256), 256, 0, s>>>(
+  k_clear_accepted<<<blocks_for(n_inst, 64), 64, 0, s>>>(n_inst, d_flag, d_vmax, d_facc);
+  const unsigned cgrid = blocks_for(n, 256) < kCommitMaxGrid ? blocks_for(n, 256) : kCommitMaxGrid;
+  k_rk45_commit<<<cgrid, 256, 0, s>>>(
       n, d_inst, d_h, d_flag, d_a, d_cellpar, d_yTno, d_yTt1o, d_yTt2o, d_ystate, d_Tno,
       d_Tt1o, d_Tt2o, d_state, d_slipv1, d_slipv2, d_slipv, d_rake, d_slip1, d_slip2, d_vmax,
-      n_inst);
+      n_inst, d_b, d_f0, V0, d_facc);
   k_rk45_advance<<<blocks_for(n_inst, 64), 64, 0, s>>>(n_inst, d_flag, d_h, d_dtnext, d_nrj, d_t,
                                                         d_nacc, d_vmax, d_run, d_lstate,
                                                         d_samples, sample_cap);

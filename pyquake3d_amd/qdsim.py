@@ -599,6 +599,10 @@ class DeviceEnsemble(EnsembleQDsim):
             ls[:, 2] = 1.0               # last dtnext / h before the first step
             self.lstate = torch.from_numpy(ls).to(dev)
             self.facc = torch.zeros(4 * nb, dtype=torch.int64, device=dev)
+            # the slots hold the features of every instance's current state:
+            # computed in full once (and after a state load), then kept
+            # current by the commit of the accepted instances
+            self._feat_current = False
             self.ncell = torch.from_numpy(np.diff(self.off).astype(np.int64)).to(dev)
             self.V0 = float(problems[0].V0)
             if len({float(p.V0) for p in problems}) != 1:
@@ -619,9 +623,12 @@ class DeviceEnsemble(EnsembleQDsim):
     def _graph_state(self):
         # the matrix-core product's configuration selects the step lists the
         # graph points at (the autotuner frees the ones it does not keep)
+        # and a graph recorded with current feature slots leaves out their
+        # full reduction, so a state load (slots no longer current) drops it
         cfg = getattr(self.op, "tuning", None)
         return (getattr(self.op, "accumulate", None), self.mlp is not None, self.rounds,
-                None if cfg is None else tuple(sorted(cfg.items())))
+                None if cfg is None else tuple(sorted(cfg.items())),
+                getattr(self, "_feat_current", None))
 
     def capture(self):
         """Record one step (operator product, attempt rounds, commit/advance:
@@ -630,11 +637,14 @@ class DeviceEnsemble(EnsembleQDsim):
         the state is unchanged. The operator's scratch buffers are created
         first; a later change of the operator's accumulation mode, of its
         matrix-core configuration (``tuning``) or of ``rounds`` drops the
-        graph."""
+        graph. With the learned proposal the feature slots are brought up to
+        date first (outside the graph); a later state load drops the graph."""
         if self.timing:
             raise ValueError("capture needs timing off (events are not recorded into the graph)")
         if hasattr(self.op, "_bufs"):
             self.op._bufs(self.nrhs)
+        if self.learned and not self._feat_current:
+            self._features(_lib.stream_handle())
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
@@ -659,9 +669,8 @@ class DeviceEnsemble(EnsembleQDsim):
     def _enqueue_step(self):
         s = _lib.stream_handle()
         if self.learned:
-            _lib.call("pq3d_features", self.ne, self.nb, _lib.ptr(self.inst),
-                      _lib.ptr(self.slipv), _lib.ptr(self.state), _lib.ptr(self.b),
-                      _lib.ptr(self.f0), self.V0, _lib.ptr(self.facc), s)
+            if not self._feat_current:
+                self._features(s)
             _lib.call("pq3d_learned_propose", self.nb, _lib.ptr(self.facc), _lib.ptr(self.ncell),
                       _lib.ptr(self.flag), _lib.ptr(self.nrj), _lib.ptr(self.hcur),
                       _lib.ptr(self.lstate), _lib.ptr(self.mlp.prm) if self.mlp else None,
@@ -686,7 +695,19 @@ class DeviceEnsemble(EnsembleQDsim):
                   _lib.ptr(self.slipv1), _lib.ptr(self.slipv2), _lib.ptr(self.slipv),
                   _lib.ptr(self.rake), _lib.ptr(self.slip1), _lib.ptr(self.slip2),
                   _lib.ptr(self.vmax), _lib.ptr(self.run), _lib.ptr(self.lstate),
-                  _lib.ptr(self.samples_buf), self.record_cap, self.nb, s)
+                  _lib.ptr(self.samples_buf), self.record_cap, self.nb,
+                  _lib.ptr(self.b) if self.learned else None,
+                  _lib.ptr(self.f0) if self.learned else None,
+                  self.V0 if self.learned else 0.0,
+                  _lib.ptr(self.facc) if self.learned else None, s)
+
+    def _features(self, s):
+        """Feature slots of every instance from the current state (one
+        reduction over all cells)."""
+        _lib.call("pq3d_features", self.ne, self.nb, _lib.ptr(self.inst),
+                  _lib.ptr(self.slipv), _lib.ptr(self.state), _lib.ptr(self.b),
+                  _lib.ptr(self.f0), self.V0, _lib.ptr(self.facc), s)
+        self._feat_current = True
 
     DEVICE_FIELDS = ("hcur", "dtnext", "flag", "nrj", "status", "tdev", "nacc", "run")
 
@@ -705,6 +726,8 @@ class DeviceEnsemble(EnsembleQDsim):
                 getattr(self, k).copy_(d[k])
         if self.lstate is not None and "lstate" in d:
             self.lstate.copy_(d["lstate"])
+        if self.learned:
+            self._feat_current = False
         self.sync()
 
     def sync(self):

@@ -191,6 +191,56 @@ def test_device_proposal_matches_host_rule(gpu, hm):
         assert got_h[b] == pytest.approx(want, rel=1e-5)
 
 
+def test_commit_keeps_feature_slots_current(gpu, hm):
+    """The feature slots of the learned proposal are reduced in full only
+    once; afterwards the commit recomputes the accepted instances' slots from
+    their new state. After every iteration the slots of the instances about
+    to start a fresh step (the only ones the proposal reads) equal a full
+    pq3d_features pass bit for bit, a run that reduces in full every step
+    follows the same trajectory bit for bit, and a state load marks the
+    slots stale."""
+    from pyquake3d_amd.qdsim import DeviceEnsemble
+    p, H = hm
+    H = H[torch.float32]
+    draws = [p] + [undulating_c3(nx=40, nz=10, seed=3, friction_seed=k, mesh=p.mesh)
+                   for k in range(1, 4)]
+    rec = DeviceEnsemble(draws, H, device=gpu, rounds=8, record=4 * 300)
+    for _ in range(300):
+        rec.step()
+    rec.sync()
+    X, T = rec.samples()
+    mlp = _trained_mlp(gpu, X, T)
+    # two attempt rounds per iteration: some instances carry a step over
+    fused = DeviceEnsemble(draws, H, device=gpu, rounds=2, mlp=mlp)
+    full = DeviceEnsemble(draws, H, device=gpu, rounds=2, mlp=mlp)
+    tmp = torch.zeros_like(fused.facc)
+    checked = carried = 0
+    for _ in range(120):
+        fused.step()
+        full._feat_current = False          # full reduction at every step
+        full.step()
+        fresh = ((fused.flag == 0) & (fused.nrj == 0)).cpu().numpy()
+        _lib.call("pq3d_features", fused.ne, fused.nb, _lib.ptr(fused.inst),
+                  _lib.ptr(fused.slipv), _lib.ptr(fused.state), _lib.ptr(fused.b),
+                  _lib.ptr(fused.f0), fused.V0, _lib.ptr(tmp), _lib.stream_handle())
+        got = fused.facc.cpu().numpy().reshape(-1, 4)
+        want = tmp.cpu().numpy().reshape(-1, 4)
+        assert np.array_equal(got[fresh], want[fresh])
+        checked += int(fresh.sum())
+        carried += int((~fresh).sum())
+    assert checked > 0 and carried > 0
+    fused.sync()
+    full.sync()
+    assert not fused.aborted().any()
+    for name in ("state", "Tt1o", "slipv", "slip1", "hcur", "tdev", "nacc", "lstate"):
+        assert torch.equal(getattr(fused, name), getattr(full, name)), name
+    d = fused.save_state()
+    fused.load_state(d)
+    assert not fused._feat_current
+    fused.step()
+    assert fused._feat_current
+
+
 def test_learned_loop_on_device(gpu, hm):
     """The learned controller on the device end to end (reduced scale):
     trained on the reference controller's own rows, it takes every step

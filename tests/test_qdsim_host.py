@@ -148,8 +148,10 @@ class DeviceStepChecker(StepChecker):
 
     def pq3d_rk45_advance(self, n, inst, h, flag, nrj, dtn, t, nacc, a, cp, y0, y1, y2, y3, Tno,
                           Tt1o, Tt2o, st, s1, s2, V, rake, sl1, sl2, vmax, run, lstate, samples,
-                          sample_cap, ninst, stream):
+                          sample_cap, ninst, b, f0, V0, facc, stream):
         self._cells(n, a, y0, y1, y2, y3, Tno, Tt1o, Tt2o, st, s1, s2, V, rake, sl1, sl2)
+        # feature slots only with the learned proposal (not run here)
+        assert facc is None and b is None and f0 is None and V0 == 0.0
         for q in (h, flag, nrj, dtn, t, nacc, vmax):
             assert self.T(q).numel() >= ninst
         assert sample_cap >= 0 and (samples is None or lstate is not None)
