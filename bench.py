@@ -265,15 +265,20 @@ def event_errors(fast, ref):
     return out
 
 
-def cpu_baseline_hstep(problem, H, gpu_res, nsteps=3, nproc=8):
+# host cores one GPU's job may use on the box (its worker-pool share)
+CPU_BASELINE_PROCS = (8, 16)
+
+
+def cpu_baseline_hstep(problem, H, gpu_res, nsteps=3, nprocs=CPU_BASELINE_PROCS):
     """Reference-style CPU step on the same blocks, timed over ``nsteps``
     steps from the initial state: the per-block numpy FP64 matvec of the
     four A operators (as the reference applies them with a fixed normal
     traction) and the numpy RK45 step, once on one process (one BLAS
-    thread) and once with the blocks scattered over ``nproc`` processes as
-    the reference's MPI ranks hold them (partial products summed); the
-    rates are expressed for the plain GPU run's simulated interval and
-    accepted steps. ``value`` is the ``nproc``-process rate."""
+    thread) and once for each count in ``nprocs`` with the blocks scattered
+    over that many processes as the reference's MPI ranks hold them
+    (partial products summed); the rates are expressed for the plain GPU
+    run's simulated interval and accepted steps. ``value`` is the rate of
+    the largest process count."""
     import numpy as np
     from threadpoolctl import threadpool_limits
 
@@ -299,13 +304,17 @@ def cpu_baseline_hstep(problem, H, gpu_res, nsteps=3, nproc=8):
             _, dt = ora.simu_forward(dt)
         return (time.perf_counter() - t0) / nsteps
 
+    per = {}
     with threadpool_limits(limits=1):
         one = timed(lambda v1, v2: C.matvec(v1, v2, planes=planes))
-        S = ScatterMatvec(C, nproc)
-        try:
-            many = timed(lambda v1, v2: S.matvec(v1, v2, planes=planes))
-        finally:
-            S.close()
+        for q in nprocs:
+            S = ScatterMatvec(C, q)
+            try:
+                per[q] = timed(lambda v1, v2: S.matvec(v1, v2, planes=planes))
+            finally:
+                S.close()
+    nproc = max(nprocs)
+    many = per[nproc]
     rate = lambda per_step: gpu_res["sim_years"] / (per_step * steps / 3600.0)
     blocks = f"{len(C.dense)} dense + {len(C.lowrank)} low-rank blocks"
     return {"value": rate(many), "unit": "sim-years/wall-hour", "cores": nproc, "kind": "port",
@@ -316,6 +325,7 @@ def cpu_baseline_hstep(problem, H, gpu_res, nsteps=3, nproc=8):
                       f"accepted steps / {gpu_res['sim_years']:.1f} simulated years; host "
                       f"setup {setup:.1f} s",
             "single_process": {"value": rate(one), "cores": 1, "s_per_step": one},
+            "by_processes": {str(q): {"value": rate(t), "s_per_step": t} for q, t in per.items()},
             "s_per_step": many}
 
 
