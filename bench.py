@@ -631,8 +631,10 @@ def bench_learned(args):
     3. Ten seismic cycles at N = 100,352 (the hstep draws, [0, --cycle-years]
        yr), plain reference controller against the learned proposal (RK45
        estimator accepts or rejects every attempt, PI fallback), both on the
-       device controller at the same rtol (1e-4) and rounds: rejected-step
-       fraction, wall time, accepted steps, event onsets of every draw."""
+       device controller at the same rtol (1e-4) and attempt rounds, and the
+       reference again at the single run's rounds (the ratio is taken
+       against its faster run): rejected-step fraction, wall time, accepted
+       steps, event onsets of every draw."""
     import numpy as np
     import torch
 
@@ -682,15 +684,22 @@ def bench_learned(args):
     H, _ = build_hmatrix(draws[0], torch.float32)
     H.accumulate = args.accumulate
     tune_product(H, args, 2)
+    # the reference controller at the ensemble's attempt rounds and at the
+    # single run's (its trajectory is the same, only the iterations differ);
+    # the ratio is taken against the faster of the two
     plain, _ = run_cycles(draws, H, "plain_controller", args.cycle_years, args.warmup,
                           max_wall=args.max_wall)
+    plain8, _ = run_cycles(draws, H, f"plain_controller_r{PLAIN_ROUNDS}", args.cycle_years,
+                           args.warmup, max_wall=args.max_wall, rounds=PLAIN_ROUNDS)
     learned, _ = run_cycles(draws, H, "learned_controller", args.cycle_years, args.warmup,
                             mlp=mlp, max_wall=args.max_wall, margin=LEARNED_MARGIN)
     acc = event_errors(learned, plain)
+    best = min((plain, plain8), key=lambda r: r["wall_s"])
     return {
         "metric": "wall-time ratio over ten seismic cycles, reference / learned step-size "
                   "controller at equal rtol (N=1e5, rejected-step fractions alongside)",
-        "value": plain["wall_s"] / learned["wall_s"], "unit": "x",
+        "value": best["wall_s"] / learned["wall_s"], "unit": "x",
+        "value_same_rounds": plain["wall_s"] / learned["wall_s"],
         "n_gpus": 1, "steps": learned["iterations"], "warmup": args.warmup,
         "ms_per_step": learned["ms_per_iteration"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -700,15 +709,16 @@ def bench_learned(args):
                                f"controller ({args.nb} draws, N=20000, {args.record} "
                                f"iterations), evaluated over [0, {args.cycle_years:g}] yr of "
                                f"{args.nb} draws at N={draws[0].n}, rtol 1e-4, FP32 H-matrix, "
-                               f"device controller, {CYCLE_ROUNDS} attempt rounds per iteration, proposal "
-                               f"margin {LEARNED_MARGIN}"},
+                               f"device controller, {CYCLE_ROUNDS} attempt rounds per iteration "
+                               f"(the reference also at {PLAIN_ROUNDS}; value against its faster "
+                               f"run), proposal margin {LEARNED_MARGIN}"},
         "rejected_fraction": {"reference": plain["rejected_fraction"],
                               "learned": learned["rejected_fraction"]},
         "accepted_steps": {"reference": int(np.sum(plain["accepted_steps"])),
                            "learned": int(np.sum(learned["accepted_steps"]))},
         "sim_years_per_hour": {"reference": plain["years_per_hour"],
                                "learned": learned["years_per_hour"]},
-        "training": fit, "reference": plain, "learned": learned,
+        "training": fit, "reference": plain, "reference_rounds8": plain8, "learned": learned,
         "event_onsets_learned_vs_reference": acc,
     }
 
