@@ -67,7 +67,7 @@ YEAR = 3.15576e7
 # FETCH scaled by 2.0 and WRITE by 1.0: the factors the same counters read
 # for a 2 GiB device-to-device copy (2 GiB read as 1,048,608 KiB fetched,
 # 2 GiB written as 2,097,152 KiB). Key: (storage, right-hand sides, row
-# planes, accumulation); N = 100,352 (profiles/hmatvec_traffic_N100352.md).
+# planes, accumulation); N = 100,352, the tuned configuration 4,2,cost,2,4.
 MEASURED_TRAFFIC = {("f32", 16, 2, "mfma"): 5_169_650_432}
 
 # FP64 operations of one (receiver, source) pair in tde_pair_traction_sd
@@ -83,7 +83,7 @@ FLOPS_PER_PAIR = 1571.0
 # launch (x 64 lanes / 4e8 pairs; an FMA counts 2): full space 3,309 VALU
 # instructions and 3,830 FP64 operations per pair (the correctly rounded
 # divisions and square roots expand into ~10 instructions each), half space
-# 23,919 and 24,526 (profiles/tde_traction_block_N20000.md).
+# 23,919 and 24,526.
 VALU_INSTS_PER_PAIR = {False: 3309.0, True: 23919.0}
 FP64_OPS_PER_PAIR = {False: 3830.0, True: 24526.0}
 # displacement of one (point, triangle) pair, counted per operation in the
