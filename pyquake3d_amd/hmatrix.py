@@ -857,7 +857,7 @@ class HMatrix:
                           torch.from_numpy(tl).to(dev))
         return cache[key]
 
-    def tune(self, planes=2, reps=10, depths=MFS_DEPTHS, waves=(1, 2), orders=MFS_ORDERS,
+    def tune(self, planes=2, reps=10, depths=MFS_DEPTHS, waves=MFS_WAVES, orders=MFS_ORDERS,
              p1_depths=MFP_DEPTHS, p1_waves=MFP_WAVES, seed=0) -> dict:
         """Autotuner of the 16-RHS matrix-core product, timed on this
         operator with HIP events (``reps`` products per candidate after two

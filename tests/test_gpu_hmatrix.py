@@ -200,7 +200,7 @@ def test_step_stream_configurations_and_autotuner(gpu, pq3d_lib, case, planes, t
             assert _relerr(y[:, :, b], y64[:, :, b]) < 1e-5, (cfg, b)
     H.tuning = None
     rep = H.tune(planes=planes, reps=3)
-    assert len(rep["candidates"]) == 15
+    assert len(rep["candidates"]) == 21
     assert rep["choice"] == min(rep["candidates"], key=lambda c: c[1])[0]
     assert H.tuning == rep["choice"] and H.accumulate == "mfma"
     y = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()
