@@ -319,7 +319,7 @@ int pq3d_hmat_vpanel_pack(const int64_t* d_tab, int64_t nblocks, const void* d_p
  * (pool element of the step's first column at its tile's first row, column
  * stride, plane stride, (columns - 1) << 28 | weight row), grouped by tile;
  * d_wptr: ntiles*waves + 1 offsets, the contiguous step range of wavefront g
- * of tile q at [wptr[q*waves + g], wptr[q*waves + g + 1]); depth (2 or 4)
+ * of tile q at [wptr[q*waves + g], wptr[q*waves + g + 1]); depth (2, 4, 6 or 8)
  * steps in flight per wavefront, waves (1, 2 or 4) wavefronts per tile.
  * FP32 sums per wavefront, FP64 across wavefronts; outputs FP64 as in
  * pq3d_hmat_matvec. d_y3 may be NULL (two planes). */

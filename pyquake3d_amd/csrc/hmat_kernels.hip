@@ -1992,7 +1992,8 @@ int pq3d_hmat_matvec_mfs(const int64_t* d_punits, int64_t npunits, const float* 
   PQ3D_REQUIRE(ntiles >= 0 && npunits >= 0 && nred >= 0 && n >= 0 && nt >= 0,
                "sizes must be >= 0");
   PQ3D_REQUIRE(y_bstride >= 0, "y_bstride must be >= 0");
-  PQ3D_REQUIRE(depth == 2 || depth == 4, "depth must be 2 or 4");
+  PQ3D_REQUIRE(depth == 2 || depth == 4 || depth == 6 || depth == 8,
+               "depth must be 2, 4, 6 or 8");
   PQ3D_REQUIRE(waves == 1 || waves == 2 || waves == 4, "waves must be 1, 2 or 4");
   PQ3D_REQUIRE(p1_depth == 2 || p1_depth == 4, "p1_depth must be 2 or 4");
   PQ3D_REQUIRE(p1_waves == 2 || p1_waves == 4, "p1_waves must be 2 or 4");
@@ -2037,7 +2038,9 @@ int pq3d_hmat_matvec_mfs(const int64_t* d_punits, int64_t npunits, const float* 
   }
 #define PQ3D_MFS_ALL(NPV, Y3)                                                              \
   PQ3D_MFS(NPV, 2, 1, Y3) PQ3D_MFS(NPV, 2, 2, Y3) PQ3D_MFS(NPV, 2, 4, Y3)                  \
-  PQ3D_MFS(NPV, 4, 1, Y3) PQ3D_MFS(NPV, 4, 2, Y3) PQ3D_MFS(NPV, 4, 4, Y3)
+  PQ3D_MFS(NPV, 4, 1, Y3) PQ3D_MFS(NPV, 4, 2, Y3) PQ3D_MFS(NPV, 4, 4, Y3)                  \
+  PQ3D_MFS(NPV, 6, 1, Y3) PQ3D_MFS(NPV, 6, 2, Y3) PQ3D_MFS(NPV, 6, 4, Y3)                  \
+  PQ3D_MFS(NPV, 8, 1, Y3) PQ3D_MFS(NPV, 8, 2, Y3) PQ3D_MFS(NPV, 8, 4, Y3)
     if (d_y3) {
       PQ3D_MFS_ALL(3, d_y3)
     } else {

@@ -43,7 +43,7 @@ NRHS = (1, 2, 4, 8, 16)
 ACCUMULATE = ("f64", "mfma", "mfma_comp")
 # matrix-core step stream: candidates of the autotuner (HMatrix.tune) and the
 # configuration used before tuning
-MFS_DEPTHS = (2, 4)           # 4-column steps in flight per wavefront
+MFS_DEPTHS = (2, 4, 6, 8)     # 4-column steps in flight per wavefront
 MFS_WAVES = (1, 2, 4)         # wavefronts per row tile
 MFS_ORDERS = ("row", "cost", "rowbal")  # tile launch order: leaf (row) order, XCD-balanced
                                         # by cost, or leaf order in cost-balanced XCD ranges

@@ -4,7 +4,8 @@ import numpy as np
 import pytest
 import torch
 
-from pyquake3d_amd.hmatrix import MFS_DEFAULT, HMatrix
+from pyquake3d_amd.hmatrix import (MFP_DEPTHS, MFP_WAVES, MFS_DEFAULT, MFS_DEPTHS,
+                                   MFS_ORDERS, MFS_WAVES, HMatrix)
 from pyquake3d_amd.mesh import get_eleVec, undulating_fault
 
 pytestmark = pytest.mark.gpu
@@ -189,7 +190,7 @@ def test_step_stream_configurations_and_autotuner(gpu, pq3d_lib, case, planes, t
     y64 = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()
     H.accumulate = "mfma"
     cfgs = [dict(MFS_DEFAULT, order=o, waves=w, depth=d)
-            for o in ("row", "cost", "rowbal") for w in (1, 2, 4) for d in (2, 4)]
+            for o in MFS_ORDERS for w in MFS_WAVES for d in MFS_DEPTHS]
     cfgs += [dict(MFS_DEFAULT, p1_depth=d, p1_waves=w) for d in (2, 4) for w in (2, 4)]
     for cfg in cfgs:
         H.tuning = cfg
@@ -200,7 +201,8 @@ def test_step_stream_configurations_and_autotuner(gpu, pq3d_lib, case, planes, t
             assert _relerr(y[:, :, b], y64[:, :, b]) < 1e-5, (cfg, b)
     H.tuning = None
     rep = H.tune(planes=planes, reps=3)
-    assert len(rep["candidates"]) == 21
+    assert len(rep["candidates"]) == (len(MFS_DEPTHS) * len(MFS_WAVES) * len(MFS_ORDERS)
+                                      + len(MFP_DEPTHS) * len(MFP_WAVES) - 1)
     assert rep["choice"] == min(rep["candidates"], key=lambda c: c[1])[0]
     assert H.tuning == rep["choice"] and H.accumulate == "mfma"
     y = H.matvec(X1, X2, nrhs=16, planes=planes).view(3, n, 16)[:planes].cpu().numpy()
