@@ -619,6 +619,7 @@ class DeviceEnsemble(EnsembleQDsim):
 
     _graph = None
     _graph_key = None
+    _graph_steps = 1
 
     def _graph_state(self):
         # the matrix-core product's configuration selects the step lists the
@@ -630,17 +631,25 @@ class DeviceEnsemble(EnsembleQDsim):
                 None if cfg is None else tuple(sorted(cfg.items())),
                 getattr(self, "_feat_current", None))
 
-    def capture(self):
-        """Record one step (operator product, attempt rounds, commit/advance:
-        device launches on fixed buffers, no host synchronisation) as a HIP
-        graph; later ``step()`` calls replay it. Recording runs nothing, so
-        the state is unchanged. The operator's scratch buffers are created
-        first; a later change of the operator's accumulation mode, of its
-        matrix-core configuration (``tuning``) or of ``rounds`` drops the
-        graph. With the learned proposal the feature slots are brought up to
-        date first (outside the graph); a later state load drops the graph."""
+    def capture(self, steps=1):
+        """Record ``steps`` consecutive iterations (operator product, attempt
+        rounds, commit/advance: device launches on fixed buffers, no host
+        synchronisation) as one HIP graph; every later ``step()`` replays it
+        and advances ``steps`` iterations, so the host launches once per
+        ``steps`` iterations. The controller runs on the device, so an
+        instance that finishes inside the graph stays finished for the rest
+        of it (its later iterations skip it); the caller reads the finish
+        flags between replays. Recording runs nothing, so the state is
+        unchanged. The operator's scratch buffers are created first; a later
+        change of the operator's accumulation mode, of its matrix-core
+        configuration (``tuning``) or of ``rounds`` drops the graph. With
+        the learned proposal the feature slots are brought up to date first
+        (outside the graph); a later state load drops the graph."""
         if self.timing:
             raise ValueError("capture needs timing off (events are not recorded into the graph)")
+        steps = int(steps)
+        if steps < 1:
+            raise ValueError("steps must be >= 1")
         if hasattr(self.op, "_bufs"):
             self.op._bufs(self.nrhs)
         if self.learned and not self._feat_current:
@@ -648,9 +657,11 @@ class DeviceEnsemble(EnsembleQDsim):
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            self._enqueue_step()
+            for _ in range(steps):
+                self._enqueue_step()
         torch.cuda.synchronize(self.device)
         self._graph = g
+        self._graph_steps = steps
         self._graph_key = self._graph_state()
 
     def release_graph(self):
@@ -658,13 +669,16 @@ class DeviceEnsemble(EnsembleQDsim):
         self._graph_key = None
 
     def step(self):
+        """One iteration, or ``steps`` of them when a graph recorded by
+        ``capture(steps)`` is replayed."""
         if self._graph is not None and self._graph_key != self._graph_state():
             self.release_graph()
         if self._graph is not None:
             self._graph.replay()
+            self.iterations += self._graph_steps
         else:
             self._enqueue_step()
-        self.iterations += 1
+            self.iterations += 1
 
     def _enqueue_step(self):
         s = _lib.stream_handle()

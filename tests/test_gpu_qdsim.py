@@ -330,6 +330,58 @@ def test_restore_and_vtk_restart(gpu, pq3d_lib, small, tmp_path):
 
 
 @pytest.mark.parametrize("accumulate", ["f64", "mfma"])
+def test_device_ensemble_multi_iteration_graph_matches_eager(gpu, pq3d_lib, accumulate):
+    """capture(steps=5): one graph of five iterations, one launch per five
+    iterations; after every replay the state equals five eager iterations
+    bit for bit, also while instances finish inside a replay (their later
+    iterations skip them)."""
+    from dataclasses import replace
+
+    from pyquake3d_amd.hmatrix import HMatrix
+    from pyquake3d_amd.qdsim import DeviceEnsemble
+
+    base = planar_c1(nx=16, nz=8, seed=0)
+    rng = np.random.default_rng(9)
+    probs = [replace(base, a=base.a * (1 + 0.05 * rng.standard_normal(base.n)),
+                     dc=base.dc * rng.uniform(0.8, 1.2)) for _ in range(16)]
+    H = HMatrix.from_mesh(base.mesh, base.mu, base.lambda_, leaf=16, eta=2.0, eps=1e-6,
+                          dtype=torch.float32, device=gpu)
+    H.accumulate = accumulate
+    # end times spread over the run (instance b ends where a probe run of
+    # the same ensemble is after 8 + 6 b iterations), so that instances
+    # finish inside replays
+    probe = DeviceEnsemble(probs, H, device=gpu, rounds=4)
+    times = []
+    for _ in range(100):
+        probe.step()
+        times.append(probe.tdev.clone())
+    t_end = torch.stack([times[7 + 6 * b][b] for b in range(16)])
+    eager = DeviceEnsemble(probs, H, device=gpu, rounds=4)
+    graph = DeviceEnsemble(probs, H, device=gpu, rounds=4)
+    for e in (eager, graph):
+        e.run[:, 0] = t_end
+    fields = ("Tno", "Tt1o", "Tt2o", "state", "slipv", "slip1", "tdev", "hcur", "nacc", "flag")
+    for _ in range(2):
+        eager.step()
+        graph.step()
+    graph.capture(steps=5)
+    with pytest.raises(ValueError):
+        eager.capture(steps=0)
+    for it in range(24):
+        for _ in range(5):
+            eager.step()
+        graph.step()
+        torch.cuda.synchronize()
+        for f in fields:
+            assert torch.equal(getattr(eager, f), getattr(graph, f)), (it, f)
+    graph.sync()
+    eager.sync()
+    assert graph.iterations == eager.iterations == 2 + 24 * 5
+    assert np.array_equal(graph.naccepted, eager.naccepted) and graph.naccepted.min() > 0
+    assert graph.finished().all()
+
+
+@pytest.mark.parametrize("accumulate", ["f64", "mfma"])
 def test_device_ensemble_graph_replay_matches_eager(gpu, pq3d_lib, accumulate):
     """A recorded step (HIP graph) replayed many times gives, after every
     replay, the state of the same number of eager steps bit for bit, with
