@@ -144,6 +144,7 @@ CYCLE_ROUNDS = 4
 # faster with 8 rounds (41,000 iterations / 65.6 s over [0, 380] yr against
 # 43,200 / 68.6 s with 4), so it keeps 8
 PLAIN_ROUNDS = 8
+RAGGED_GRAPH_STEPS = 10   # ragged: iterations per captured HIP graph
 
 
 def c3_draws(nb, nx=448, nz=112, dc_scale=C3_DC_SCALE):
@@ -754,6 +755,7 @@ def bench_ragged(args):
         ens.step()
     ens.sync()
     t_sim0 = ens.time.copy()
+    s0 = ens.save_state()
     ens.timing = True
     ens.matvec_events = []
     w0 = time.perf_counter()
@@ -764,6 +766,25 @@ def bench_ragged(args):
     ens.sync()
     mv = ens.matvec_ms()
     yrs = (ens.time - t_sim0) / YEAR
+    # the same steps again as captured HIP graphs of RAGGED_GRAPH_STEPS
+    # iterations each (one host launch per graph), from the same state: they
+    # must end in the eager run's state bit for bit
+    end = {k: getattr(ens, k).clone() for k in ("tdev", "slipv", "state", "nacc")}
+    ens.timing = False
+    ens.load_state(s0)
+    ens.capture(steps=RAGGED_GRAPH_STEPS)
+    torch.cuda.synchronize()
+    w1 = time.perf_counter()
+    for _ in range(args.steps // RAGGED_GRAPH_STEPS):
+        ens.step()
+    for _ in range(args.steps % RAGGED_GRAPH_STEPS):
+        ens.release_graph()
+        ens.step()
+    torch.cuda.synchronize()
+    wall_graph = time.perf_counter() - w1
+    ens.release_graph()
+    graph_equal = all(torch.equal(getattr(ens, k), v) for k, v in end.items())
+    ens.sync()
 
     def solo(k, H):
         """Instance k alone to the ensemble's accepted-step count; event
@@ -815,6 +836,9 @@ def bench_ragged(args):
                      "frac": achieved / MI355X_HBM_GBS, "traffic": None, "kernel_ms": kern_ms,
                      "algorithmic_bytes": algo},
         "sizes": [int(p.n) for p in probs], "build_s": build_s,
+        "captured_graphs": {"iterations_per_graph": RAGGED_GRAPH_STEPS,
+                            "ms_per_step": 1e3 * wall_graph / args.steps,
+                            "state_equals_eager": graph_equal},
         "accepted_steps_min_max": [int(ens.naccepted.min()), int(ens.naccepted.max())],
         "accuracy_sampled": acc,
     }

@@ -381,6 +381,37 @@ def test_device_ensemble_multi_iteration_graph_matches_eager(gpu, pq3d_lib, accu
     assert graph.finished().all()
 
 
+def test_ragged_ensemble_multi_iteration_graph_matches_eager(gpu, pq3d_lib):
+    """The ragged ensemble (one block-diagonal operator over meshes of
+    uneven size) as captured steps: graphs of four iterations replayed ten
+    times give the state of forty eager iterations bit for bit."""
+    from pyquake3d_amd.hmatrix import HMatrix, MultiHMatrix
+    from pyquake3d_amd.problem import ensemble_c4
+    from pyquake3d_amd.qdsim import DeviceEnsemble
+
+    probs = ensemble_c4(8, seed=2, n_min=100, n_max=600)
+    M = MultiHMatrix([HMatrix.from_mesh(p.mesh, p.mu, p.lambda_, leaf=16, eta=2.0, eps=1e-6,
+                                        dtype=torch.float32, device=gpu) for p in probs])
+    eager = DeviceEnsemble(probs, M, device=gpu, rounds=4)
+    graph = DeviceEnsemble(probs, M, device=gpu, rounds=4)
+    for _ in range(2):
+        eager.step()
+        graph.step()
+    graph.capture(steps=4)
+    fields = ("Tno", "Tt1o", "Tt2o", "state", "slipv", "slip1", "tdev", "hcur", "nacc", "flag")
+    for it in range(10):
+        for _ in range(4):
+            eager.step()
+        graph.step()
+        torch.cuda.synchronize()
+        for f in fields:
+            assert torch.equal(getattr(eager, f), getattr(graph, f)), (it, f)
+    graph.sync()
+    eager.sync()
+    assert graph.iterations == eager.iterations == 42
+    assert np.array_equal(graph.naccepted, eager.naccepted) and graph.naccepted.min() > 0
+
+
 @pytest.mark.parametrize("accumulate", ["f64", "mfma"])
 def test_device_ensemble_graph_replay_matches_eager(gpu, pq3d_lib, accumulate):
     """A recorded step (HIP graph) replayed many times gives, after every
